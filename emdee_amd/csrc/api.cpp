@@ -1,0 +1,741 @@
+// api.cpp -- the EmDee C ABI (include/emdee.h) over the HIP device layer.
+//
+// Each entry point follows the reference procedure cited beside it
+// (src/EmDeeCode.f90): same argument meaning, 1-based indices, same error
+// messages and exit(1) on misuse, same energy/virial bookkeeping in the host
+// struct.  Per-atom state lives on the GPU (device.hip); this file keeps the
+// system description and drives the kernels.
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/emdee.h"
+#include "../../include/emdee_gpu.h"
+#include "engine.h"
+#include "models.h"
+
+using namespace emdee;
+
+namespace {
+
+const char* VERSION = "MI355X 1.0";
+const double EPS = 2.220446049250313e-16;   // epsilon(1.0_rb)
+
+double wtime() {
+  using namespace std::chrono;
+  return duration<double>(steady_clock::now().time_since_epoch()).count();
+}
+
+struct EnergyRec { double Potential, Dispersion, Coulomb, Bond, Angle, Dihedral, ShadowPotential; bool UpToDate; };
+struct VirialRec { double Total, Body; };
+
+// tData (EmDeeData.f90:66-151), host part.
+struct System {
+  int natoms = 0, ntypes = 1, nlayers = 1, layer = 1, nthreads = 1;
+  int nbodies = 0, nfree = 0;
+  double Rc = 0, skin = 0, RcSq = 0, xRc = 0, xRcSq = 0, skinSq = 0, InRc = 0, InRcSq = 0, xInRcSq = 0;
+  double totalMass = 0, startTime = 0;
+  bool hasL = false, hasR = false, initialized = false;
+  double L = 0;
+  std::vector<int> atomType;            // 1-based types, [N]
+  std::vector<int> atomBody;            // reference body ids, [N]
+  std::vector<int> freeList;            // 1-based free atoms
+  std::vector<double> mass, invMass, charge;
+  std::vector<char> charged;
+  Kiss rng;
+  std::vector<std::vector<int>> excluded;   // per atom, sorted, 1-based
+  std::vector<PairEntry> pair;              // pair(ntypes, ntypes, nlayers)
+  std::vector<Model> coul;                  // coul(nlayers)
+  bool kspace_active = false;
+  Model kspace;
+  std::vector<char> multilayer, overridable, interact;
+  std::vector<char> pairs_exist, bonded, useInRc, forcesUpToDate;   // 1-based layers
+  std::vector<EnergyRec> layerEnergy;
+  std::vector<VirialRec> layerVirial;
+  bool multilayer_coulomb = false;
+  int nbonds = 0, nangles = 0;
+  int precision = 0;
+  Device* dev = nullptr;
+  std::vector<double> mirror;                // host buffer for EmDee_memory_address
+
+  PairEntry& P(int i, int j, int l) { return pair[(i - 1) + ntypes * ((j - 1) + ntypes * (l - 1))]; }
+  char& T2(std::vector<char>& a, int i, int j) { return a[(i - 1) + ntypes * (j - 1)]; }
+};
+
+System* S(const tEmDee& md) {
+  if (!md.Data) fatal("EmDee", "invalid system handle");
+  return static_cast<System*>(md.Data);
+}
+
+bool ranged(std::initializer_list<int> v, int imax) {
+  for (int i : v) if (!(i > 0 && i <= imax)) return false;
+  return true;
+}
+
+double phi_fn(double x) {   // math.f90:230-237
+  if (std::fabs(x) > 1e-4) return (1.0 - std::exp(-x)) / x;
+  return 1.0 + 0.5 * x * ((1.0 / 3.0) * x * (1.0 - 0.25 * x) - 1.0);
+}
+
+void require_supported(System* me, const char* task) {
+  if (me->nbodies > 0) fatal(task, "rigid bodies are not available in this build of the engine");
+  if (me->nbonds > 0 || me->nangles > 0) fatal(task, "bonds and angles are not available in this build of the engine");
+  if (me->kspace_active) fatal(task, "kspace solvers are not available in this build of the engine");
+}
+
+void mark_stale(System* me, tEmDee* md) {
+  for (int l = 1; l <= me->nlayers; l++) { me->forcesUpToDate[l] = 0; me->layerEnergy[l - 1].UpToDate = false; }
+  md->Energy.UpToDate = false;
+}
+
+// EmDeeData.f90:194-223
+void check_actual_interactions(System* me) {
+  int nt = me->ntypes, nl = me->nlayers;
+  std::vector<char> neutral(nt + 1);
+  std::vector<char> inter((size_t)nt * nt * nl, 0);
+  for (int i = 1; i <= nt; i++) {
+    int c = 0;
+    for (int a = 0; a < me->natoms; a++) if (me->atomType[a] == i && me->charged[a]) c++;
+    neutral[i] = c == 0;
+  }
+  for (int i = 1; i <= nt; i++)
+    for (int j = 1; j <= i; j++) {
+      bool any = false;
+      for (int k = 1; k <= nl; k++) {
+        const PairEntry& p = me->P(i, j, k);
+        bool no_pair = p.model.kind == PAIR_NONE;
+        bool no_coul = me->coul[k - 1].kind == COUL_NONE || !p.coulomb;
+        bool coul_only = no_pair && !no_coul;
+        bool inert = (no_pair && no_coul) || (coul_only && neutral[i] && neutral[j]);
+        inter[(i - 1) + nt * ((j - 1) + nt * (k - 1))] = !inert;
+        any = any || !inert;
+      }
+      me->T2(me->interact, i, j) = any;
+      me->T2(me->interact, j, i) = any;
+    }
+  for (int k = 1; k <= nl; k++) {
+    bool e = false;
+    for (int i = 1; i <= nt; i++) for (int j = 1; j <= i; j++) e = e || inter[(i - 1) + nt * ((j - 1) + nt * (k - 1))];
+    me->pairs_exist[k] = e;
+  }
+}
+
+void upload_models(System* me) {
+  int nt = me->ntypes;
+  std::vector<uint8_t> inter((size_t)nt * nt);
+  for (int i = 1; i <= nt; i++) for (int j = 1; j <= nt; j++) inter[(i - 1) * nt + (j - 1)] = me->T2(me->interact, i, j);
+  dev_set_interact(me->dev, inter);
+  for (int l = 1; l <= me->nlayers; l++) {
+    std::vector<DevPair> t((size_t)nt * nt);
+    for (int i = 1; i <= nt; i++) for (int j = 1; j <= nt; j++) t[(i - 1) * nt + (j - 1)] = pack_pair(me->P(i, j, l));
+    dev_set_models(me->dev, l - 1, t, pack_coul(me->coul[l - 1]));
+  }
+  std::vector<int> off(me->natoms + 1, 0), items;
+  for (int i = 0; i < me->natoms; i++) {
+    for (int j : me->excluded[i]) items.push_back(j - 1);
+    off[i + 1] = (int)items.size();
+  }
+  dev_set_exclusions(me->dev, off, items);
+  dev_set_charges(me->dev, me->charge);
+}
+
+// EmDeeData.f90:359-416
+void perform_initialization(System* me, tEmDee* md) {
+  const char* task = "system initialization";
+  int bodyDoF = 6 * me->nbodies;
+  md->RotDoF = bodyDoF - 3 * me->nbodies;
+  md->DoF = 3 * me->nfree + bodyDoF - 3;
+  check_actual_interactions(me);
+  bool any_required = false;
+  double kspaceRc = 0;
+  for (int l = 1; l <= me->nlayers; l++)
+    if (me->coul[l - 1].requires_kspace) {
+      double lrc = me->useInRc[l] ? me->InRc : me->Rc;
+      if (!any_required) kspaceRc = lrc;
+      else if (lrc != kspaceRc) fatal(task, "all layers with ewald-like coulomb models must have the same cutoff");
+      any_required = true;
+    }
+  if (any_required != me->kspace_active) {
+    if (me->kspace_active) me->kspace_active = false;
+    else fatal(task, "a kspace solver is required, but has not been defined");
+  }
+  require_supported(me, task);
+  upload_models(me);
+  me->initialized = true;
+}
+
+void switch_layer(System* me, tEmDee* md, int layer) {
+  if (layer != me->layer) {
+    if (layer < 1 || layer > me->nlayers) fatal("model layer switch", "selected layer is out of range");
+    me->layer = layer;
+    std::memcpy(&md->Energy, &me->layerEnergy[layer - 1], sizeof(EnergyRec));
+    md->Virial = me->layerVirial[layer - 1].Total;
+    md->BodyVirial = me->layerVirial[layer - 1].Body;
+  }
+}
+
+void compute_forces(tEmDee* md);
+
+// EmDeeCode.f90:914-923
+void initialize_system(System* me, tEmDee* md) {
+  perform_initialization(me, md);
+  for (int layer = me->nlayers; layer >= 1; layer--) {
+    switch_layer(me, md, layer);
+    compute_forces(md);
+  }
+}
+
+// EmDeeCode.f90:1215-1277
+void compute_forces(tEmDee* md) {
+  System* me = S(*md);
+  if (!me->initialized) fatal("force computation", "box and coordinates have not been defined");
+  double E[5] = { 0, 0, 0, 0, 0 }, W[5] = { 0, 0, 0, 0, 0 };
+  enum { PAIR_ = 0, COUL_ = 1, LONG_ = 2, BOND_ = 3, ANGLE_ = 4 };
+  // handle_neighbor_lists: neighbor_lists.f90:175-195
+  md->Time.Neighbor -= wtime();
+  if (dev_needs_rebuild(me->dev, me->skinSq)) {
+    dev_rebuild(me->dev, me->L, me->xRc, std::sqrt(me->xInRcSq));
+    md->Builds += 1;
+  }
+  md->Time.Neighbor += wtime();
+  md->Time.Pair -= wtime();
+  bool compute = md->Options.Compute;
+  int l = me->layer;
+  PairCall c;
+  c.layer = l - 1;
+  c.ntypes = me->ntypes;
+  c.energy = compute ? 1 : 0;
+  c.mixed = me->precision;
+  c.L = me->L;
+  c.invL2 = 1.0 / (me->L * me->L);
+  c.Rc2 = (me->useInRc[l] ? me->InRcSq : me->RcSq) * c.invL2;
+  ForceSums fs{ 0, 0, 0, 0 };
+  if (me->pairs_exist[l]) {
+    dev_compute_pairs(me->dev, c, &fs);
+  } else {
+    std::vector<double> zero(3 * (size_t)me->natoms, 0.0);
+    dev_upload_forces(me->dev, l - 1, zero.data());
+  }
+  E[PAIR_] = fs.Epair; E[COUL_] = fs.Ecoul; W[PAIR_] = fs.Wpair; W[COUL_] = fs.Wcoul;
+  md->Virial = W[0] + W[1] + W[2] + W[3] + W[4];
+  if (compute) {
+    md->Energy.Dispersion = E[PAIR_];
+    md->Energy.Coulomb = E[COUL_] + E[LONG_];
+    md->Energy.Bond = E[BOND_];
+    md->Energy.Angle = E[ANGLE_];
+    md->Energy.Potential = E[0] + E[1] + E[2] + E[3] + E[4];
+    md->Energy.ShadowPotential = md->Energy.Potential;
+  }
+  md->Energy.UpToDate = compute;
+  me->forcesUpToDate[l] = 1;
+  std::memcpy(&me->layerEnergy[l - 1], &md->Energy, sizeof(EnergyRec));
+  me->layerVirial[l - 1].Total = md->Virial;
+  me->layerVirial[l - 1].Body = md->BodyVirial;
+  double t = wtime();
+  md->Time.Pair += t;
+  md->Time.Total = t - me->startTime;
+}
+
+void set_kinetic(tEmDee* md, const double* twoKEt, const double* twoKEr) {
+  for (int x = 0; x < 3; x++) { md->Kinetic.RotPart[x] = 0.5 * twoKEr[x]; md->Kinetic.TransPart[x] = 0.5 * twoKEt[x]; }
+  md->Kinetic.Rotational = md->Kinetic.RotPart[0] + md->Kinetic.RotPart[1] + md->Kinetic.RotPart[2];
+  md->Kinetic.Kinetic = (md->Kinetic.TransPart[0] + md->Kinetic.TransPart[1] + md->Kinetic.TransPart[2]) + md->Kinetic.Rotational;
+  md->Kinetic.ShadowKinetic = md->Kinetic.Kinetic;
+  md->Kinetic.ShadowRotational = md->Kinetic.Rotational;
+  md->Kinetic.UpToDate = true;
+}
+
+// EmDeeData.f90:268-355 (body bookkeeping; rigid-body dynamics are rejected
+// at initialization by this build)
+void allocate_rigid_bodies(System* me, const int* bodies) {
+  int N = me->natoms;
+  me->atomBody.assign(N, 0);
+  if (bodies) {
+    std::vector<int> index(N + 1, 0), saved(N + 1, 0), amount(N + 1, 0), first(N + 1, 0);
+    int n = 0;
+    for (int i = 1; i <= N; i++) {
+      int ib = bodies[i - 1];
+      if (ib > 0) {
+        int j = 0; bool not_found = true;
+        while (not_found && j < n) { j++; not_found = saved[j] != ib; }
+        if (not_found) { n++; amount[n] = 1; saved[n] = ib; first[n] = i; index[i] = 0; }
+        else { amount[j]++; index[i] = j; index[first[j]] = j; }
+      }
+    }
+    int i = 0;
+    for (int j = 1; j <= n; j++) if (amount[j] > 1) { i++; saved[j] = i; }
+    int nb = 0;
+    for (int k = 1; k <= N; k++) { if (index[k] > 0) index[k] = saved[index[k]]; me->atomBody[k - 1] = index[k]; nb = std::max(nb, index[k]); }
+    me->nbodies = nb;
+    me->freeList.clear();
+    for (int k = 1; k <= N; k++) if (me->atomBody[k - 1] == 0) me->freeList.push_back(k);
+    i = nb;
+    for (int k = 0; k < N; k++) if (me->atomBody[k] == 0) me->atomBody[k] = ++i;
+  } else {
+    me->nbodies = 0;
+    me->freeList.resize(N);
+    for (int k = 0; k < N; k++) { me->freeList[k] = k + 1; me->atomBody[k] = k + 1; }
+  }
+  me->nfree = (int)me->freeList.size();
+}
+
+void add_exclusion(System* me, int i, int j) {
+  std::vector<int>& v = me->excluded[i - 1];
+  auto it = std::lower_bound(v.begin(), v.end(), j);
+  if (it != v.end() && *it == j) return;
+  v.insert(it, j);
+}
+
+}  // namespace
+
+extern "C" {
+
+// EmDeeCode.f90:69-208
+tEmDee EmDee_system(int threads, int layers, double rc, double skin, int N, int* types, double* masses, int* bodies) {
+  if (!getenv("EMDEE_QUIET")) printf("EmDee (version: %s)\n", VERSION);
+  tEmDee md;
+  std::memset(&md, 0, sizeof md);
+  System* me = new System();
+  me->nthreads = threads < 1 ? 1 : threads;
+  me->nlayers = layers;
+  if (layers < 1) fatal("system setup", "invalid number of layers");
+  me->Rc = rc; me->skin = skin; me->RcSq = rc * rc;
+  me->xRc = rc + skin; me->xRcSq = me->xRc * me->xRc;
+  me->InRc = me->Rc; me->InRcSq = me->RcSq; me->xInRcSq = me->xRcSq;
+  me->skinSq = skin * skin;
+  me->natoms = N;
+  me->atomType.assign(N, 1);
+  if (types) {
+    int mn = N ? types[0] : 1, mx = N ? types[0] : 1;
+    for (int i = 0; i < N; i++) { mn = std::min(mn, types[i]); mx = std::max(mx, types[i]); }
+    if (mn != 1) fatal("system setup", "wrong specification of atom types");
+    me->ntypes = mx;
+    for (int i = 0; i < N; i++) me->atomType[i] = types[i];
+  }
+  me->mass.assign(N, 1.0);
+  me->invMass.assign(N, 1.0);
+  if (masses) {
+    double tm = 0;
+    for (int i = 0; i < N; i++) { double m = masses[me->atomType[i] - 1]; me->mass[i] = m; me->invMass[i] = 1.0 / m; tm += m; }
+    me->totalMass = tm;
+  } else {
+    me->totalMass = (double)N;
+  }
+  me->startTime = wtime();
+  me->charge.assign(N, 0.0);
+  me->charged.assign(N, 0);
+  allocate_rigid_bodies(me, bodies);
+  me->excluded.assign(N, std::vector<int>());
+  int nt = me->ntypes;
+  me->pair.assign((size_t)nt * nt * layers, PairEntry());
+  me->multilayer.assign(nt * nt, 0);
+  me->overridable.assign(nt * nt, 1);
+  me->interact.assign(nt * nt, 0);
+  me->coul.assign(layers, Model());
+  for (auto& c : me->coul) c.kind = COUL_NONE;
+  me->pairs_exist.assign(layers + 1, 0);
+  me->useInRc.assign(layers + 1, 0);
+  me->bonded.assign(layers + 1, 1);
+  me->forcesUpToDate.assign(layers + 1, 0);
+  me->layerEnergy.assign(layers, EnergyRec{ 0, 0, 0, 0, 0, 0, 0, false });
+  me->layerVirial.assign(layers, VirialRec{ 0, 0 });
+  const char* prec = getenv("EMDEE_PRECISION");
+  me->precision = (prec && std::string(prec) == "mixed") ? 1 : 0;
+  // device state
+  DeviceSetup ds;
+  ds.N = N; ds.ntypes = nt; ds.nlayers = layers;
+  ds.type.resize(N); ds.body.resize(N); ds.invMass = me->invMass; ds.charge = me->charge; ds.freeFlag.resize(N);
+  for (int i = 0; i < N; i++) { ds.type[i] = me->atomType[i] - 1; ds.body[i] = me->atomBody[i]; ds.freeFlag[i] = 0; }
+  for (int k : me->freeList) ds.freeFlag[k - 1] = 1;
+  me->dev = dev_create(ds);
+  md.Builds = 0;
+  md.DoF = 3 * (N - 1);
+  md.RotDoF = 0;
+  md.Data = me;
+  md.Options.Translate = true;
+  md.Options.Rotate = true;
+  md.Options.RotationMode = 0;
+  md.Options.AutoBodyUpdate = true;
+  md.Options.Compute = true;
+  return md;
+}
+
+// EmDeeCode.f90:212-235.  The engine's arrays live in GPU memory; the
+// address returned is a host snapshot taken at the time of the call.
+void* EmDee_memory_address(tEmDee md, char* option) {
+  System* me = S(md);
+  std::string item(option);
+  me->mirror.assign(3 * (size_t)me->natoms * (item == "layerForces" ? me->nlayers : 1), 0.0);
+  if (item == "coordinates") dev_download_positions(me->dev, me->mirror.data());
+  else if (item == "momenta") dev_download_momenta(me->dev, me->mirror.data());
+  else if (item == "forces") dev_download_forces(me->dev, me->layer - 1, me->mirror.data());
+  else if (item == "layerForces") {
+    for (int l = 0; l < me->nlayers; l++) dev_download_forces(me->dev, l, me->mirror.data() + 3 * (size_t)me->natoms * l);
+  } else fatal("memory address retrieving", "invalid option " + item);
+  return me->mirror.data();
+}
+
+// EmDeeCode.f90:239-269
+void EmDee_share_phase_space(tEmDee mdkeep, tEmDee* mdlose) {
+  (void)mdkeep; (void)mdlose;
+  fatal("phase space sharing", "not available in this build of the engine");
+}
+
+// EmDeeCode.f90:273-305
+void EmDee_layer_based_parameters(tEmDee md, double InternalRc, int* Apply, int* Bonded) {
+  const char* task = "layer-based parameter setting";
+  System* me = S(md);
+  if (me->initialized) fatal(task, "system has already been initialized");
+  bool any = false;
+  for (int l = 1; l <= me->nlayers; l++) { me->useInRc[l] = Apply[l - 1] != 0; any = any || me->useInRc[l]; }
+  if (any && (InternalRc <= 0.0 || InternalRc > me->Rc)) fatal(task, "invalid internal cutoff specification");
+  me->InRc = InternalRc;
+  me->InRcSq = InternalRc * InternalRc;
+  me->xInRcSq = (InternalRc + me->skin) * (InternalRc + me->skin);
+  for (int l = 1; l <= me->nlayers; l++) me->bonded[l] = Bonded[l - 1] != 0;
+  for (int l = 1; l <= me->nlayers; l++) coul_cutoff_setup(me->coul[l - 1], me->useInRc[l] ? me->InRc : me->Rc);
+}
+
+}  // extern "C"
+
+// EmDeeData.f90:227-264
+static void set_pair_type(System* me, int itype, int jtype, int layer, const Model& model, double kCoul) {
+  double layerRc = me->useInRc[layer] ? me->InRc : me->Rc;
+  if (itype == jtype) {
+    PairEntry& p = me->P(itype, itype, layer);
+    p.model = model;
+    p.coulomb = kCoul != 0.0;
+    if (p.coulomb) p.kCoul = kCoul;
+    modifier_setup(p.model, layerRc);
+    for (int k = 1; k <= me->ntypes; k++)
+      if (k != itype && me->T2(me->overridable, itype, k)) {
+        me->P(itype, k, layer) = mix_pairs(me->P(k, k, layer), me->P(itype, itype, layer));
+        modifier_setup(me->P(itype, k, layer).model, layerRc);
+        me->P(k, itype, layer) = me->P(itype, k, layer);
+      }
+  } else {
+    PairEntry& p = me->P(itype, jtype, layer);
+    p.model = model;
+    p.coulomb = kCoul != 0.0;
+    if (p.coulomb) p.kCoul = kCoul;
+    modifier_setup(p.model, layerRc);
+    me->P(jtype, itype, layer) = p;
+  }
+}
+
+extern "C" {
+
+// EmDeeCode.f90:309-359
+void EmDee_set_pair_model(tEmDee md, int itype, int jtype, void* model, double kCoul) {
+  const char* task = "pair model setup";
+  System* me = S(md);
+  if (me->initialized) fatal(task, "cannot set model after coordinates have been defined");
+  if (!ranged({ itype, jtype }, me->ntypes)) fatal(task, "provided type index is out of range");
+  if (!model) fatal(task, "a valid pair model must be provided");
+  const Model& m = *static_cast<Model*>(model);
+  if (!is_pair(m)) fatal(task, "a valid pair model must be provided");
+  for (int l = 1; l <= me->nlayers; l++) set_pair_type(me, itype, jtype, l, m, kCoul);
+  me->T2(me->multilayer, itype, jtype) = 0;
+  me->T2(me->multilayer, jtype, itype) = 0;
+  if (itype == jtype) {
+    for (int k = 1; k <= me->ntypes; k++)
+      if (k != itype && me->T2(me->overridable, itype, k)) {
+        me->T2(me->multilayer, itype, k) = me->T2(me->multilayer, k, k);
+        me->T2(me->multilayer, k, itype) = me->T2(me->multilayer, k, k);
+      }
+  } else {
+    me->T2(me->overridable, itype, jtype) = 0;
+    me->T2(me->overridable, jtype, itype) = 0;
+  }
+}
+
+// EmDeeCode.f90:363-412
+void EmDee_set_pair_multimodel(tEmDee md, int itype, int jtype, void* model[], double kCoul[]) {
+  const char* task = "pair multimodel setup";
+  System* me = S(md);
+  if (me->initialized) fatal(task, "cannot set model after coordinates have been defined");
+  if (!ranged({ itype, jtype }, me->ntypes)) fatal(task, "provided type index is out of range");
+  for (int l = 1; l <= me->nlayers; l++) {
+    if (!model[l - 1]) fatal(task, std::to_string(me->nlayers) + " valid pair models must be provided");
+    const Model& m = *static_cast<Model*>(model[l - 1]);
+    if (!is_pair(m)) fatal(task, "a valid pair model must be provided");
+    set_pair_type(me, itype, jtype, l, m, kCoul[l - 1]);
+  }
+  me->T2(me->multilayer, itype, jtype) = 1;
+  me->T2(me->multilayer, jtype, itype) = 1;
+  if (itype == jtype) {
+    for (int k = 1; k <= me->ntypes; k++)
+      if (k != itype && me->T2(me->overridable, itype, k)) { me->T2(me->multilayer, itype, k) = 1; me->T2(me->multilayer, k, itype) = 1; }
+  } else {
+    me->T2(me->overridable, itype, jtype) = 0;
+    me->T2(me->overridable, jtype, itype) = 0;
+  }
+}
+
+// EmDeeCode.f90:416-444
+void EmDee_set_kspace_model(tEmDee md, void* model) {
+  const char* task = "kspace model setup";
+  System* me = S(md);
+  if (me->initialized) fatal(task, "cannot set model after coordinates have been defined");
+  if (!model) fatal(task, "a valid kspace model must be provided");
+  const Model& m = *static_cast<Model*>(model);
+  if (m.kind != KSPACE_EWALD) fatal(task, "a valid kspace model must be provided");
+  me->kspace = m;
+  me->kspace_active = true;
+}
+
+// EmDeeCode.f90:448-482
+void EmDee_set_coul_model(tEmDee md, void* model) {
+  const char* task = "coulomb model setup";
+  System* me = S(md);
+  if (me->initialized) fatal(task, "cannot set model after coordinates have been defined");
+  if (!model) fatal(task, "a valid coulomb model must be provided");
+  const Model& m = *static_cast<Model*>(model);
+  if (!is_coul(m)) fatal(task, "a valid coulomb model must be provided");
+  for (int l = 1; l <= me->nlayers; l++) {
+    double layerRc = me->useInRc[l] ? me->InRc : me->Rc;
+    me->coul[l - 1] = m;
+    coul_cutoff_setup(me->coul[l - 1], layerRc);
+    modifier_setup(me->coul[l - 1], layerRc);
+  }
+  me->multilayer_coulomb = false;
+}
+
+// EmDeeCode.f90:486-520
+void EmDee_set_coul_multimodel(tEmDee md, void* model[]) {
+  const char* task = "coulomb multimodel setup";
+  System* me = S(md);
+  if (me->initialized) fatal(task, "cannot set model after coordinates have been defined");
+  for (int l = 1; l <= me->nlayers; l++) {
+    double layerRc = me->useInRc[l] ? me->InRc : me->Rc;
+    if (!model[l - 1]) fatal(task, std::to_string(me->nlayers) + " valid coulomb models must be provided");
+    const Model& m = *static_cast<Model*>(model[l - 1]);
+    if (!is_coul(m)) fatal(task, "a valid coulomb model must be provided");
+    me->coul[l - 1] = m;
+    coul_cutoff_setup(me->coul[l - 1], layerRc);
+    modifier_setup(me->coul[l - 1], layerRc);
+  }
+  me->multilayer_coulomb = true;
+}
+
+// EmDeeCode.f90:524-570
+void EmDee_ignore_pair(tEmDee md, int i, int j) {
+  System* me = S(md);
+  if (i != j && ranged({ i, j }, me->natoms)) { add_exclusion(me, i, j); add_exclusion(me, j, i); }
+}
+
+// EmDeeCode.f90:574-597
+void EmDee_add_bond(tEmDee md, int i, int j, void* model) {
+  System* me = S(md);
+  if (!ranged({ i, j }, me->natoms)) fatal("add_bond", "atom index out of range");
+  if (!model) fatal("add_bond", "a valid model must be provided");
+  const Model& m = *static_cast<Model*>(model);
+  if (m.kind != BOND_HARMONIC && m.kind != BOND_NONE) fatal("add_bond", "the provided model must be a bond model");
+  me->nbonds++;
+  EmDee_ignore_pair(md, i, j);
+}
+
+// EmDeeCode.f90:600-626
+void EmDee_add_angle(tEmDee md, int i, int j, int k, void* model) {
+  System* me = S(md);
+  if (!ranged({ i, j, k }, me->natoms)) fatal("add_angle", "atom index out of range");
+  if (!model) fatal("add_angle", "a valid model must be provided");
+  const Model& m = *static_cast<Model*>(model);
+  if (m.kind != ANGLE_HARMONIC && m.kind != ANGLE_NONE) fatal("add_angle", "the provided model must be an angle model");
+  me->nangles++;
+  EmDee_ignore_pair(md, i, j);
+  EmDee_ignore_pair(md, i, k);
+  EmDee_ignore_pair(md, j, k);
+}
+
+// EmDeeCode.f90:628-655 (no dihedral model exists; only exclusions apply)
+void EmDee_add_dihedral(tEmDee md, int i, int j, int k, int l, void* model) {
+  System* me = S(md);
+  if (!ranged({ i, j, k, l }, me->natoms)) fatal("add_dihedral", "atom index out of range");
+  if (!model) fatal("add_dihedral", "a valid model must be provided");
+  const Model& m = *static_cast<Model*>(model);
+  if (m.kind != DIHEDRAL_NONE) fatal("add_dihedral", "the provided model must be a dihedral model");
+  EmDee_ignore_pair(md, i, j); EmDee_ignore_pair(md, i, k); EmDee_ignore_pair(md, i, l);
+  EmDee_ignore_pair(md, j, k); EmDee_ignore_pair(md, j, l); EmDee_ignore_pair(md, k, l);
+}
+
+// EmDeeCode.f90:659-801
+void EmDee_download(tEmDee md, char* option, double* address) {
+  System* me = S(md);
+  std::string item(option);
+  if (!address) fatal("download", "provided address is invalid");
+  if (item == "box") *address = me->L;
+  else if (item == "coordinates") {
+    if (!me->hasR) fatal("download", "coordinates have not been allocated");
+    dev_download_positions(me->dev, address);
+  } else if (item == "momenta") dev_download_momenta(me->dev, address);
+  else if (item == "forces") {
+    if (!me->forcesUpToDate[me->layer]) compute_forces(&md);
+    dev_download_forces(me->dev, me->layer - 1, address);
+  } else if (item == "centersOfMass") {
+    std::vector<double> R(3 * (size_t)me->natoms);
+    dev_download_positions(me->dev, R.data());
+    for (int j = 0; j < me->nfree; j++) for (int x = 0; x < 3; x++) address[3 * (j + me->nbodies) + x] = R[3 * (me->freeList[j] - 1) + x];
+  } else if (item == "quaternions" || item == "quatmom" || item == "quattau" || item == "angmom" || item == "bodycoord" ||
+             item == "bodymom" || item == "bodyforces" || item == "torques" || item == "inertia") {
+    // no rigid bodies exist in systems this build accepts
+  } else fatal("download", "invalid option");
+}
+
+// EmDeeCode.f90:805-925
+void EmDee_upload(tEmDee* md, char* option, double* address) {
+  System* me = S(*md);
+  std::string item(option);
+  if (!address) fatal("upload", "provided address is invalid");
+  if (item == "box") {
+    me->L = *address;
+    me->hasL = true;
+    if (me->initialized) mark_stale(me, md);
+    else if (me->hasR) initialize_system(me, md);
+  } else if (item == "coordinates") {
+    dev_upload_positions(me->dev, address);
+    me->hasR = true;
+    if (me->initialized) mark_stale(me, md);
+    else if (me->hasL) initialize_system(me, md);
+  } else if (item == "momenta") {
+    if (!me->initialized) fatal("upload", "box and coordinates have not been defined");
+    dev_upload_momenta(me->dev, address);
+    double t[3], r[3] = { 0, 0, 0 };
+    // assign_momenta, EmDeeData.f90:157-189: 2KE in free-atom order
+    for (int x = 0; x < 3; x++) t[x] = 0;
+    for (int j : me->freeList)
+      for (int x = 0; x < 3; x++) t[x] += me->invMass[j - 1] * address[3 * (j - 1) + x] * address[3 * (j - 1) + x];
+    set_kinetic(md, t, r);
+  } else if (item == "forces") {
+    if (!me->initialized) fatal("upload", "box and coordinates have not been defined");
+    dev_upload_forces(me->dev, me->layer - 1, address);
+  } else if (item == "charges") {
+    if (me->initialized) fatal("upload", "cannot set charges after box and coordinates initialization");
+    for (int i = 0; i < me->natoms; i++) { me->charge[i] = address[i]; me->charged[i] = std::fabs(address[i]) > EPS; }
+    mark_stale(me, md);
+  } else fatal("upload", "invalid option");
+}
+
+// EmDeeCode.f90:929-946
+void EmDee_switch_model_layer(tEmDee* md, int layer) { switch_layer(S(*md), md, layer); }
+
+// EmDeeCode.f90:950-1020
+void EmDee_random_momenta(tEmDee* md, double kT, bool adjust, int seed) {
+  System* me = S(*md);
+  if (me->rng.seeding_required) me->rng.setup(seed);
+  double twoKEt[3] = { 0, 0, 0 }, twoKEr[3] = { 0, 0, 0 };
+  int N = me->natoms;
+  std::vector<double> P(3 * (size_t)N, 0.0);
+  if (me->nbodies != 0 && !me->initialized) fatal("random_momenta", "coordinates have not defined");
+  for (int i : me->freeList) {
+    double f = std::sqrt(me->mass[i - 1] * kT);
+    for (int x = 0; x < 3; x++) P[3 * (i - 1) + x] = f * me->rng.normal();
+    for (int x = 0; x < 3; x++) twoKEt[x] += me->invMass[i - 1] * P[3 * (i - 1) + x] * P[3 * (i - 1) + x];
+  }
+  if (adjust) {   // adjust_momenta, EmDeeCode.f90:984-1018
+    double vcm[3];
+    for (int x = 0; x < 3; x++) {
+      double s1 = 0;
+      for (int i : me->freeList) s1 += P[3 * (i - 1) + x];
+      double s2 = 0;
+      vcm[x] = (s1 + s2) / me->totalMass;
+    }
+    for (int x = 0; x < 3; x++) twoKEt[x] = 0;
+    for (int i : me->freeList)
+      for (int x = 0; x < 3; x++) {
+        P[3 * (i - 1) + x] = P[3 * (i - 1) + x] - me->mass[i - 1] * vcm[x];
+        twoKEt[x] += me->invMass[i - 1] * P[3 * (i - 1) + x] * P[3 * (i - 1) + x];
+      }
+    int dofsum = 6 * me->nbodies;
+    double tot = (twoKEt[0] + twoKEr[0]) + (twoKEt[1] + twoKEr[1]) + (twoKEt[2] + twoKEr[2]);
+    double factor = std::sqrt((double)(3 * me->nfree + dofsum - 3) * kT / tot);
+    for (int i : me->freeList) for (int x = 0; x < 3; x++) P[3 * (i - 1) + x] *= factor;
+    for (int x = 0; x < 3; x++) { twoKEt[x] = factor * factor * twoKEt[x]; twoKEr[x] = factor * factor * twoKEr[x]; }
+  }
+  dev_upload_momenta(me->dev, P.data());
+  set_kinetic(md, twoKEt, twoKEr);
+}
+
+// EmDeeCode.f90:1024-1065
+void EmDee_boost(tEmDee* md, double lambda, double alpha, double dt) {
+  System* me = S(*md);
+  double CF = phi_fn(alpha * dt) * dt;
+  double CP = 1.0 - alpha * CF;
+  CF = lambda * CF;
+  if (lambda != 0.0 && !me->forcesUpToDate[me->layer]) compute_forces(md);
+  bool compute = md->Options.Compute, translate = md->Options.Translate;
+  double twoKEt[3] = { 0, 0, 0 };
+  dev_boost(me->dev, me->layer - 1, CP, CF, translate, compute ? twoKEt : nullptr);
+  if (compute) {
+    if (translate) for (int x = 0; x < 3; x++) md->Kinetic.TransPart[x] = 0.5 * twoKEt[x];
+    if (md->Options.Rotate) {
+      for (int x = 0; x < 3; x++) md->Kinetic.RotPart[x] = 0.0;
+      md->Kinetic.Rotational = 0.0;
+    }
+    md->Kinetic.Kinetic = (md->Kinetic.TransPart[0] + md->Kinetic.TransPart[1] + md->Kinetic.TransPart[2]) + md->Kinetic.Rotational;
+  }
+  md->Kinetic.UpToDate = compute;
+}
+
+// EmDeeCode.f90:1069-1103
+static void displace(tEmDee* md, double lambda, double alpha, double dt) {
+  System* me = S(*md);
+  md->Time.Motion -= wtime();
+  double CR, CP;
+  if (alpha != 0.0) {
+    CP = phi_fn(alpha * dt) * dt;
+    CR = 1.0 - alpha * CP;
+    me->L = CR * me->L;
+  } else {
+    CP = dt;
+    CR = 1.0;
+  }
+  CP = lambda * CP;
+  if (md->Options.Translate) dev_displace(me->dev, CR, CP);
+  mark_stale(me, md);
+  md->Time.Motion += wtime();
+}
+void EmDee_displace(tEmDee* md, double lambda, double alpha, double dt) { displace(md, lambda, alpha, dt); }
+void EmDee_move(tEmDee* md, double lambda, double alpha, double dt) { displace(md, lambda, alpha, dt); }
+
+// EmDeeCode.f90:1107-1211
+void EmDee_verlet_step(tEmDee* md, double dt) {
+  (void)md; (void)dt;
+  fatal("verlet step", "not available in this build of the engine");
+}
+
+// EmDeeCode.f90:1215-1277
+void EmDee_compute_forces(tEmDee* md) { compute_forces(md); }
+
+// EmDeeCode.f90:1281-1395 (analysis observable, not part of the force engine)
+void EmDee_rdf(tEmDee md, int bins, double Rc, int pairs, int itype[], int jtype[], double g[]) {
+  (void)md; (void)bins; (void)Rc; (void)pairs; (void)itype; (void)jtype; (void)g;
+  fatal("radial distribution calculation", "not available in this build of the engine");
+}
+
+// ---- extensions (include/emdee_gpu.h) --------------------------------------
+
+void EmDeeGPU_set_precision(tEmDee md, int mode) {
+  System* me = S(md);
+  me->precision = mode ? 1 : 0;
+  for (int l = 1; l <= me->nlayers; l++) me->forcesUpToDate[l] = 0;
+}
+int EmDeeGPU_get_precision(tEmDee md) { return S(md)->precision; }
+int EmDeeGPU_neighbor_pairs(tEmDee md, int max, int* pairs) { return dev_neighbor_pairs(S(md)->dev, max, pairs); }
+void EmDeeGPU_profile(tEmDee md, int on) { dev_set_profiling(S(md)->dev, on != 0); }
+void EmDeeGPU_kernel_stats(tEmDee md, double* stats) { dev_kernel_stats(S(md)->dev, stats, 6); }
+void EmDeeGPU_synchronize(tEmDee md) { dev_sync(S(md)->dev); }
+void EmDeeGPU_destroy(tEmDee* md) {
+  if (!md || !md->Data) return;
+  System* me = S(*md);
+  dev_destroy(me->dev);
+  delete me;
+  md->Data = nullptr;
+}
+
+}  // extern "C"

@@ -1,0 +1,833 @@
+// device.hip -- HBM state and gfx950 kernels of the EmDee force engine.
+//
+// Layout in HBM (one system):
+//   * every per-atom array is SoA and stored in *cell-sorted order*: slot s
+//     holds original atom perm[s].  Coordinates R, momenta P, forces F (one
+//     set per model layer), scaled coordinates Rs = R/L and the positions R0
+//     at the latest list build are FP64 [3][Npad].
+//   * the Verlet list is a FULL list in ELL layout: neighbor k of slot s is
+//     nbr[k*Npad + s], so lanes of a wavefront (consecutive slots) read
+//     consecutive words.  A full list (every pair stored twice) lets one
+//     thread own one atom's force: no atomics, deterministic sums.
+//   * the list is rebuilt when the reference trigger fires
+//     (neighbor_lists.f90:41-59, 175-195): cells of side >= (rc+skin)/2, a
+//     counting sort of atoms by cell (stable: ascending original index within
+//     a cell), then a 125-cell scan per atom with the reference's FP64
+//     distance test on R/L.
+//
+// All launches go to one stream per system.  Host syncs happen only where
+// the C ABI must return host-visible values (energies, virial, builds).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <climits>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <vector>
+
+#include "engine.h"
+#include "pair_eval.h"
+
+namespace emdee {
+
+#define HIP_CHECK(x)                                                             \
+  do {                                                                           \
+    hipError_t e_ = (x);                                                         \
+    if (e_ != hipSuccess) fatal("HIP runtime", std::string(#x) + ": " + hipGetErrorString(e_)); \
+  } while (0)
+
+constexpr int BLOCK = 256;
+constexpr int NXCD = 8;
+
+static inline int div_up(long a, long b) { return (int)((a + b - 1) / b); }
+
+// Blocks are dispatched round-robin over the 8 XCDs (block b runs on XCD
+// b % 8).  Remap so that XCD x processes one contiguous range of slots: the
+// neighbor positions a wave gathers then mostly live in that XCD's L2.
+__device__ __forceinline__ int xcd_block() {
+  int b = blockIdx.x, nb = gridDim.x;
+  int q = nb / NXCD, rem = nb % NXCD, x = b % NXCD, r = b / NXCD;
+  return x * q + (x < rem ? x : rem) + r;
+}
+
+template <typename T>
+__device__ __forceinline__ T wave_sum(T v) {
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
+  return v;
+}
+
+// Sums NV values over the block; thread 0 of the block gets the totals.
+template <int NV>
+__device__ __forceinline__ void block_sum(double (&v)[NV]) {
+  __shared__ double red[BLOCK / 64][NV];
+  int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+#pragma unroll
+  for (int k = 0; k < NV; k++) v[k] = wave_sum(v[k]);
+  if (lane == 0) {
+#pragma unroll
+    for (int k = 0; k < NV; k++) red[wid][k] = v[k];
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+#pragma unroll
+    for (int k = 0; k < NV; k++) {
+      double s = 0;
+      for (int w = 0; w < BLOCK / 64; w++) s += red[w][k];
+      v[k] = s;
+    }
+  }
+}
+
+// ------------------------------------------------------------ device state
+
+template <typename T>
+struct DBuf {
+  T* p = nullptr;
+  size_t n = 0;
+  void alloc(size_t count) {
+    if (count <= n && p) return;
+    if (p) HIP_CHECK(hipFree(p));
+    HIP_CHECK(hipMalloc(&p, std::max<size_t>(count, 1) * sizeof(T)));
+    n = count;
+  }
+  void release() { if (p) (void)hipFree(p); p = nullptr; n = 0; }
+};
+
+struct Device {
+  int N = 0, Npad = 0, ntypes = 0, nlayers = 0;
+  hipStream_t stream = nullptr;
+  // per-atom SoA arrays (sorted order)
+  DBuf<double> R, P, Rs, R0, invMass, charge, F;   // F holds nlayers * 3 * Npad
+  DBuf<int> type, body, perm, freeFlag, chargedFlag;
+  DBuf<double> tmpD;      // permutation scratch
+  DBuf<int> tmpI;
+  // transfers
+  DBuf<double> stage;     // 3N AoS, original order
+  double* hstage = nullptr;  // pinned host mirror
+  // cells
+  int M = 0, ncells = 0;
+  DBuf<int> key, cellCount, cellStart, order, blockSums;
+  // neighbor list
+  int cap = 0;
+  DBuf<int> nbr, nnb;
+  DBuf<int> flags;        // [0] overflow max count
+  bool built = false;
+  // models
+  DBuf<DevPair> pairs;    // nlayers * nt * nt
+  DBuf<DevCoul> couls;    // nlayers
+  DBuf<uint8_t> interact;
+  DBuf<int> exOff, exItem;
+  int nexcl = 0;
+  std::vector<int> hostType;
+  // reductions
+  DBuf<double> partials, result;
+  double* hresult = nullptr;
+  DBuf<double> d2;
+  DBuf<unsigned long long> trig;   // [0] max bits, [1] min orig index, [2] before bits
+  // profiling
+  bool profiling = false;
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  double pairMs = 0, listMs = 0;
+  long pairLaunches = 0, listLaunches = 0;
+  long pairsVisited = 0;
+};
+
+[[noreturn]] void fatal(const std::string& task, const std::string& msg) {
+  fprintf(stderr, "Error in %s: %s.\n", task.c_str(), msg.c_str());
+  fflush(stderr);
+  exit(1);
+}
+void warning(const std::string& msg) { fprintf(stderr, "WARNING: %s.\n", msg.c_str()); }
+
+// --------------------------------------------------------------- kernels
+
+__global__ void k_iota(int* p, int n) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = i;
+}
+
+// AoS (original order) -> SoA (sorted order)
+__global__ void k_gather_aos(double* __restrict__ dst, const double* __restrict__ aos,
+                             const int* __restrict__ perm, const int* __restrict__ mask, int N, int Npad) {
+  int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= N) return;
+  if (mask && !mask[s]) return;
+  int o = perm[s];
+  dst[s] = aos[3 * o];
+  dst[Npad + s] = aos[3 * o + 1];
+  dst[2 * Npad + s] = aos[3 * o + 2];
+}
+// SoA (sorted order) -> AoS (original order)
+__global__ void k_scatter_aos(double* __restrict__ aos, const double* __restrict__ src,
+                              const int* __restrict__ perm, int N, int Npad) {
+  int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= N) return;
+  int o = perm[s];
+  aos[3 * o] = src[s];
+  aos[3 * o + 1] = src[Npad + s];
+  aos[3 * o + 2] = src[2 * Npad + s];
+}
+__global__ void k_add_aos(double* __restrict__ dst, const double* __restrict__ aos,
+                          const int* __restrict__ perm, int N, int Npad) {
+  int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= N) return;
+  int o = perm[s];
+  dst[s] += aos[3 * o];
+  dst[Npad + s] += aos[3 * o + 1];
+  dst[2 * Npad + s] += aos[3 * o + 2];
+}
+template <typename T>
+__global__ void k_gather_scalar(T* __restrict__ dst, const T* __restrict__ src, const int* __restrict__ perm, int N) {
+  int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s < N) dst[s] = src[perm[s]];
+}
+
+// Rs = R / L, exactly as EmDeeCode.f90:1228 (IEEE division).
+__global__ void k_scale(double* __restrict__ Rs, const double* __restrict__ R, double L, int n) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) Rs[i] = R[i] / L;
+}
+
+// ---- rebuild trigger, neighbor_lists.f90:41-59 ---------------------------
+// With d_k = |R_k - R0_k|^2 in ORIGINAL atom order, the reference returns
+// max + 2 sqrt(max*next) + next where next = d_1 if atom 1 holds the maximum
+// and otherwise the maximum over atoms before the first one that attains it.
+// Three passes: (1) max, (2) first original index attaining it, (3) maximum
+// over atoms with a smaller original index.  Non-negative doubles compare as
+// their bit patterns, so the passes use 64-bit integer atomics.
+__global__ void k_disp_max(double* __restrict__ d2, const double* __restrict__ R, const double* __restrict__ R0,
+                           unsigned long long* trig, int N, int Npad) {
+  int s = blockIdx.x * blockDim.x + threadIdx.x;
+  double v = 0;
+  if (s < N) {
+#pragma clang fp contract(off)
+    double dx = R[s] - R0[s], dy = R[Npad + s] - R0[Npad + s], dz = R[2 * Npad + s] - R0[2 * Npad + s];
+    v = dx * dx + dy * dy + dz * dz;
+    d2[s] = v;
+  }
+  for (int off = 32; off > 0; off >>= 1) v = fmax(v, __shfl_down(v, off, 64));
+  if ((threadIdx.x & 63) == 0) atomicMax(&trig[0], (unsigned long long)__double_as_longlong(v));
+}
+__global__ void k_disp_first(const double* __restrict__ d2, const int* __restrict__ perm,
+                             unsigned long long* trig, int N) {
+  int s = blockIdx.x * blockDim.x + threadIdx.x;
+  unsigned long long mx = trig[0];
+  int idx = INT_MAX;
+  if (s < N && (unsigned long long)__double_as_longlong(d2[s]) == mx) idx = perm[s];
+  for (int off = 32; off > 0; off >>= 1) idx = min(idx, __shfl_down(idx, off, 64));
+  if ((threadIdx.x & 63) == 0 && idx != INT_MAX) atomicMin(&trig[1], (unsigned long long)idx);
+}
+__global__ void k_disp_before(const double* __restrict__ d2, const int* __restrict__ perm,
+                              unsigned long long* trig, int N) {
+  int s = blockIdx.x * blockDim.x + threadIdx.x;
+  unsigned long long k = trig[1];
+  double v = 0;
+  if (s < N && (unsigned long long)perm[s] < k) v = d2[s];
+  for (int off = 32; off > 0; off >>= 1) v = fmax(v, __shfl_down(v, off, 64));
+  if ((threadIdx.x & 63) == 0) atomicMax(&trig[2], (unsigned long long)__double_as_longlong(v));
+}
+
+// ---- cell sort ------------------------------------------------------------
+// Cell of slot s: icoord = int(M*(Rs - floor(Rs))) (neighbor_lists.f90:126);
+// a value that rounds to M is the periodic image 0.
+__device__ __forceinline__ int cell_coord(double s, int M) {
+  int c = (int)(M * (s - floor(s)));
+  return c >= M ? c - M : c;
+}
+__global__ void k_cell_key(int* __restrict__ key, int* __restrict__ count, const double* __restrict__ Rs,
+                           int M, int N, int Npad) {
+  int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= N) return;
+  int ix = cell_coord(Rs[s], M), iy = cell_coord(Rs[Npad + s], M), iz = cell_coord(Rs[2 * Npad + s], M);
+  int c = ix + M * (iy + M * iz);
+  key[s] = c;
+  atomicAdd(&count[c], 1);
+}
+// Exclusive scan, three phases: per-block scan, scan of block sums, add.
+__global__ void k_scan_block(int* __restrict__ out, int* __restrict__ bsum, const int* __restrict__ in, int n) {
+  __shared__ int sh[BLOCK];
+  int i = blockIdx.x * BLOCK + threadIdx.x;
+  int v = i < n ? in[i] : 0;
+  sh[threadIdx.x] = v;
+  __syncthreads();
+  for (int off = 1; off < BLOCK; off <<= 1) {
+    int t = threadIdx.x >= off ? sh[threadIdx.x - off] : 0;
+    __syncthreads();
+    sh[threadIdx.x] += t;
+    __syncthreads();
+  }
+  if (i < n) out[i] = sh[threadIdx.x] - v;
+  if (threadIdx.x == BLOCK - 1) bsum[blockIdx.x] = sh[BLOCK - 1];
+}
+__global__ void k_scan_sums(int* bsum, int nb, int* total) {
+  // one block; serial carry across chunks of BLOCK
+  __shared__ int sh[BLOCK];
+  int carry = 0;
+  for (int base = 0; base < nb; base += BLOCK) {
+    int i = base + threadIdx.x;
+    int v = i < nb ? bsum[i] : 0;
+    sh[threadIdx.x] = v;
+    __syncthreads();
+    for (int off = 1; off < BLOCK; off <<= 1) {
+      int t = threadIdx.x >= off ? sh[threadIdx.x - off] : 0;
+      __syncthreads();
+      sh[threadIdx.x] += t;
+      __syncthreads();
+    }
+    if (i < nb) bsum[i] = sh[threadIdx.x] - v + carry;
+    int add = sh[BLOCK - 1];
+    __syncthreads();
+    carry += add;
+  }
+  if (threadIdx.x == 0 && total) *total = carry;
+}
+__global__ void k_scan_add(int* __restrict__ out, const int* __restrict__ bsum, int n) {
+  int i = blockIdx.x * BLOCK + threadIdx.x;
+  if (i < n) out[i] += bsum[blockIdx.x];
+}
+__global__ void k_cell_scatter(int* __restrict__ order, int* __restrict__ fill, const int* __restrict__ key,
+                               const int* __restrict__ start, int N) {
+  int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= N) return;
+  int c = key[s];
+  order[start[c] + atomicAdd(&fill[c], 1)] = s;
+}
+// Deterministic order inside each cell: ascending original atom index.
+__global__ void k_cell_sort(int* __restrict__ order, const int* __restrict__ start, const int* __restrict__ perm,
+                            int ncells, int N) {
+  int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= ncells) return;
+  int a = start[c], b = (c + 1 < ncells) ? start[c + 1] : N;
+  for (int x = a + 1; x < b; x++) {
+    int v = order[x], pv = perm[v], y = x - 1;
+    while (y >= a && perm[order[y]] > pv) { order[y + 1] = order[y]; y--; }
+    order[y + 1] = v;
+  }
+}
+template <typename T>
+__global__ void k_permute(T* __restrict__ dst, const T* __restrict__ src, const int* __restrict__ order, int N) {
+  int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s < N) dst[s] = src[order[s]];
+}
+
+// ---- Verlet list ------------------------------------------------------------
+// Full list: for slot i, every j != i in the 125 cells within +-2 cells with
+// r^2 < (rc+skin)^2/L^2 computed as in neighbor_lists.f90:244 (FP64,
+// minimum image by anint, no FMA contraction), that is not excluded, not in
+// the same rigid body and whose types interact (neighbor_lists.f90:250-259).
+__global__ void __launch_bounds__(BLOCK)
+k_build_list(int* __restrict__ nbr, int* __restrict__ nnb, int* __restrict__ flags,
+             const double* __restrict__ Rs, const int* __restrict__ key, const int* __restrict__ start,
+             const int* __restrict__ type, const int* __restrict__ body, const int* __restrict__ perm,
+             const uint8_t* __restrict__ interact, const int* __restrict__ exOff, const int* __restrict__ exItem,
+             int nexcl, int ntypes, int M, int ncells, double xRc2, int cap, int N, int Npad) {
+  int i = xcd_block() * BLOCK + threadIdx.x;
+  if (i >= N) return;
+  double xi = Rs[i], yi = Rs[Npad + i], zi = Rs[2 * Npad + i];
+  int c = key[i];
+  int cx = c % M, cy = (c / M) % M, cz = c / (M * M);
+  int ti = type[i], bi = body[i];
+  int oi = perm[i];
+  int e0 = 0, e1 = 0;
+  if (nexcl) { e0 = exOff[oi]; e1 = exOff[oi + 1]; }
+  int cnt = 0;
+  for (int dz = -2; dz <= 2; dz++) {
+    int z = cz + dz; z = z < 0 ? z + M : (z >= M ? z - M : z);
+    for (int dy = -2; dy <= 2; dy++) {
+      int y = cy + dy; y = y < 0 ? y + M : (y >= M ? y - M : y);
+      for (int dx = -2; dx <= 2; dx++) {
+        int x = cx + dx; x = x < 0 ? x + M : (x >= M ? x - M : x);
+        int cc = x + M * (y + M * z);
+        int a = start[cc], b = (cc + 1 < ncells) ? start[cc + 1] : N;
+        for (int j = a; j < b; j++) {
+          if (j == i) continue;
+          double r2;
+          {
+#pragma clang fp contract(off)
+            double d0 = xi - Rs[j], d1 = yi - Rs[Npad + j], d2 = zi - Rs[2 * Npad + j];
+            d0 = d0 - round(d0); d1 = d1 - round(d1); d2 = d2 - round(d2);
+            r2 = d0 * d0 + d1 * d1 + d2 * d2;
+          }
+          if (r2 < xRc2) {
+            if (body[j] == bi) continue;
+            if (!interact[type[j] * ntypes + ti]) continue;
+            if (nexcl) {
+              int oj = perm[j];
+              bool ex = false;
+              for (int e = e0; e < e1; e++) if (exItem[e] == oj) { ex = true; break; }
+              if (ex) continue;
+            }
+            if (cnt < cap) nbr[(size_t)cnt * Npad + i] = j;
+            cnt++;
+          }
+        }
+      }
+    }
+  }
+  nnb[i] = cnt;
+  if (cnt > cap) atomicMax(&flags[0], cnt);
+}
+
+// ---- pair forces ----------------------------------------------------------
+// One thread per atom over its full list; the per-pair body is
+// compute.f90:46-92.  Energies and virials of a full list count every pair
+// twice and are halved at the end (exact).
+template <bool ENERGY, bool MIXED>
+__global__ void __launch_bounds__(BLOCK)
+k_pairs(double* __restrict__ F, double* __restrict__ partials,
+        const double* __restrict__ Rs, const int* __restrict__ nbr, const int* __restrict__ nnb,
+        const int* __restrict__ type, const double* __restrict__ charge, const int* __restrict__ charged,
+        const DevPair* __restrict__ pairs, const DevCoul* __restrict__ coulp,
+        int ntypes, double L, double invL2, double Rc2, int N, int Npad) {
+  using T = typename std::conditional<MIXED, float, double>::type;
+  int blk = xcd_block();
+  int i = blk * BLOCK + threadIdx.x;
+  double Ep = 0, Ec = 0, Wp = 0, Wc = 0;
+  if (i < N) {
+    const DevCoul cm = *coulp;
+    double xi = Rs[i], yi = Rs[Npad + i], zi = Rs[2 * Npad + i];
+    int ti = type[i];
+    double qi = charge[i];
+    int ich = charged[i];
+    const DevPair* prow = pairs + ti * ntypes;
+    double fx = 0, fy = 0, fz = 0;
+    int n = nnb[i];
+    for (int k = 0; k < n; k++) {
+      int j = nbr[(size_t)k * Npad + i];
+      double dx = xi - Rs[j], dy = yi - Rs[Npad + j], dz = zi - Rs[2 * Npad + j];
+      dx = dx - round(dx); dy = dy - round(dy); dz = dz - round(dz);
+      double r2;
+      {
+#pragma clang fp contract(off)
+        r2 = dx * dx + dy * dy + dz * dz;
+      }
+      if (r2 < Rc2) {
+        int tj = type[j];
+        const DevPair p = prow[tj];
+        T invR2 = MIXED ? T(invL2) * rcp<T>(T(r2)) : T(invL2 / r2);
+        T invR = sqrt(invR2);
+        T E, W;
+        pair_model<T>(p, invR2, E, W);
+        apply_modifier<T>(p.modifier, T(p.eshift), T(p.fshift), T(p.factor), T(p.Rm2fac), invR, invR2, E, W);
+        if (ENERGY) Ep += (double)E;
+        Wp += (double)W;
+        T Wsum = W;
+        if (ich && charged[j] && p.coulomb) {
+          coul_model<T, ENERGY>(cm, invR, invR2, E, W);
+          apply_modifier<T>(cm.modifier, T(cm.eshift), T(cm.fshift), T(cm.factor), T(cm.Rm2fac), invR, invR2, E, W);
+          double qiqj = p.kCoul * qi * charge[j];
+          if (ENERGY) Ec += qiqj * (double)E;
+          T Wq = T(qiqj) * W;
+          Wc += (double)Wq;
+          Wsum = Wsum + Wq;
+        }
+        T g = Wsum * invR2;
+        fx += (double)(g * T(dx));
+        fy += (double)(g * T(dy));
+        fz += (double)(g * T(dz));
+      }
+    }
+    F[i] = L * fx;
+    F[Npad + i] = L * fy;
+    F[2 * Npad + i] = L * fz;
+  }
+  double v[4] = { Ep, Ec, Wp, Wc };
+  block_sum<4>(v);
+  if (threadIdx.x == 0) {
+    partials[4 * blk + 0] = 0.5 * v[0];
+    partials[4 * blk + 1] = 0.5 * v[1];
+    partials[4 * blk + 2] = 0.5 * v[2];
+    partials[4 * blk + 3] = 0.5 * v[3];
+  }
+}
+
+// Deterministic final reduction of per-block partials (fixed order).
+template <int NV>
+__global__ void k_reduce_partials(double* __restrict__ out, const double* __restrict__ partials, int nblocks) {
+  double v[NV];
+#pragma unroll
+  for (int k = 0; k < NV; k++) v[k] = 0;
+  for (int b = threadIdx.x; b < nblocks; b += BLOCK) {
+#pragma unroll
+    for (int k = 0; k < NV; k++) v[k] += partials[NV * b + k];
+  }
+  block_sum<NV>(v);
+  if (threadIdx.x == 0) {
+#pragma unroll
+    for (int k = 0; k < NV; k++) out[k] = v[k];
+  }
+}
+
+// ---- integrator -------------------------------------------------------------
+// boost, EmDeeData.f90:864-896 (free atoms): P = CP*P + CF*F;
+// kinetic_energies, EmDeeData.f90:900-922: 2KE_x = sum invMass*P_x^2.
+template <bool KE>
+__global__ void k_boost(double* __restrict__ P, double* __restrict__ partials, const double* __restrict__ F,
+                        const double* __restrict__ invMass, const int* __restrict__ freeFlag,
+                        double CP, double CF, int translate, int N, int Npad) {
+  int blk = blockIdx.x;
+  int s = blk * BLOCK + threadIdx.x;
+  double v[3] = { 0, 0, 0 };
+  if (s < N && freeFlag[s]) {
+    double im = invMass[s];
+#pragma unroll
+    for (int x = 0; x < 3; x++) {
+      double p = P[x * Npad + s];
+      if (translate) { p = CP * p + CF * F[x * Npad + s]; P[x * Npad + s] = p; }
+      if (KE) v[x] = im * p * p;
+    }
+  }
+  if (KE) {
+    block_sum<3>(v);
+    if (threadIdx.x == 0) { partials[3 * blk] = v[0]; partials[3 * blk + 1] = v[1]; partials[3 * blk + 2] = v[2]; }
+  }
+}
+// move, EmDeeData.f90:823-860 (free atoms): R = CR*R + CP*P*invMass.
+__global__ void k_displace(double* __restrict__ R, const double* __restrict__ P, const double* __restrict__ invMass,
+                           const int* __restrict__ freeFlag, double CR, double CP, int N, int Npad) {
+  int s = blockIdx.x * BLOCK + threadIdx.x;
+  if (s >= N || !freeFlag[s]) return;
+  double im = invMass[s];
+#pragma unroll
+  for (int x = 0; x < 3; x++) R[x * Npad + s] = CR * R[x * Npad + s] + CP * P[x * Npad + s] * im;
+}
+
+// ------------------------------------------------------------ host side
+
+static void launch_check(const char* what) {
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) fatal("HIP kernel launch", std::string(what) + ": " + hipGetErrorString(e));
+}
+
+Device* dev_create(const DeviceSetup& s) {
+  Device* d = new Device();
+  d->N = s.N;
+  d->Npad = div_up(std::max(s.N, 1), BLOCK) * BLOCK;
+  d->ntypes = s.ntypes;
+  d->nlayers = s.nlayers;
+  int Np = d->Npad;
+  HIP_CHECK(hipStreamCreateWithFlags(&d->stream, hipStreamNonBlocking));
+  HIP_CHECK(hipEventCreate(&d->ev0));
+  HIP_CHECK(hipEventCreate(&d->ev1));
+  d->R.alloc(3 * (size_t)Np); d->P.alloc(3 * (size_t)Np); d->Rs.alloc(3 * (size_t)Np); d->R0.alloc(3 * (size_t)Np);
+  d->F.alloc(3 * (size_t)Np * s.nlayers);
+  d->invMass.alloc(Np); d->charge.alloc(Np);
+  d->type.alloc(Np); d->body.alloc(Np); d->perm.alloc(Np); d->freeFlag.alloc(Np); d->chargedFlag.alloc(Np);
+  d->tmpD.alloc(3 * (size_t)Np); d->tmpI.alloc(Np);
+  d->stage.alloc(3 * (size_t)std::max(s.N, 1));
+  HIP_CHECK(hipHostMalloc(&d->hstage, 3 * sizeof(double) * std::max(s.N, 1), hipHostMallocDefault));
+  HIP_CHECK(hipHostMalloc(&d->hresult, 64 * sizeof(double), hipHostMallocDefault));
+  d->result.alloc(64);
+  d->partials.alloc(8 * (size_t)(Np / BLOCK + 1));
+  d->d2.alloc(Np);
+  d->trig.alloc(4);
+  d->key.alloc(Np); d->order.alloc(Np); d->nnb.alloc(Np); d->flags.alloc(4);
+  HIP_CHECK(hipMemsetAsync(d->R.p, 0, 3 * sizeof(double) * Np, d->stream));
+  HIP_CHECK(hipMemsetAsync(d->P.p, 0, 3 * sizeof(double) * Np, d->stream));
+  HIP_CHECK(hipMemsetAsync(d->R0.p, 0, 3 * sizeof(double) * Np, d->stream));
+  HIP_CHECK(hipMemsetAsync(d->F.p, 0, 3 * sizeof(double) * Np * s.nlayers, d->stream));
+  HIP_CHECK(hipMemsetAsync(d->nnb.p, 0, sizeof(int) * Np, d->stream));
+  HIP_CHECK(hipMemsetAsync(d->freeFlag.p, 0, sizeof(int) * Np, d->stream));
+  int g = div_up(s.N, BLOCK);
+  if (s.N > 0) hipLaunchKernelGGL(k_iota, dim3(g), dim3(BLOCK), 0, d->stream, d->perm.p, s.N);
+  launch_check("iota");
+  // static per-atom data, initial order = original order
+  std::vector<double> im(s.N), q(s.N);
+  std::vector<int> free_(s.N), chg(s.N);
+  for (int i = 0; i < s.N; i++) { im[i] = s.invMass[i]; q[i] = s.charge[i]; free_[i] = s.freeFlag[i]; chg[i] = std::fabs(s.charge[i]) > 2.220446049250313e-16; }
+  HIP_CHECK(hipMemcpyAsync(d->type.p, s.type.data(), sizeof(int) * s.N, hipMemcpyHostToDevice, d->stream));
+  HIP_CHECK(hipMemcpyAsync(d->body.p, s.body.data(), sizeof(int) * s.N, hipMemcpyHostToDevice, d->stream));
+  HIP_CHECK(hipMemcpyAsync(d->invMass.p, im.data(), sizeof(double) * s.N, hipMemcpyHostToDevice, d->stream));
+  HIP_CHECK(hipMemcpyAsync(d->charge.p, q.data(), sizeof(double) * s.N, hipMemcpyHostToDevice, d->stream));
+  HIP_CHECK(hipMemcpyAsync(d->freeFlag.p, free_.data(), sizeof(int) * s.N, hipMemcpyHostToDevice, d->stream));
+  HIP_CHECK(hipMemcpyAsync(d->chargedFlag.p, chg.data(), sizeof(int) * s.N, hipMemcpyHostToDevice, d->stream));
+  d->hostType = s.type;
+  d->pairs.alloc((size_t)s.nlayers * s.ntypes * s.ntypes);
+  d->couls.alloc(s.nlayers);
+  d->interact.alloc((size_t)s.ntypes * s.ntypes);
+  HIP_CHECK(hipMemsetAsync(d->interact.p, 0, s.ntypes * s.ntypes, d->stream));
+  d->exOff.alloc(s.N + 1);
+  d->exItem.alloc(1);
+  HIP_CHECK(hipStreamSynchronize(d->stream));
+  return d;
+}
+
+void dev_destroy(Device* d) {
+  if (!d) return;
+  (void)hipStreamSynchronize(d->stream);
+  for (DBuf<double>* b : { &d->R, &d->P, &d->Rs, &d->R0, &d->invMass, &d->charge, &d->F, &d->tmpD, &d->stage, &d->partials, &d->result, &d->d2 }) b->release();
+  for (DBuf<int>* b : { &d->type, &d->body, &d->perm, &d->freeFlag, &d->chargedFlag, &d->tmpI, &d->key, &d->cellCount, &d->cellStart, &d->order, &d->blockSums, &d->nbr, &d->nnb, &d->flags, &d->exOff, &d->exItem }) b->release();
+  d->pairs.release(); d->couls.release(); d->interact.release(); d->trig.release();
+  if (d->hstage) (void)hipHostFree(d->hstage);
+  if (d->hresult) (void)hipHostFree(d->hresult);
+  (void)hipEventDestroy(d->ev0); (void)hipEventDestroy(d->ev1);
+  (void)hipStreamDestroy(d->stream);
+  delete d;
+}
+
+void dev_set_charges(Device* d, const std::vector<double>& q) {
+  int N = d->N;
+  std::vector<int> chg(N);
+  for (int i = 0; i < N; i++) chg[i] = std::fabs(q[i]) > 2.220446049250313e-16;
+  // original order -> sorted order through perm
+  DBuf<double> tq; tq.alloc(N);
+  DBuf<int> tc; tc.alloc(N);
+  HIP_CHECK(hipMemcpyAsync(tq.p, q.data(), sizeof(double) * N, hipMemcpyHostToDevice, d->stream));
+  HIP_CHECK(hipMemcpyAsync(tc.p, chg.data(), sizeof(int) * N, hipMemcpyHostToDevice, d->stream));
+  int g = div_up(N, BLOCK);
+  hipLaunchKernelGGL(k_gather_scalar<double>, dim3(g), dim3(BLOCK), 0, d->stream, d->charge.p, tq.p, d->perm.p, N);
+  hipLaunchKernelGGL(k_gather_scalar<int>, dim3(g), dim3(BLOCK), 0, d->stream, d->chargedFlag.p, tc.p, d->perm.p, N);
+  launch_check("set_charges");
+  HIP_CHECK(hipStreamSynchronize(d->stream));
+  tq.release(); tc.release();
+}
+
+void dev_set_exclusions(Device* d, const std::vector<int>& offset, const std::vector<int>& items) {
+  d->nexcl = (int)items.size();
+  d->exOff.alloc(offset.size());
+  d->exItem.alloc(std::max<size_t>(items.size(), 1));
+  HIP_CHECK(hipMemcpyAsync(d->exOff.p, offset.data(), sizeof(int) * offset.size(), hipMemcpyHostToDevice, d->stream));
+  if (!items.empty())
+    HIP_CHECK(hipMemcpyAsync(d->exItem.p, items.data(), sizeof(int) * items.size(), hipMemcpyHostToDevice, d->stream));
+  HIP_CHECK(hipStreamSynchronize(d->stream));
+}
+
+void dev_set_interact(Device* d, const std::vector<uint8_t>& table) {
+  HIP_CHECK(hipMemcpyAsync(d->interact.p, table.data(), table.size(), hipMemcpyHostToDevice, d->stream));
+  HIP_CHECK(hipStreamSynchronize(d->stream));
+}
+
+void dev_set_models(Device* d, int layer, const std::vector<DevPair>& pairs, const DevCoul& coul) {
+  size_t nt2 = (size_t)d->ntypes * d->ntypes;
+  HIP_CHECK(hipMemcpyAsync(d->pairs.p + nt2 * layer, pairs.data(), sizeof(DevPair) * nt2, hipMemcpyHostToDevice, d->stream));
+  HIP_CHECK(hipMemcpyAsync(d->couls.p + layer, &coul, sizeof(DevCoul), hipMemcpyHostToDevice, d->stream));
+  HIP_CHECK(hipStreamSynchronize(d->stream));
+}
+
+static void upload_aos(Device* d, double* dst, const double* host, const int* mask) {
+  int N = d->N;
+  memcpy(d->hstage, host, 3 * sizeof(double) * N);
+  HIP_CHECK(hipMemcpyAsync(d->stage.p, d->hstage, 3 * sizeof(double) * N, hipMemcpyHostToDevice, d->stream));
+  hipLaunchKernelGGL(k_gather_aos, dim3(div_up(N, BLOCK)), dim3(BLOCK), 0, d->stream, dst, d->stage.p, d->perm.p, mask, N, d->Npad);
+  launch_check("gather_aos");
+  HIP_CHECK(hipStreamSynchronize(d->stream));
+}
+static void download_aos(Device* d, double* host, const double* src) {
+  int N = d->N;
+  hipLaunchKernelGGL(k_scatter_aos, dim3(div_up(N, BLOCK)), dim3(BLOCK), 0, d->stream, d->stage.p, src, d->perm.p, N, d->Npad);
+  launch_check("scatter_aos");
+  HIP_CHECK(hipMemcpyAsync(d->hstage, d->stage.p, 3 * sizeof(double) * N, hipMemcpyDeviceToHost, d->stream));
+  HIP_CHECK(hipStreamSynchronize(d->stream));
+  memcpy(host, d->hstage, 3 * sizeof(double) * N);
+}
+
+void dev_upload_positions(Device* d, const double* R) { if (d->N) upload_aos(d, d->R.p, R, nullptr); }
+void dev_download_positions(Device* d, double* R) { if (d->N) download_aos(d, R, d->R.p); }
+void dev_upload_momenta(Device* d, const double* P) { if (d->N) upload_aos(d, d->P.p, P, d->freeFlag.p); }
+void dev_download_momenta(Device* d, double* P) { if (d->N) download_aos(d, P, d->P.p); }
+void dev_upload_forces(Device* d, int layer, const double* F) {
+  if (d->N) upload_aos(d, d->F.p + 3 * (size_t)d->Npad * layer, F, nullptr);
+}
+void dev_download_forces(Device* d, int layer, double* F) {
+  if (d->N) download_aos(d, F, d->F.p + 3 * (size_t)d->Npad * layer);
+}
+void dev_add_forces(Device* d, int layer, const double* F) {
+  int N = d->N;
+  if (!N) return;
+  memcpy(d->hstage, F, 3 * sizeof(double) * N);
+  HIP_CHECK(hipMemcpyAsync(d->stage.p, d->hstage, 3 * sizeof(double) * N, hipMemcpyHostToDevice, d->stream));
+  hipLaunchKernelGGL(k_add_aos, dim3(div_up(N, BLOCK)), dim3(BLOCK), 0, d->stream, d->F.p + 3 * (size_t)d->Npad * layer, d->stage.p, d->perm.p, N, d->Npad);
+  launch_check("add_aos");
+  HIP_CHECK(hipStreamSynchronize(d->stream));
+}
+
+bool dev_needs_rebuild(Device* d, double skinSq) {
+  int N = d->N;
+  if (N == 0) return false;
+  int g = div_up(N, BLOCK);
+  unsigned long long* init = reinterpret_cast<unsigned long long*>(d->hresult + 32);
+  init[0] = 0ull; init[1] = (unsigned long long)INT_MAX; init[2] = 0ull; init[3] = 0ull;
+  HIP_CHECK(hipMemcpyAsync(d->trig.p, init, 4 * sizeof(unsigned long long), hipMemcpyHostToDevice, d->stream));
+  hipLaunchKernelGGL(k_disp_max, dim3(g), dim3(BLOCK), 0, d->stream, d->d2.p, d->R.p, d->R0.p, d->trig.p, N, d->Npad);
+  hipLaunchKernelGGL(k_disp_first, dim3(g), dim3(BLOCK), 0, d->stream, d->d2.p, d->perm.p, d->trig.p, N);
+  hipLaunchKernelGGL(k_disp_before, dim3(g), dim3(BLOCK), 0, d->stream, d->d2.p, d->perm.p, d->trig.p, N);
+  launch_check("trigger");
+  unsigned long long* h = reinterpret_cast<unsigned long long*>(d->hresult);
+  HIP_CHECK(hipMemcpyAsync(h, d->trig.p, 3 * sizeof(unsigned long long), hipMemcpyDeviceToHost, d->stream));
+  HIP_CHECK(hipStreamSynchronize(d->stream));
+  double maximum, before, next;
+  memcpy(&maximum, &h[0], 8);
+  memcpy(&before, &h[2], 8);
+  next = (h[1] == 0ull) ? maximum : before;
+  double approach = maximum + 2 * std::sqrt(maximum * next) + next;
+  return approach > skinSq;
+}
+
+static void ensure_cells(Device* d, int M) {
+  if (M == d->M) return;
+  d->M = M;
+  d->ncells = M * M * M;
+  d->cellCount.alloc(d->ncells);
+  d->cellStart.alloc(d->ncells);
+  d->blockSums.alloc(div_up(d->ncells, BLOCK) + 1);
+}
+
+void dev_rebuild(Device* d, double L, double xRc, double xInRc) {
+  (void)xInRc;
+  int N = d->N, Np = d->Npad;
+  if (N == 0) return;
+  int g = div_up(N, BLOCK);
+  int M = (int)std::floor(2 * L / xRc);
+  if (M < 5) M = 5;
+  if ((long)M * M * M > 64l * 1024 * 1024) M = 400;   // bound the cell table
+  ensure_cells(d, M);
+  int nc = d->ncells, gc = div_up(nc, BLOCK);
+  // current scaled positions
+  hipLaunchKernelGGL(k_scale, dim3(div_up(3l * Np, BLOCK)), dim3(BLOCK), 0, d->stream, d->Rs.p, d->R.p, L, 3 * Np);
+  // counting sort by cell
+  HIP_CHECK(hipMemsetAsync(d->cellCount.p, 0, sizeof(int) * nc, d->stream));
+  hipLaunchKernelGGL(k_cell_key, dim3(g), dim3(BLOCK), 0, d->stream, d->key.p, d->cellCount.p, d->Rs.p, M, N, Np);
+  hipLaunchKernelGGL(k_scan_block, dim3(gc), dim3(BLOCK), 0, d->stream, d->cellStart.p, d->blockSums.p, d->cellCount.p, nc);
+  hipLaunchKernelGGL(k_scan_sums, dim3(1), dim3(BLOCK), 0, d->stream, d->blockSums.p, gc, (int*)nullptr);
+  hipLaunchKernelGGL(k_scan_add, dim3(gc), dim3(BLOCK), 0, d->stream, d->cellStart.p, d->blockSums.p, nc);
+  HIP_CHECK(hipMemsetAsync(d->cellCount.p, 0, sizeof(int) * nc, d->stream));
+  hipLaunchKernelGGL(k_cell_scatter, dim3(g), dim3(BLOCK), 0, d->stream, d->order.p, d->cellCount.p, d->key.p, d->cellStart.p, N);
+  hipLaunchKernelGGL(k_cell_sort, dim3(gc), dim3(BLOCK), 0, d->stream, d->order.p, d->cellStart.p, d->perm.p, nc, N);
+  launch_check("cell sort");
+  // permute every per-atom array into the new order
+  auto permD = [&](double* a) {
+    hipLaunchKernelGGL(k_permute<double>, dim3(g), dim3(BLOCK), 0, d->stream, d->tmpD.p, a, d->order.p, N);
+    HIP_CHECK(hipMemcpyAsync(a, d->tmpD.p, sizeof(double) * N, hipMemcpyDeviceToDevice, d->stream));
+  };
+  auto permI = [&](int* a) {
+    hipLaunchKernelGGL(k_permute<int>, dim3(g), dim3(BLOCK), 0, d->stream, d->tmpI.p, a, d->order.p, N);
+    HIP_CHECK(hipMemcpyAsync(a, d->tmpI.p, sizeof(int) * N, hipMemcpyDeviceToDevice, d->stream));
+  };
+  for (int x = 0; x < 3; x++) {
+    permD(d->R.p + (size_t)x * Np);
+    permD(d->P.p + (size_t)x * Np);
+    permD(d->Rs.p + (size_t)x * Np);
+    for (int l = 0; l < d->nlayers; l++) permD(d->F.p + 3 * (size_t)Np * l + (size_t)x * Np);
+  }
+  permD(d->invMass.p); permD(d->charge.p);
+  permI(d->type.p); permI(d->body.p); permI(d->perm.p); permI(d->freeFlag.p); permI(d->chargedFlag.p);
+  // keys in the new order (a stable property: recomputed from permuted Rs)
+  HIP_CHECK(hipMemsetAsync(d->cellCount.p, 0, sizeof(int) * nc, d->stream));
+  hipLaunchKernelGGL(k_cell_key, dim3(g), dim3(BLOCK), 0, d->stream, d->key.p, d->cellCount.p, d->Rs.p, M, N, Np);
+  // R0 = R
+  HIP_CHECK(hipMemcpyAsync(d->R0.p, d->R.p, 3 * sizeof(double) * Np, hipMemcpyDeviceToDevice, d->stream));
+  // Verlet list, growing the ELL capacity on overflow
+  double xRc2 = (xRc * xRc) / (L * L);
+  if (d->cap == 0) {
+    double vol = 4.18879020478639 * xRc * xRc * xRc * ((double)N / (L * L * L));
+    d->cap = std::max(32, (int)(1.35 * vol + 16));
+    d->cap = (d->cap + 7) & ~7;
+  }
+  for (;;) {
+    d->nbr.alloc((size_t)d->cap * Np);
+    HIP_CHECK(hipMemsetAsync(d->flags.p, 0, sizeof(int) * 4, d->stream));
+    if (d->profiling) HIP_CHECK(hipEventRecord(d->ev0, d->stream));
+    hipLaunchKernelGGL(k_build_list, dim3(g), dim3(BLOCK), 0, d->stream, d->nbr.p, d->nnb.p, d->flags.p, d->Rs.p, d->key.p,
+                       d->cellStart.p, d->type.p, d->body.p, d->perm.p, d->interact.p, d->exOff.p, d->exItem.p,
+                       d->nexcl, d->ntypes, M, nc, xRc2, d->cap, N, Np);
+    launch_check("build_list");
+    if (d->profiling) HIP_CHECK(hipEventRecord(d->ev1, d->stream));
+    int* hf = reinterpret_cast<int*>(d->hresult);
+    HIP_CHECK(hipMemcpyAsync(hf, d->flags.p, sizeof(int), hipMemcpyDeviceToHost, d->stream));
+    HIP_CHECK(hipStreamSynchronize(d->stream));
+    if (d->profiling) {
+      float ms = 0; HIP_CHECK(hipEventElapsedTime(&ms, d->ev0, d->ev1));
+      d->listMs += ms; d->listLaunches++;
+    }
+    if (hf[0] == 0) break;
+    d->cap = ((int)(hf[0] * 1.2) + 15) & ~7;
+  }
+  d->built = true;
+}
+
+int dev_neighbor_pairs(Device* d, int max, int* out) {
+  int N = d->N, Np = d->Npad;
+  std::vector<int> nnb(N), perm(N), nbr((size_t)d->cap * Np);
+  HIP_CHECK(hipMemcpyAsync(nnb.data(), d->nnb.p, sizeof(int) * N, hipMemcpyDeviceToHost, d->stream));
+  HIP_CHECK(hipMemcpyAsync(perm.data(), d->perm.p, sizeof(int) * N, hipMemcpyDeviceToHost, d->stream));
+  HIP_CHECK(hipMemcpyAsync(nbr.data(), d->nbr.p, sizeof(int) * (size_t)d->cap * Np, hipMemcpyDeviceToHost, d->stream));
+  HIP_CHECK(hipStreamSynchronize(d->stream));
+  int n = 0;
+  for (int s = 0; s < N; s++)
+    for (int k = 0; k < nnb[s]; k++) {
+      int t = nbr[(size_t)k * Np + s];
+      int a = perm[s] + 1, b = perm[t] + 1;
+      if (a < b) {
+        if (n < max) { out[2 * n] = a; out[2 * n + 1] = b; }
+        n++;
+      }
+    }
+  return n;
+}
+
+void dev_compute_pairs(Device* d, const PairCall& c, ForceSums* out) {
+  int N = d->N, Np = d->Npad;
+  double* F = d->F.p + 3 * (size_t)Np * c.layer;
+  if (N == 0) { *out = ForceSums{ 0, 0, 0, 0 }; return; }
+  int g = div_up(N, BLOCK);
+  hipLaunchKernelGGL(k_scale, dim3(div_up(3l * Np, BLOCK)), dim3(BLOCK), 0, d->stream, d->Rs.p, d->R.p, c.L, 3 * Np);
+  const DevPair* pt = d->pairs.p + (size_t)c.layer * c.ntypes * c.ntypes;
+  const DevCoul* cp = d->couls.p + c.layer;
+  if (d->profiling) HIP_CHECK(hipEventRecord(d->ev0, d->stream));
+#define LAUNCH_PAIRS(E, MX)                                                                          \
+  hipLaunchKernelGGL((k_pairs<E, MX>), dim3(g), dim3(BLOCK), 0, d->stream, F, d->partials.p, d->Rs.p, \
+                     d->nbr.p, d->nnb.p, d->type.p, d->charge.p, d->chargedFlag.p, pt, cp, c.ntypes, c.L, c.invL2, c.Rc2, N, Np)
+  if (c.energy) { if (c.mixed) LAUNCH_PAIRS(true, true); else LAUNCH_PAIRS(true, false); }
+  else { if (c.mixed) LAUNCH_PAIRS(false, true); else LAUNCH_PAIRS(false, false); }
+#undef LAUNCH_PAIRS
+  launch_check("pairs");
+  if (d->profiling) HIP_CHECK(hipEventRecord(d->ev1, d->stream));
+  hipLaunchKernelGGL(k_reduce_partials<4>, dim3(1), dim3(BLOCK), 0, d->stream, d->result.p, d->partials.p, g);
+  launch_check("reduce");
+  HIP_CHECK(hipMemcpyAsync(d->hresult, d->result.p, 4 * sizeof(double), hipMemcpyDeviceToHost, d->stream));
+  HIP_CHECK(hipStreamSynchronize(d->stream));
+  if (d->profiling) {
+    float ms = 0; HIP_CHECK(hipEventElapsedTime(&ms, d->ev0, d->ev1));
+    d->pairMs += ms; d->pairLaunches++;
+  }
+  out->Epair = d->hresult[0];
+  out->Ecoul = d->hresult[1];
+  out->Wpair = d->hresult[2];
+  out->Wcoul = d->hresult[3];
+}
+
+void dev_boost(Device* d, int layer, double CP, double CF, bool translate, double* twoKE) {
+  int N = d->N, Np = d->Npad;
+  if (N == 0) { if (twoKE) twoKE[0] = twoKE[1] = twoKE[2] = 0; return; }
+  int g = div_up(N, BLOCK);
+  const double* F = d->F.p + 3 * (size_t)Np * layer;
+  if (twoKE) {
+    hipLaunchKernelGGL(k_boost<true>, dim3(g), dim3(BLOCK), 0, d->stream, d->P.p, d->partials.p, F, d->invMass.p, d->freeFlag.p, CP, CF, (int)translate, N, Np);
+    hipLaunchKernelGGL(k_reduce_partials<3>, dim3(1), dim3(BLOCK), 0, d->stream, d->result.p, d->partials.p, g);
+    HIP_CHECK(hipMemcpyAsync(d->hresult, d->result.p, 3 * sizeof(double), hipMemcpyDeviceToHost, d->stream));
+    HIP_CHECK(hipStreamSynchronize(d->stream));
+    for (int x = 0; x < 3; x++) twoKE[x] = d->hresult[x];
+  } else if (translate) {
+    hipLaunchKernelGGL(k_boost<false>, dim3(g), dim3(BLOCK), 0, d->stream, d->P.p, d->partials.p, F, d->invMass.p, d->freeFlag.p, CP, CF, 1, N, Np);
+  }
+  launch_check("boost");
+}
+
+void dev_kinetic(Device* d, double* twoKE) { dev_boost(d, 0, 1.0, 0.0, false, twoKE); }
+
+void dev_displace(Device* d, double CR, double CP) {
+  int N = d->N;
+  if (N == 0) return;
+  hipLaunchKernelGGL(k_displace, dim3(div_up(N, BLOCK)), dim3(BLOCK), 0, d->stream, d->R.p, d->P.p, d->invMass.p, d->freeFlag.p, CR, CP, N, d->Npad);
+  launch_check("displace");
+}
+
+void dev_set_profiling(Device* d, bool on) { d->profiling = on; d->pairMs = d->listMs = 0; d->pairLaunches = d->listLaunches = 0; }
+void dev_kernel_stats(Device* d, double* out, int n) {
+  double v[6] = { d->pairMs, (double)d->pairLaunches, d->listMs, (double)d->listLaunches, (double)d->cap, (double)d->M };
+  for (int k = 0; k < n && k < 6; k++) out[k] = v[k];
+}
+void dev_sync(Device* d) { HIP_CHECK(hipStreamSynchronize(d->stream)); }
+
+}  // namespace emdee
