@@ -1,0 +1,172 @@
+"""Throughput of the EmDee force engine on one MI355X.
+
+Workload (BASELINE config "Large LJ, FP64 plain vs FP32-pair/FP64-accumulate"):
+Lennard-Jones fluid of 1,048,576 atoms (fcc 64^3 cells) at density 0.8442,
+T = 1.0, lj_cut with rc = 2.5 and skin 0.3, velocity Verlet dt = 0.005 driven
+exactly as the reference hosts drive it (boost dt/2, displace dt, boost dt/2
+through the C ABI; energies every step).  Both arithmetic forms run: plain
+FP64 and FP32 pair arithmetic with FP64 accumulation; `value` is the mixed
+form, reported with its force RMS error against the FP64 forces on identical
+coordinates and its ratio to the plain FP64 form.
+
+The roofline entry is the pair kernel: algorithmic bytes per launch are
+N * (4 * n_half + 24 + 24) -- the int32 half Verlet list, one FP64 read of the
+positions and one FP64 write of the forces per atom -- divided by the
+kernel's mean duration from HIP events recorded on the system's stream.
+
+cpu_baseline times the CPU oracle (the plain-C restatement of the reference
+algorithm, OpenMP over the host cores) on the same system for a bounded
+number of steps.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--natoms N]
+"""
+import argparse
+import ctypes
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "atom-steps/s vs plain FP64 at stated force error; HMC effective samples/s"
+HBM_PEAK_GBS = 8000.0
+
+
+def fcc_lattice(natoms, density):
+    nc = int(np.ceil((natoms / 4) ** (1.0 / 3.0) - 1e-9))
+    a = (4.0 / density) ** (1.0 / 3.0)
+    basis = np.array([[0, 0, 0], [0.5, 0.5, 0], [0.5, 0, 0.5], [0, 0.5, 0.5]])
+    g = np.stack(np.meshgrid(np.arange(nc), np.arange(nc), np.arange(nc), indexing="ij"), -1).reshape(-1, 3)
+    R = (g[:, None, :] + basis[None, :, :]).reshape(-1, 3)[:natoms] * a
+    L = (natoms / density) ** (1.0 / 3.0)
+    return np.ascontiguousarray(R), L
+
+
+def make_system(lib, R, L, threads, rc=2.5, skin=0.3, kT=1.0, seed=86245):
+    md = lib.system(threads, 1, rc, skin, R.shape[0], None, None, None)
+    lib.EmDee_set_pair_model(md, 1, 1, lib.EmDee_pair_lj_cut(1.0, 1.0), 0.0)
+    lib.upload(md, "box", np.array([L]))
+    lib.upload(md, "coordinates", R)
+    lib.random_momenta(md, kT, True, seed)
+    return md
+
+
+def run_steps(lib, md, nsteps, dt):
+    for _ in range(nsteps):
+        md.Options.Compute = True
+        lib.boost(md, 1.0, 0.0, 0.5 * dt)
+        lib.displace(md, 1.0, 0.0, dt)
+        lib.boost(md, 1.0, 0.0, 0.5 * dt)
+
+
+def gpu_run(lib, R, L, mode, steps, warmup, dt):
+    md = make_system(lib, R, L, 1)
+    lib.EmDeeGPU_set_precision(md, mode)
+    run_steps(lib, md, warmup, dt)
+    lib.EmDeeGPU_synchronize(md)
+    lib.EmDeeGPU_profile(md, 1)
+    builds0 = md.Builds
+    t0 = time.perf_counter()
+    run_steps(lib, md, steps, dt)
+    lib.EmDeeGPU_synchronize(md)
+    t = time.perf_counter() - t0
+    st = lib.kernel_stats(md)
+    return md, dict(seconds=t, pair_ms=st[0] / max(st[1], 1), pair_launches=int(st[1]),
+                    list_ms=st[2] / max(st[3], 1), list_builds=int(st[3]), builds=md.Builds - builds0,
+                    list_entries=st[6], energy=md.Energy.Potential + md.Kinetic.Total)
+
+
+def force_error(lib, md, N):
+    """RMS force error of the mixed form against FP64 on identical coordinates."""
+    lib.EmDeeGPU_set_precision(md, 0)
+    F64 = lib.download(md, "forces", (N, 3))
+    lib.EmDeeGPU_set_precision(md, 1)
+    F32 = lib.download(md, "forces", (N, 3))
+    return float(np.linalg.norm(F32 - F64) / np.linalg.norm(F64))
+
+
+def cpu_baseline(R, L, steps, dt, threads):
+    from emdee_amd.emdee import EmDeeLibrary
+    so = os.path.join(ROOT, "oracle", "liboracle_emdee_fast.so")
+    if not os.path.exists(so):
+        import subprocess
+        subprocess.run(["make", "-C", os.path.join(ROOT, "oracle")], check=True, capture_output=True)
+    lib = EmDeeLibrary(so)
+    md = make_system(lib, R, L, threads)
+    run_steps(lib, md, 1, dt)
+    t0 = time.perf_counter()
+    run_steps(lib, md, steps, dt)
+    t = time.perf_counter() - t0
+    return R.shape[0] * steps / t, t
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=500)
+    ap.add_argument("--warmup", type=int, default=50)
+    ap.add_argument("--natoms", type=int, default=1048576)
+    ap.add_argument("--cpu-steps", type=int, default=20)
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py: this build measures one GPU; --gpus {args.gpus} is not supported "
+              "(the engine runs one system per GPU; see DESIGN.md for the multi-GPU design)", file=sys.stderr)
+        sys.exit(2)
+    os.environ.setdefault("EMDEE_QUIET", "1")
+    import emdee_amd
+    lib = emdee_amd.load()
+    N, dt = args.natoms, 0.005
+    R, L = fcc_lattice(N, 0.8442)
+
+    print(f"# fp64 run: N={N} steps={args.steps} warmup={args.warmup}", file=sys.stderr, flush=True)
+    md64, r64 = gpu_run(lib, R, L, 0, args.steps, args.warmup, dt)
+    print(f"# fp64: {r64}", file=sys.stderr, flush=True)
+    print("# mixed run", file=sys.stderr, flush=True)
+    mdmx, rmx = gpu_run(lib, R, L, 1, args.steps, args.warmup, dt)
+    print(f"# mixed: {rmx}", file=sys.stderr, flush=True)
+    err = force_error(lib, mdmx, N)
+    print(f"# force rms error (mixed vs fp64) = {err:.3e}", file=sys.stderr, flush=True)
+
+    v64 = N * args.steps / r64["seconds"]
+    vmx = N * args.steps / rmx["seconds"]
+    use_mixed = err <= 1e-5
+    best = rmx if use_mixed else r64
+    value = vmx if use_mixed else v64
+    n_half = best["list_entries"] / 2 / N
+    bytes_per_launch = N * (4 * n_half + 24 + 24)
+    achieved = bytes_per_launch / (best["pair_ms"] * 1e-3) / 1e9
+
+    cpu = None
+    if not args.no_cpu:
+        threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+        print(f"# cpu baseline: oracle, {threads} threads, {args.cpu_steps} steps", file=sys.stderr, flush=True)
+        cv, ct = cpu_baseline(R, L, args.cpu_steps, dt, threads)
+        cpu = {"value": cv, "unit": "atom-steps/s", "cores": threads, "kind": "port",
+               "build": "oracle/liboracle_emdee_fast.so (-Ofast)",
+               "sample": f"{N}-atom LJ, {args.cpu_steps} steps after 1 warm-up step ({ct:.1f} s), same set-up"}
+
+    line = {
+        "metric": METRIC, "value": value, "unit": "atom-steps/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": 1e3 * best["seconds"] / args.steps,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f32-pair/f64-accumulate" if use_mixed else "f64", "data": "synthetic",
+        "config": {"workload": "lj_fluid_fcc", "natoms": N, "density": 0.8442, "T": 1.0, "rc": 2.5,
+                   "skin": 0.3, "dt": dt, "integrator": "boost/displace/boost via C ABI, energies every step",
+                   "fp64_value": v64, "mixed_value": vmx, "mixed_over_fp64": vmx / v64,
+                   "force_rms_error": err, "list_builds": best["builds"], "half_list_pairs_per_atom": n_half},
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "kernel": "k_pairs", "kernel_ms": best["pair_ms"], "bytes_per_launch": bytes_per_launch},
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

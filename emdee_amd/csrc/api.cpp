@@ -728,7 +728,7 @@ void EmDeeGPU_set_precision(tEmDee md, int mode) {
 int EmDeeGPU_get_precision(tEmDee md) { return S(md)->precision; }
 int EmDeeGPU_neighbor_pairs(tEmDee md, int max, int* pairs) { return dev_neighbor_pairs(S(md)->dev, max, pairs); }
 void EmDeeGPU_profile(tEmDee md, int on) { dev_set_profiling(S(md)->dev, on != 0); }
-void EmDeeGPU_kernel_stats(tEmDee md, double* stats) { dev_kernel_stats(S(md)->dev, stats, 6); }
+void EmDeeGPU_kernel_stats(tEmDee md, double* stats) { dev_kernel_stats(S(md)->dev, stats, 7); }
 void EmDeeGPU_synchronize(tEmDee md) { dev_sync(S(md)->dev); }
 void EmDeeGPU_destroy(tEmDee* md) {
   if (!md || !md->Data) return;

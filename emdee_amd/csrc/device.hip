@@ -98,16 +98,17 @@ struct Device {
   int N = 0, Npad = 0, ntypes = 0, nlayers = 0;
   hipStream_t stream = nullptr;
   // per-atom SoA arrays (sorted order)
-  DBuf<double> R, P, Rs, R0, invMass, charge, F;   // F holds nlayers * 3 * Npad
+  DBuf<double> R, P, R0, invMass, charge, F;   // F holds nlayers * 3 * Npad
+  DBuf<double4> Rs4;      // {x/L, y/L, z/L, type}: one 32-byte gather per neighbor
   DBuf<int> type, body, perm, freeFlag, chargedFlag;
-  DBuf<double> tmpD;      // permutation scratch
+  DBuf<double> tmp3, tmpF, tmp1;   // permutation scratch (swapped with the arrays)
   DBuf<int> tmpI;
   // transfers
   DBuf<double> stage;     // 3N AoS, original order
   double* hstage = nullptr;  // pinned host mirror
   // cells
   int M = 0, ncells = 0;
-  DBuf<int> key, cellCount, cellStart, order, blockSums;
+  DBuf<int> key, cellCount, cellStart, order, blockSums;   // cellStart has ncells+1 entries
   // neighbor list
   int cap = 0;
   DBuf<int> nbr, nnb;
@@ -124,13 +125,13 @@ struct Device {
   DBuf<double> partials, result;
   double* hresult = nullptr;
   DBuf<double> d2;
-  DBuf<unsigned long long> trig;   // [0] max bits, [1] min orig index, [2] before bits
+  DBuf<double> trig;      // [0] max, [1] first original index holding it, [2] max before it
   // profiling
   bool profiling = false;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   double pairMs = 0, listMs = 0;
   long pairLaunches = 0, listLaunches = 0;
-  long pairsVisited = 0;
+  double listEntries = 0;
 };
 
 [[noreturn]] void fatal(const std::string& task, const std::string& msg) {
@@ -183,49 +184,97 @@ __global__ void k_gather_scalar(T* __restrict__ dst, const T* __restrict__ src, 
   if (s < N) dst[s] = src[perm[s]];
 }
 
-// Rs = R / L, exactly as EmDeeCode.f90:1228 (IEEE division).
-__global__ void k_scale(double* __restrict__ Rs, const double* __restrict__ R, double L, int n) {
-  int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n) Rs[i] = R[i] / L;
+// Rs = R / L, exactly as EmDeeCode.f90:1228 (IEEE division), packed with
+// the atom type so that a neighbor costs one 32-byte gather.
+__global__ void k_scale4(double4* __restrict__ Rs4, const double* __restrict__ R, const int* __restrict__ type,
+                         double L, int N, int Npad) {
+  int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s < N) Rs4[s] = make_double4(R[s] / L, R[Npad + s] / L, R[2 * Npad + s] / L, (double)type[s]);
 }
 
 // ---- rebuild trigger, neighbor_lists.f90:41-59 ---------------------------
 // With d_k = |R_k - R0_k|^2 in ORIGINAL atom order, the reference returns
 // max + 2 sqrt(max*next) + next where next = d_1 if atom 1 holds the maximum
 // and otherwise the maximum over atoms before the first one that attains it.
-// Three passes: (1) max, (2) first original index attaining it, (3) maximum
-// over atoms with a smaller original index.  Non-negative doubles compare as
-// their bit patterns, so the passes use 64-bit integer atomics.
-__global__ void k_disp_max(double* __restrict__ d2, const double* __restrict__ R, const double* __restrict__ R0,
-                           unsigned long long* trig, int N, int Npad) {
-  int s = blockIdx.x * blockDim.x + threadIdx.x;
-  double v = 0;
+// Pass A: per block (max, smallest original index attaining it); its
+// reduction gives the maximum and the first index k.  Pass C: per block max
+// over atoms with original index < k.  Both reductions are deterministic.
+__global__ void k_disp_a(double* __restrict__ d2, double* __restrict__ part, const double* __restrict__ R,
+                         const double* __restrict__ R0, const int* __restrict__ perm, int N, int Npad) {
+  __shared__ double smax[BLOCK / 64];
+  __shared__ int sidx[BLOCK / 64];
+  int s = blockIdx.x * BLOCK + threadIdx.x;
+  double v = -1.0;
+  int idx = INT_MAX;
   if (s < N) {
 #pragma clang fp contract(off)
     double dx = R[s] - R0[s], dy = R[Npad + s] - R0[Npad + s], dz = R[2 * Npad + s] - R0[2 * Npad + s];
     v = dx * dx + dy * dy + dz * dz;
     d2[s] = v;
+    idx = perm[s];
   }
-  for (int off = 32; off > 0; off >>= 1) v = fmax(v, __shfl_down(v, off, 64));
-  if ((threadIdx.x & 63) == 0) atomicMax(&trig[0], (unsigned long long)__double_as_longlong(v));
+  for (int off = 32; off > 0; off >>= 1) {
+    double ov = __shfl_down(v, off, 64);
+    int oi = __shfl_down(idx, off, 64);
+    if (ov > v || (ov == v && oi < idx)) { v = ov; idx = oi; }
+  }
+  int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  if (lane == 0) { smax[wid] = v; sidx[wid] = idx; }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    for (int w = 1; w < BLOCK / 64; w++)
+      if (smax[w] > v || (smax[w] == v && sidx[w] < idx)) { v = smax[w]; idx = sidx[w]; }
+    part[2 * blockIdx.x] = v;
+    part[2 * blockIdx.x + 1] = (double)idx;
+  }
 }
-__global__ void k_disp_first(const double* __restrict__ d2, const int* __restrict__ perm,
-                             unsigned long long* trig, int N) {
-  int s = blockIdx.x * blockDim.x + threadIdx.x;
-  unsigned long long mx = trig[0];
-  int idx = INT_MAX;
-  if (s < N && (unsigned long long)__double_as_longlong(d2[s]) == mx) idx = perm[s];
-  for (int off = 32; off > 0; off >>= 1) idx = min(idx, __shfl_down(idx, off, 64));
-  if ((threadIdx.x & 63) == 0 && idx != INT_MAX) atomicMin(&trig[1], (unsigned long long)idx);
+__global__ void k_disp_a_reduce(double* __restrict__ trig, const double* __restrict__ part, int nb) {
+  __shared__ double smax[BLOCK];
+  __shared__ double sidx[BLOCK];
+  double v = -1.0, idx = 2147483647.0;
+  for (int b = threadIdx.x; b < nb; b += BLOCK) {
+    double ov = part[2 * b], oi = part[2 * b + 1];
+    if (ov > v || (ov == v && oi < idx)) { v = ov; idx = oi; }
+  }
+  smax[threadIdx.x] = v; sidx[threadIdx.x] = idx;
+  __syncthreads();
+  for (int h = BLOCK / 2; h > 0; h >>= 1) {
+    if (threadIdx.x < h) {
+      double ov = smax[threadIdx.x + h], oi = sidx[threadIdx.x + h];
+      if (ov > smax[threadIdx.x] || (ov == smax[threadIdx.x] && oi < sidx[threadIdx.x])) {
+        smax[threadIdx.x] = ov; sidx[threadIdx.x] = oi;
+      }
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) { trig[0] = smax[0]; trig[1] = sidx[0]; }
 }
-__global__ void k_disp_before(const double* __restrict__ d2, const int* __restrict__ perm,
-                              unsigned long long* trig, int N) {
-  int s = blockIdx.x * blockDim.x + threadIdx.x;
-  unsigned long long k = trig[1];
-  double v = 0;
-  if (s < N && (unsigned long long)perm[s] < k) v = d2[s];
+__global__ void k_disp_c(double* __restrict__ part, const double* __restrict__ d2, const int* __restrict__ perm,
+                         const double* __restrict__ trig, int N) {
+  __shared__ double smax[BLOCK / 64];
+  int s = blockIdx.x * BLOCK + threadIdx.x;
+  double k = trig[1];
+  double v = 0.0;
+  if (s < N && (double)perm[s] < k) v = d2[s];
   for (int off = 32; off > 0; off >>= 1) v = fmax(v, __shfl_down(v, off, 64));
-  if ((threadIdx.x & 63) == 0) atomicMax(&trig[2], (unsigned long long)__double_as_longlong(v));
+  if ((threadIdx.x & 63) == 0) smax[threadIdx.x >> 6] = v;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    for (int w = 1; w < BLOCK / 64; w++) v = fmax(v, smax[w]);
+    part[blockIdx.x] = v;
+  }
+}
+__global__ void k_disp_c_reduce(double* __restrict__ trig, const double* __restrict__ part, int nb) {
+  __shared__ double smax[BLOCK];
+  double v = 0.0;
+  for (int b = threadIdx.x; b < nb; b += BLOCK) v = fmax(v, part[b]);
+  smax[threadIdx.x] = v;
+  __syncthreads();
+  for (int h = BLOCK / 2; h > 0; h >>= 1) {
+    if (threadIdx.x < h) smax[threadIdx.x] = fmax(smax[threadIdx.x], smax[threadIdx.x + h]);
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) trig[2] = smax[0];
 }
 
 // ---- cell sort ------------------------------------------------------------
@@ -235,11 +284,12 @@ __device__ __forceinline__ int cell_coord(double s, int M) {
   int c = (int)(M * (s - floor(s)));
   return c >= M ? c - M : c;
 }
-__global__ void k_cell_key(int* __restrict__ key, int* __restrict__ count, const double* __restrict__ Rs,
-                           int M, int N, int Npad) {
+__global__ void k_cell_key(int* __restrict__ key, int* __restrict__ count, const double4* __restrict__ Rs4,
+                           int M, int N) {
   int s = blockIdx.x * blockDim.x + threadIdx.x;
   if (s >= N) return;
-  int ix = cell_coord(Rs[s], M), iy = cell_coord(Rs[Npad + s], M), iz = cell_coord(Rs[2 * Npad + s], M);
+  double4 r = Rs4[s];
+  int ix = cell_coord(r.x, M), iy = cell_coord(r.y, M), iz = cell_coord(r.z, M);
   int c = ix + M * (iy + M * iz);
   key[s] = c;
   atomicAdd(&count[c], 1);
@@ -310,129 +360,208 @@ __global__ void k_permute(T* __restrict__ dst, const T* __restrict__ src, const 
   int s = blockIdx.x * blockDim.x + threadIdx.x;
   if (s < N) dst[s] = src[order[s]];
 }
+// ncomp consecutive [Npad] components at once
+__global__ void k_permute_n(double* __restrict__ dst, const double* __restrict__ src, const int* __restrict__ order,
+                            int ncomp, int N, int Npad) {
+  int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= N) return;
+  int o = order[s];
+  for (int c = 0; c < ncomp; c++) dst[(size_t)c * Npad + s] = src[(size_t)c * Npad + o];
+}
 
 // ---- Verlet list ------------------------------------------------------------
 // Full list: for slot i, every j != i in the 125 cells within +-2 cells with
 // r^2 < (rc+skin)^2/L^2 computed as in neighbor_lists.f90:244 (FP64,
 // minimum image by anint, no FMA contraction), that is not excluded, not in
 // the same rigid body and whose types interact (neighbor_lists.f90:250-259).
-__global__ void __launch_bounds__(BLOCK)
-k_build_list(int* __restrict__ nbr, int* __restrict__ nnb, int* __restrict__ flags,
-             const double* __restrict__ Rs, const int* __restrict__ key, const int* __restrict__ start,
-             const int* __restrict__ type, const int* __restrict__ body, const int* __restrict__ perm,
-             const uint8_t* __restrict__ interact, const int* __restrict__ exOff, const int* __restrict__ exItem,
-             int nexcl, int ntypes, int M, int ncells, double xRc2, int cap, int N, int Npad) {
-  int i = xcd_block() * BLOCK + threadIdx.x;
-  if (i >= N) return;
-  double xi = Rs[i], yi = Rs[Npad + i], zi = Rs[2 * Npad + i];
-  int c = key[i];
-  int cx = c % M, cy = (c / M) % M, cz = c / (M * M);
-  int ti = type[i], bi = body[i];
-  int oi = perm[i];
-  int e0 = 0, e1 = 0;
-  if (nexcl) { e0 = exOff[oi]; e1 = exOff[oi + 1]; }
-  int cnt = 0;
-  for (int dz = -2; dz <= 2; dz++) {
-    int z = cz + dz; z = z < 0 ? z + M : (z >= M ? z - M : z);
-    for (int dy = -2; dy <= 2; dy++) {
-      int y = cy + dy; y = y < 0 ? y + M : (y >= M ? y - M : y);
-      for (int dx = -2; dx <= 2; dx++) {
-        int x = cx + dx; x = x < 0 ? x + M : (x >= M ? x - M : x);
-        int cc = x + M * (y + M * z);
-        int a = start[cc], b = (cc + 1 < ncells) ? start[cc + 1] : N;
-        for (int j = a; j < b; j++) {
-          if (j == i) continue;
-          double r2;
-          {
+// Slots are sorted by cell with x fastest, so for each of the 25 (dy, dz)
+// offsets the five cells x-2..x+2 form one contiguous slot range (two when
+// the row wraps); lanes of a wave (neighbouring slots) scan nearly the same
+// ranges, which keeps the gathers in L1.
+__device__ __forceinline__ void scan_range(int a, int b, int i, double xi, double yi, double zi, int ti, int bi, int oi,
+                                           const double4* __restrict__ Rs4, const int* __restrict__ body,
+                                           const int* __restrict__ perm, const uint8_t* __restrict__ interact,
+                                           const int* __restrict__ exItem, int e0, int e1, int ntypes, double xRc2,
+                                           int cap, int Npad, int* __restrict__ nbr, int& cnt) {
+  for (int j = a; j < b; j++) {
+    double4 rj = Rs4[j];
+    double r2;
+    {
 #pragma clang fp contract(off)
-            double d0 = xi - Rs[j], d1 = yi - Rs[Npad + j], d2 = zi - Rs[2 * Npad + j];
-            d0 = d0 - round(d0); d1 = d1 - round(d1); d2 = d2 - round(d2);
-            r2 = d0 * d0 + d1 * d1 + d2 * d2;
-          }
-          if (r2 < xRc2) {
-            if (body[j] == bi) continue;
-            if (!interact[type[j] * ntypes + ti]) continue;
-            if (nexcl) {
-              int oj = perm[j];
-              bool ex = false;
-              for (int e = e0; e < e1; e++) if (exItem[e] == oj) { ex = true; break; }
-              if (ex) continue;
-            }
-            if (cnt < cap) nbr[(size_t)cnt * Npad + i] = j;
-            cnt++;
-          }
+      double d0 = xi - rj.x, d1 = yi - rj.y, d2 = zi - rj.z;
+      d0 = d0 - rint(d0); d1 = d1 - rint(d1); d2 = d2 - rint(d2);
+      r2 = d0 * d0 + d1 * d1 + d2 * d2;
+    }
+    if (r2 < xRc2 && j != i) {
+      if (body[j] == bi) continue;
+      if (!interact[(int)rj.w * ntypes + ti]) continue;
+      if (e1 > e0) {
+        int oj = perm[j];
+        bool ex = false;
+        for (int e = e0; e < e1; e++) if (exItem[e] == oj) { ex = true; break; }
+        if (ex) continue;
+      }
+      if (cnt < cap) nbr[(size_t)cnt * Npad + i] = j;
+      cnt++;
+    }
+  }
+}
+__global__ void __launch_bounds__(BLOCK)
+k_build_list(int* __restrict__ nbr, int* __restrict__ nnb, int* __restrict__ flags, double* __restrict__ part,
+             const double4* __restrict__ Rs4, const int* __restrict__ key, const int* __restrict__ start,
+             const int* __restrict__ body, const int* __restrict__ perm,
+             const uint8_t* __restrict__ interact, const int* __restrict__ exOff, const int* __restrict__ exItem,
+             int nexcl, int ntypes, int M, double xRc2, int cap, int N, int Npad) {
+  int blk = xcd_block();
+  int i = blk * BLOCK + threadIdx.x;
+  int cnt = 0;
+  if (i < N) {
+    double4 ri = Rs4[i];
+    int c = key[i];
+    int cx = c % M, cy = (c / M) % M, cz = c / (M * M);
+    int ti = (int)ri.w, bi = body[i], oi = perm[i];
+    int e0 = 0, e1 = 0;
+    if (nexcl) { e0 = exOff[oi]; e1 = exOff[oi + 1]; }
+    int xa = cx - 2, xb = cx + 2;
+    for (int dz = -2; dz <= 2; dz++) {
+      int z = cz + dz; z = z < 0 ? z + M : (z >= M ? z - M : z);
+      for (int dy = -2; dy <= 2; dy++) {
+        int y = cy + dy; y = y < 0 ? y + M : (y >= M ? y - M : y);
+        int base = M * (y + M * z);
+        if (xa >= 0 && xb < M) {
+          scan_range(start[base + xa], start[base + xb + 1], i, ri.x, ri.y, ri.z, ti, bi, oi, Rs4, body, perm, interact,
+                     exItem, e0, e1, ntypes, xRc2, cap, Npad, nbr, cnt);
+        } else {
+          int a0 = xa < 0 ? xa + M : xa, b1 = xb >= M ? xb - M : xb;
+          scan_range(start[base + a0], start[base + M], i, ri.x, ri.y, ri.z, ti, bi, oi, Rs4, body, perm, interact,
+                     exItem, e0, e1, ntypes, xRc2, cap, Npad, nbr, cnt);
+          scan_range(start[base], start[base + b1 + 1], i, ri.x, ri.y, ri.z, ti, bi, oi, Rs4, body, perm, interact,
+                     exItem, e0, e1, ntypes, xRc2, cap, Npad, nbr, cnt);
         }
       }
     }
+    nnb[i] = cnt;
+    if (cnt > cap) atomicMax(&flags[0], cnt);
   }
-  nnb[i] = cnt;
-  if (cnt > cap) atomicMax(&flags[0], cnt);
+  double v[1] = { (double)cnt };
+  block_sum<1>(v);
+  if (threadIdx.x == 0) part[blk] = v[0];
 }
 
 // ---- pair forces ----------------------------------------------------------
 // One thread per atom over its full list; the per-pair body is
 // compute.f90:46-92.  Energies and virials of a full list count every pair
 // twice and are halved at the end (exact).
-template <bool ENERGY, bool MIXED>
+//
+// Minimum image: the reference subtracts anint(d) (half away from zero); rint
+// (half to even, one v_rndne_f64) differs only at exact ties |d| = k + 1/2,
+// where both give |d - n| = 1/2 -- the same r^2 -- and the pair is then at
+// least L/2 apart, i.e. beyond any cutoff, so no force differs.
+
+__device__ __forceinline__ double fast_rcp(double x) {
+  // v_rcp_f64 + two Newton steps: within 1 ulp of 1/x
+  double y = __builtin_amdgcn_rcp(x);
+  double e = fma(-x, y, 1.0);
+  y = fma(y, e, y);
+  e = fma(-x, y, 1.0);
+  return fma(y, e, y);
+}
+
+struct PairAcc { double fx, fy, fz, Ep, Ec, Wp, Wc; };
+
+template <typename T, bool ENERGY>
+__device__ __forceinline__ void pair_body(const DevPair& p, const DevCoul& cm, bool ijcharged, double qiqj,
+                                          double dx, double dy, double dz, double r2, double invL2, PairAcc& a) {
+  T invR2;
+  if (std::is_same<T, float>::value) invR2 = T(invL2) * rcp<T>(T(r2));
+  else invR2 = T(invL2 * fast_rcp(r2));
+  T E, W;
+  pair_model<T>(p, invR2, E, W);
+  if (p.modifier != MOD_NONE) {
+    T invR = (p.modifier == MOD_SHIFTED_FORCE || p.modifier == MOD_SMOOTHED || p.modifier == MOD_SHIFTED_SMOOTHED)
+                 ? T(sqrt(invR2)) : T(0);
+    apply_modifier<T>(p.modifier, T(p.eshift), T(p.fshift), T(p.factor), T(p.Rm2fac), invR, invR2, E, W);
+  }
+  if (ENERGY) a.Ep += (double)E;
+  a.Wp += (double)W;
+  T Wsum = W;
+  if (ijcharged) {
+    T invR = T(sqrt(invR2));
+    coul_model<T, ENERGY>(cm, invR, invR2, E, W);
+    apply_modifier<T>(cm.modifier, T(cm.eshift), T(cm.fshift), T(cm.factor), T(cm.Rm2fac), invR, invR2, E, W);
+    if (ENERGY) a.Ec += qiqj * (double)E;
+    T Wq = T(qiqj) * W;
+    a.Wc += (double)Wq;
+    Wsum = Wsum + Wq;
+  }
+  T g = Wsum * invR2;
+  a.fx += (double)(g * T(dx));
+  a.fy += (double)(g * T(dy));
+  a.fz += (double)(g * T(dz));
+}
+
+// UNIFORM: a single atom type, so the model record is loaded once.
+template <bool ENERGY, bool MIXED, bool UNIFORM>
 __global__ void __launch_bounds__(BLOCK)
 k_pairs(double* __restrict__ F, double* __restrict__ partials,
-        const double* __restrict__ Rs, const int* __restrict__ nbr, const int* __restrict__ nnb,
-        const int* __restrict__ type, const double* __restrict__ charge, const int* __restrict__ charged,
+        const double4* __restrict__ Rs4, const int* __restrict__ nbr, const int* __restrict__ nnb,
+        const double* __restrict__ charge, const int* __restrict__ charged,
         const DevPair* __restrict__ pairs, const DevCoul* __restrict__ coulp,
         int ntypes, double L, double invL2, double Rc2, int N, int Npad) {
   using T = typename std::conditional<MIXED, float, double>::type;
   int blk = xcd_block();
   int i = blk * BLOCK + threadIdx.x;
-  double Ep = 0, Ec = 0, Wp = 0, Wc = 0;
+  PairAcc a = { 0, 0, 0, 0, 0, 0, 0 };
   if (i < N) {
     const DevCoul cm = *coulp;
-    double xi = Rs[i], yi = Rs[Npad + i], zi = Rs[2 * Npad + i];
-    int ti = type[i];
+    const double4 ri = Rs4[i];
+    int ti = (int)ri.w;
     double qi = charge[i];
-    int ich = charged[i];
+    bool ich = charged[i] != 0;
     const DevPair* prow = pairs + ti * ntypes;
-    double fx = 0, fy = 0, fz = 0;
-    int n = nnb[i];
-    for (int k = 0; k < n; k++) {
-      int j = nbr[(size_t)k * Npad + i];
-      double dx = xi - Rs[j], dy = yi - Rs[Npad + j], dz = zi - Rs[2 * Npad + j];
-      dx = dx - round(dx); dy = dy - round(dy); dz = dz - round(dz);
+    DevPair pu;
+    if (UNIFORM) pu = prow[0];
+    const int n = nnb[i];
+    const int* __restrict__ col = nbr + i;
+    auto one = [&](int j, const double4& rj) {
+      double dx = ri.x - rj.x, dy = ri.y - rj.y, dz = ri.z - rj.z;
+      dx = dx - rint(dx); dy = dy - rint(dy); dz = dz - rint(dz);
       double r2;
       {
 #pragma clang fp contract(off)
         r2 = dx * dx + dy * dy + dz * dz;
       }
       if (r2 < Rc2) {
-        int tj = type[j];
-        const DevPair p = prow[tj];
-        T invR2 = MIXED ? T(invL2) * rcp<T>(T(r2)) : T(invL2 / r2);
-        T invR = sqrt(invR2);
-        T E, W;
-        pair_model<T>(p, invR2, E, W);
-        apply_modifier<T>(p.modifier, T(p.eshift), T(p.fshift), T(p.factor), T(p.Rm2fac), invR, invR2, E, W);
-        if (ENERGY) Ep += (double)E;
-        Wp += (double)W;
-        T Wsum = W;
-        if (ich && charged[j] && p.coulomb) {
-          coul_model<T, ENERGY>(cm, invR, invR2, E, W);
-          apply_modifier<T>(cm.modifier, T(cm.eshift), T(cm.fshift), T(cm.factor), T(cm.Rm2fac), invR, invR2, E, W);
-          double qiqj = p.kCoul * qi * charge[j];
-          if (ENERGY) Ec += qiqj * (double)E;
-          T Wq = T(qiqj) * W;
-          Wc += (double)Wq;
-          Wsum = Wsum + Wq;
+        if (UNIFORM) {
+          bool jc = ich && pu.coulomb && charged[j];
+          double qiqj = jc ? pu.kCoul * qi * charge[j] : 0.0;
+          pair_body<T, ENERGY>(pu, cm, jc, qiqj, dx, dy, dz, r2, invL2, a);
+        } else {
+          const DevPair p = prow[(int)rj.w];
+          bool jc = ich && p.coulomb && charged[j];
+          double qiqj = jc ? p.kCoul * qi * charge[j] : 0.0;
+          pair_body<T, ENERGY>(p, cm, jc, qiqj, dx, dy, dz, r2, invL2, a);
         }
-        T g = Wsum * invR2;
-        fx += (double)(g * T(dx));
-        fy += (double)(g * T(dy));
-        fz += (double)(g * T(dz));
       }
+    };
+    int k = 0;
+    for (; k + 4 <= n; k += 4) {
+      int j0 = __builtin_nontemporal_load(col + (size_t)(k + 0) * Npad);
+      int j1 = __builtin_nontemporal_load(col + (size_t)(k + 1) * Npad);
+      int j2 = __builtin_nontemporal_load(col + (size_t)(k + 2) * Npad);
+      int j3 = __builtin_nontemporal_load(col + (size_t)(k + 3) * Npad);
+      double4 r0 = Rs4[j0], r1 = Rs4[j1], r2 = Rs4[j2], r3 = Rs4[j3];
+      one(j0, r0); one(j1, r1); one(j2, r2); one(j3, r3);
     }
-    F[i] = L * fx;
-    F[Npad + i] = L * fy;
-    F[2 * Npad + i] = L * fz;
+    for (; k < n; k++) {
+      int j = __builtin_nontemporal_load(col + (size_t)k * Npad);
+      one(j, Rs4[j]);
+    }
+    F[i] = L * a.fx;
+    F[Npad + i] = L * a.fy;
+    F[2 * Npad + i] = L * a.fz;
   }
-  double v[4] = { Ep, Ec, Wp, Wc };
+  double v[4] = { a.Ep, a.Ec, a.Wp, a.Wc };
   block_sum<4>(v);
   if (threadIdx.x == 0) {
     partials[4 * blk + 0] = 0.5 * v[0];
@@ -510,11 +639,11 @@ Device* dev_create(const DeviceSetup& s) {
   HIP_CHECK(hipStreamCreateWithFlags(&d->stream, hipStreamNonBlocking));
   HIP_CHECK(hipEventCreate(&d->ev0));
   HIP_CHECK(hipEventCreate(&d->ev1));
-  d->R.alloc(3 * (size_t)Np); d->P.alloc(3 * (size_t)Np); d->Rs.alloc(3 * (size_t)Np); d->R0.alloc(3 * (size_t)Np);
+  d->R.alloc(3 * (size_t)Np); d->P.alloc(3 * (size_t)Np); d->Rs4.alloc(Np); d->R0.alloc(3 * (size_t)Np);
   d->F.alloc(3 * (size_t)Np * s.nlayers);
   d->invMass.alloc(Np); d->charge.alloc(Np);
   d->type.alloc(Np); d->body.alloc(Np); d->perm.alloc(Np); d->freeFlag.alloc(Np); d->chargedFlag.alloc(Np);
-  d->tmpD.alloc(3 * (size_t)Np); d->tmpI.alloc(Np);
+  d->tmp3.alloc(3 * (size_t)Np); d->tmpF.alloc(3 * (size_t)Np * s.nlayers); d->tmp1.alloc(Np); d->tmpI.alloc(Np);
   d->stage.alloc(3 * (size_t)std::max(s.N, 1));
   HIP_CHECK(hipHostMalloc(&d->hstage, 3 * sizeof(double) * std::max(s.N, 1), hipHostMallocDefault));
   HIP_CHECK(hipHostMalloc(&d->hresult, 64 * sizeof(double), hipHostMallocDefault));
@@ -522,6 +651,7 @@ Device* dev_create(const DeviceSetup& s) {
   d->partials.alloc(8 * (size_t)(Np / BLOCK + 1));
   d->d2.alloc(Np);
   d->trig.alloc(4);
+  HIP_CHECK(hipMemsetAsync(d->Rs4.p, 0, sizeof(double4) * Np, d->stream));
   d->key.alloc(Np); d->order.alloc(Np); d->nnb.alloc(Np); d->flags.alloc(4);
   HIP_CHECK(hipMemsetAsync(d->R.p, 0, 3 * sizeof(double) * Np, d->stream));
   HIP_CHECK(hipMemsetAsync(d->P.p, 0, 3 * sizeof(double) * Np, d->stream));
@@ -556,9 +686,10 @@ Device* dev_create(const DeviceSetup& s) {
 void dev_destroy(Device* d) {
   if (!d) return;
   (void)hipStreamSynchronize(d->stream);
-  for (DBuf<double>* b : { &d->R, &d->P, &d->Rs, &d->R0, &d->invMass, &d->charge, &d->F, &d->tmpD, &d->stage, &d->partials, &d->result, &d->d2 }) b->release();
+  for (DBuf<double>* b : { &d->R, &d->P, &d->R0, &d->invMass, &d->charge, &d->F, &d->tmp3, &d->tmpF, &d->tmp1, &d->stage, &d->partials, &d->result, &d->d2, &d->trig }) b->release();
+  d->Rs4.release();
   for (DBuf<int>* b : { &d->type, &d->body, &d->perm, &d->freeFlag, &d->chargedFlag, &d->tmpI, &d->key, &d->cellCount, &d->cellStart, &d->order, &d->blockSums, &d->nbr, &d->nnb, &d->flags, &d->exOff, &d->exItem }) b->release();
-  d->pairs.release(); d->couls.release(); d->interact.release(); d->trig.release();
+  d->pairs.release(); d->couls.release(); d->interact.release();
   if (d->hstage) (void)hipHostFree(d->hstage);
   if (d->hresult) (void)hipHostFree(d->hresult);
   (void)hipEventDestroy(d->ev0); (void)hipEventDestroy(d->ev1);
@@ -646,20 +777,14 @@ bool dev_needs_rebuild(Device* d, double skinSq) {
   int N = d->N;
   if (N == 0) return false;
   int g = div_up(N, BLOCK);
-  unsigned long long* init = reinterpret_cast<unsigned long long*>(d->hresult + 32);
-  init[0] = 0ull; init[1] = (unsigned long long)INT_MAX; init[2] = 0ull; init[3] = 0ull;
-  HIP_CHECK(hipMemcpyAsync(d->trig.p, init, 4 * sizeof(unsigned long long), hipMemcpyHostToDevice, d->stream));
-  hipLaunchKernelGGL(k_disp_max, dim3(g), dim3(BLOCK), 0, d->stream, d->d2.p, d->R.p, d->R0.p, d->trig.p, N, d->Npad);
-  hipLaunchKernelGGL(k_disp_first, dim3(g), dim3(BLOCK), 0, d->stream, d->d2.p, d->perm.p, d->trig.p, N);
-  hipLaunchKernelGGL(k_disp_before, dim3(g), dim3(BLOCK), 0, d->stream, d->d2.p, d->perm.p, d->trig.p, N);
+  hipLaunchKernelGGL(k_disp_a, dim3(g), dim3(BLOCK), 0, d->stream, d->d2.p, d->partials.p, d->R.p, d->R0.p, d->perm.p, N, d->Npad);
+  hipLaunchKernelGGL(k_disp_a_reduce, dim3(1), dim3(BLOCK), 0, d->stream, d->trig.p, d->partials.p, g);
+  hipLaunchKernelGGL(k_disp_c, dim3(g), dim3(BLOCK), 0, d->stream, d->partials.p, d->d2.p, d->perm.p, d->trig.p, N);
+  hipLaunchKernelGGL(k_disp_c_reduce, dim3(1), dim3(BLOCK), 0, d->stream, d->trig.p, d->partials.p, g);
   launch_check("trigger");
-  unsigned long long* h = reinterpret_cast<unsigned long long*>(d->hresult);
-  HIP_CHECK(hipMemcpyAsync(h, d->trig.p, 3 * sizeof(unsigned long long), hipMemcpyDeviceToHost, d->stream));
+  HIP_CHECK(hipMemcpyAsync(d->hresult, d->trig.p, 3 * sizeof(double), hipMemcpyDeviceToHost, d->stream));
   HIP_CHECK(hipStreamSynchronize(d->stream));
-  double maximum, before, next;
-  memcpy(&maximum, &h[0], 8);
-  memcpy(&before, &h[2], 8);
-  next = (h[1] == 0ull) ? maximum : before;
+  double maximum = d->hresult[0], next = (d->hresult[1] == 0.0) ? maximum : d->hresult[2];
   double approach = maximum + 2 * std::sqrt(maximum * next) + next;
   return approach > skinSq;
 }
@@ -668,9 +793,9 @@ static void ensure_cells(Device* d, int M) {
   if (M == d->M) return;
   d->M = M;
   d->ncells = M * M * M;
-  d->cellCount.alloc(d->ncells);
-  d->cellStart.alloc(d->ncells);
-  d->blockSums.alloc(div_up(d->ncells, BLOCK) + 1);
+  d->cellCount.alloc(d->ncells + 1);
+  d->cellStart.alloc(d->ncells + 1);
+  d->blockSums.alloc(div_up(d->ncells + 1, BLOCK) + 1);
 }
 
 void dev_rebuild(Device* d, double L, double xRc, double xInRc) {
@@ -682,39 +807,39 @@ void dev_rebuild(Device* d, double L, double xRc, double xInRc) {
   if (M < 5) M = 5;
   if ((long)M * M * M > 64l * 1024 * 1024) M = 400;   // bound the cell table
   ensure_cells(d, M);
-  int nc = d->ncells, gc = div_up(nc, BLOCK);
+  int nc = d->ncells, nc1 = nc + 1, gc = div_up(nc1, BLOCK);
   // current scaled positions
-  hipLaunchKernelGGL(k_scale, dim3(div_up(3l * Np, BLOCK)), dim3(BLOCK), 0, d->stream, d->Rs.p, d->R.p, L, 3 * Np);
-  // counting sort by cell
-  HIP_CHECK(hipMemsetAsync(d->cellCount.p, 0, sizeof(int) * nc, d->stream));
-  hipLaunchKernelGGL(k_cell_key, dim3(g), dim3(BLOCK), 0, d->stream, d->key.p, d->cellCount.p, d->Rs.p, M, N, Np);
-  hipLaunchKernelGGL(k_scan_block, dim3(gc), dim3(BLOCK), 0, d->stream, d->cellStart.p, d->blockSums.p, d->cellCount.p, nc);
+  hipLaunchKernelGGL(k_scale4, dim3(g), dim3(BLOCK), 0, d->stream, d->Rs4.p, d->R.p, d->type.p, L, N, Np);
+  // counting sort by cell: count, exclusive scan over ncells+1 entries
+  // (start[ncells] = N), scatter, order inside cells
+  HIP_CHECK(hipMemsetAsync(d->cellCount.p, 0, sizeof(int) * nc1, d->stream));
+  hipLaunchKernelGGL(k_cell_key, dim3(g), dim3(BLOCK), 0, d->stream, d->key.p, d->cellCount.p, d->Rs4.p, M, N);
+  hipLaunchKernelGGL(k_scan_block, dim3(gc), dim3(BLOCK), 0, d->stream, d->cellStart.p, d->blockSums.p, d->cellCount.p, nc1);
   hipLaunchKernelGGL(k_scan_sums, dim3(1), dim3(BLOCK), 0, d->stream, d->blockSums.p, gc, (int*)nullptr);
-  hipLaunchKernelGGL(k_scan_add, dim3(gc), dim3(BLOCK), 0, d->stream, d->cellStart.p, d->blockSums.p, nc);
-  HIP_CHECK(hipMemsetAsync(d->cellCount.p, 0, sizeof(int) * nc, d->stream));
+  hipLaunchKernelGGL(k_scan_add, dim3(gc), dim3(BLOCK), 0, d->stream, d->cellStart.p, d->blockSums.p, nc1);
+  HIP_CHECK(hipMemsetAsync(d->cellCount.p, 0, sizeof(int) * nc1, d->stream));
   hipLaunchKernelGGL(k_cell_scatter, dim3(g), dim3(BLOCK), 0, d->stream, d->order.p, d->cellCount.p, d->key.p, d->cellStart.p, N);
-  hipLaunchKernelGGL(k_cell_sort, dim3(gc), dim3(BLOCK), 0, d->stream, d->order.p, d->cellStart.p, d->perm.p, nc, N);
+  hipLaunchKernelGGL(k_cell_sort, dim3(div_up(nc, BLOCK)), dim3(BLOCK), 0, d->stream, d->order.p, d->cellStart.p, d->perm.p, nc, N);
   launch_check("cell sort");
-  // permute every per-atom array into the new order
-  auto permD = [&](double* a) {
-    hipLaunchKernelGGL(k_permute<double>, dim3(g), dim3(BLOCK), 0, d->stream, d->tmpD.p, a, d->order.p, N);
-    HIP_CHECK(hipMemcpyAsync(a, d->tmpD.p, sizeof(double) * N, hipMemcpyDeviceToDevice, d->stream));
+  // permute every per-atom array into the new order (write to scratch, swap)
+  auto permN = [&](DBuf<double>& a, DBuf<double>& tmp, int ncomp) {
+    hipLaunchKernelGGL(k_permute_n, dim3(g), dim3(BLOCK), 0, d->stream, tmp.p, a.p, d->order.p, ncomp, N, Np);
+    std::swap(a.p, tmp.p); std::swap(a.n, tmp.n);
   };
-  auto permI = [&](int* a) {
-    hipLaunchKernelGGL(k_permute<int>, dim3(g), dim3(BLOCK), 0, d->stream, d->tmpI.p, a, d->order.p, N);
-    HIP_CHECK(hipMemcpyAsync(a, d->tmpI.p, sizeof(int) * N, hipMemcpyDeviceToDevice, d->stream));
+  auto permI = [&](DBuf<int>& a) {
+    hipLaunchKernelGGL(k_permute<int>, dim3(g), dim3(BLOCK), 0, d->stream, d->tmpI.p, a.p, d->order.p, N);
+    std::swap(a.p, d->tmpI.p); std::swap(a.n, d->tmpI.n);
   };
-  for (int x = 0; x < 3; x++) {
-    permD(d->R.p + (size_t)x * Np);
-    permD(d->P.p + (size_t)x * Np);
-    permD(d->Rs.p + (size_t)x * Np);
-    for (int l = 0; l < d->nlayers; l++) permD(d->F.p + 3 * (size_t)Np * l + (size_t)x * Np);
-  }
-  permD(d->invMass.p); permD(d->charge.p);
-  permI(d->type.p); permI(d->body.p); permI(d->perm.p); permI(d->freeFlag.p); permI(d->chargedFlag.p);
-  // keys in the new order (a stable property: recomputed from permuted Rs)
-  HIP_CHECK(hipMemsetAsync(d->cellCount.p, 0, sizeof(int) * nc, d->stream));
-  hipLaunchKernelGGL(k_cell_key, dim3(g), dim3(BLOCK), 0, d->stream, d->key.p, d->cellCount.p, d->Rs.p, M, N, Np);
+  permN(d->R, d->tmp3, 3);
+  permN(d->P, d->tmp3, 3);
+  permN(d->F, d->tmpF, 3 * d->nlayers);
+  permN(d->invMass, d->tmp1, 1);
+  permN(d->charge, d->tmp1, 1);
+  permI(d->type); permI(d->body); permI(d->perm); permI(d->freeFlag); permI(d->chargedFlag);
+  launch_check("permute");
+  // scaled positions and keys in the new order (Rs4 is a function of R, type)
+  hipLaunchKernelGGL(k_scale4, dim3(g), dim3(BLOCK), 0, d->stream, d->Rs4.p, d->R.p, d->type.p, L, N, Np);
+  hipLaunchKernelGGL(k_cell_key, dim3(g), dim3(BLOCK), 0, d->stream, d->key.p, d->cellCount.p, d->Rs4.p, M, N);
   // R0 = R
   HIP_CHECK(hipMemcpyAsync(d->R0.p, d->R.p, 3 * sizeof(double) * Np, hipMemcpyDeviceToDevice, d->stream));
   // Verlet list, growing the ELL capacity on overflow
@@ -728,12 +853,14 @@ void dev_rebuild(Device* d, double L, double xRc, double xInRc) {
     d->nbr.alloc((size_t)d->cap * Np);
     HIP_CHECK(hipMemsetAsync(d->flags.p, 0, sizeof(int) * 4, d->stream));
     if (d->profiling) HIP_CHECK(hipEventRecord(d->ev0, d->stream));
-    hipLaunchKernelGGL(k_build_list, dim3(g), dim3(BLOCK), 0, d->stream, d->nbr.p, d->nnb.p, d->flags.p, d->Rs.p, d->key.p,
-                       d->cellStart.p, d->type.p, d->body.p, d->perm.p, d->interact.p, d->exOff.p, d->exItem.p,
-                       d->nexcl, d->ntypes, M, nc, xRc2, d->cap, N, Np);
+    hipLaunchKernelGGL(k_build_list, dim3(g), dim3(BLOCK), 0, d->stream, d->nbr.p, d->nnb.p, d->flags.p, d->partials.p,
+                       d->Rs4.p, d->key.p, d->cellStart.p, d->body.p, d->perm.p, d->interact.p, d->exOff.p, d->exItem.p,
+                       d->nexcl, d->ntypes, M, xRc2, d->cap, N, Np);
     launch_check("build_list");
     if (d->profiling) HIP_CHECK(hipEventRecord(d->ev1, d->stream));
-    int* hf = reinterpret_cast<int*>(d->hresult);
+    hipLaunchKernelGGL(k_reduce_partials<1>, dim3(1), dim3(BLOCK), 0, d->stream, d->result.p + 8, d->partials.p, g);
+    HIP_CHECK(hipMemcpyAsync(d->hresult + 8, d->result.p + 8, sizeof(double), hipMemcpyDeviceToHost, d->stream));
+    int* hf = reinterpret_cast<int*>(d->hresult + 16);
     HIP_CHECK(hipMemcpyAsync(hf, d->flags.p, sizeof(int), hipMemcpyDeviceToHost, d->stream));
     HIP_CHECK(hipStreamSynchronize(d->stream));
     if (d->profiling) {
@@ -743,6 +870,7 @@ void dev_rebuild(Device* d, double L, double xRc, double xInRc) {
     if (hf[0] == 0) break;
     d->cap = ((int)(hf[0] * 1.2) + 15) & ~7;
   }
+  d->listEntries = d->hresult[8];
   d->built = true;
 }
 
@@ -771,15 +899,21 @@ void dev_compute_pairs(Device* d, const PairCall& c, ForceSums* out) {
   double* F = d->F.p + 3 * (size_t)Np * c.layer;
   if (N == 0) { *out = ForceSums{ 0, 0, 0, 0 }; return; }
   int g = div_up(N, BLOCK);
-  hipLaunchKernelGGL(k_scale, dim3(div_up(3l * Np, BLOCK)), dim3(BLOCK), 0, d->stream, d->Rs.p, d->R.p, c.L, 3 * Np);
+  hipLaunchKernelGGL(k_scale4, dim3(g), dim3(BLOCK), 0, d->stream, d->Rs4.p, d->R.p, d->type.p, c.L, N, Np);
   const DevPair* pt = d->pairs.p + (size_t)c.layer * c.ntypes * c.ntypes;
   const DevCoul* cp = d->couls.p + c.layer;
   if (d->profiling) HIP_CHECK(hipEventRecord(d->ev0, d->stream));
-#define LAUNCH_PAIRS(E, MX)                                                                          \
-  hipLaunchKernelGGL((k_pairs<E, MX>), dim3(g), dim3(BLOCK), 0, d->stream, F, d->partials.p, d->Rs.p, \
-                     d->nbr.p, d->nnb.p, d->type.p, d->charge.p, d->chargedFlag.p, pt, cp, c.ntypes, c.L, c.invL2, c.Rc2, N, Np)
-  if (c.energy) { if (c.mixed) LAUNCH_PAIRS(true, true); else LAUNCH_PAIRS(true, false); }
-  else { if (c.mixed) LAUNCH_PAIRS(false, true); else LAUNCH_PAIRS(false, false); }
+#define LAUNCH_PAIRS(E, MX, U)                                                                          \
+  hipLaunchKernelGGL((k_pairs<E, MX, U>), dim3(g), dim3(BLOCK), 0, d->stream, F, d->partials.p, d->Rs4.p, \
+                     d->nbr.p, d->nnb.p, d->charge.p, d->chargedFlag.p, pt, cp, c.ntypes, c.L, c.invL2, c.Rc2, N, Np)
+  bool U = c.ntypes == 1;
+  if (c.energy) {
+    if (c.mixed) { if (U) LAUNCH_PAIRS(true, true, true); else LAUNCH_PAIRS(true, true, false); }
+    else { if (U) LAUNCH_PAIRS(true, false, true); else LAUNCH_PAIRS(true, false, false); }
+  } else {
+    if (c.mixed) { if (U) LAUNCH_PAIRS(false, true, true); else LAUNCH_PAIRS(false, true, false); }
+    else { if (U) LAUNCH_PAIRS(false, false, true); else LAUNCH_PAIRS(false, false, false); }
+  }
 #undef LAUNCH_PAIRS
   launch_check("pairs");
   if (d->profiling) HIP_CHECK(hipEventRecord(d->ev1, d->stream));
@@ -825,8 +959,9 @@ void dev_displace(Device* d, double CR, double CP) {
 
 void dev_set_profiling(Device* d, bool on) { d->profiling = on; d->pairMs = d->listMs = 0; d->pairLaunches = d->listLaunches = 0; }
 void dev_kernel_stats(Device* d, double* out, int n) {
-  double v[6] = { d->pairMs, (double)d->pairLaunches, d->listMs, (double)d->listLaunches, (double)d->cap, (double)d->M };
-  for (int k = 0; k < n && k < 6; k++) out[k] = v[k];
+  double v[7] = { d->pairMs, (double)d->pairLaunches, d->listMs, (double)d->listLaunches, (double)d->cap, (double)d->M,
+                  d->listEntries };
+  for (int k = 0; k < n && k < 7; k++) out[k] = v[k];
 }
 void dev_sync(Device* d) { HIP_CHECK(hipStreamSynchronize(d->stream)); }
 

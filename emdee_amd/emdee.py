@@ -103,6 +103,18 @@ SIGNATURES = {
 }
 
 
+# Extensions of the HIP build (include/emdee_gpu.h); bound when present.
+GPU_SIGNATURES = {
+    "EmDeeGPU_set_precision": (None, [tEmDee, c_int]),
+    "EmDeeGPU_get_precision": (c_int, [tEmDee]),
+    "EmDeeGPU_neighbor_pairs": (c_int, [tEmDee, c_int, _ip]),
+    "EmDeeGPU_profile": (None, [tEmDee, c_int]),
+    "EmDeeGPU_kernel_stats": (None, [tEmDee, _dp]),
+    "EmDeeGPU_synchronize": (None, [tEmDee]),
+    "EmDeeGPU_destroy": (None, [POINTER(tEmDee)]),
+}
+
+
 def _f64(a, n=None):
     a = np.ascontiguousarray(a, dtype=np.float64)
     if n is not None and a.size != n:
@@ -121,8 +133,15 @@ class EmDeeLibrary:
             f.restype = res
             f.argtypes = args
 
+        self.has_gpu_extensions = hasattr(self.lib, "EmDeeGPU_set_precision")
+        if self.has_gpu_extensions:
+            for name, (res, args) in GPU_SIGNATURES.items():
+                f = getattr(self.lib, name)
+                f.restype = res
+                f.argtypes = args
+
     def __getattr__(self, name):
-        if name.startswith("EmDee_"):
+        if name.startswith("EmDee"):
             return getattr(self.lib, name)
         raise AttributeError(name)
 
@@ -169,3 +188,15 @@ class EmDeeLibrary:
 
     def switch_model_layer(self, md, layer):
         self.lib.EmDee_switch_model_layer(ctypes.byref(md), layer)
+
+    def neighbor_pairs(self, md):
+        """Unordered Verlet-list pair set as an (n, 2) array of 1-based i < j."""
+        n = self.lib.EmDeeGPU_neighbor_pairs(md, 0, None)
+        out = np.zeros((max(n, 1), 2), dtype=np.int32)
+        self.lib.EmDeeGPU_neighbor_pairs(md, n, out.ctypes.data_as(_ip))
+        return out[:n]
+
+    def kernel_stats(self, md):
+        out = np.zeros(8)
+        self.lib.EmDeeGPU_kernel_stats(md, out.ctypes.data_as(_dp))
+        return out

@@ -28,9 +28,10 @@ int  EmDeeGPU_get_precision( tEmDee md );
 int EmDeeGPU_neighbor_pairs( tEmDee md, int max, int* pairs );
 
 /* Kernel timing with HIP events on the system's stream.  After
-   EmDeeGPU_profile(md, 1), EmDeeGPU_kernel_stats fills stats[0..5] with
+   EmDeeGPU_profile(md, 1), EmDeeGPU_kernel_stats fills stats[0..6] with
    {pair-kernel ms (sum), pair launches, list-build ms (sum), list builds,
-    neighbor-list capacity per atom, cells per box side}. */
+    neighbor-list capacity per atom, cells per box side,
+    full-list entries at the latest build (twice the pair count)}. */
 void EmDeeGPU_profile( tEmDee md, int on );
 void EmDeeGPU_kernel_stats( tEmDee md, double* stats );
 
