@@ -422,22 +422,41 @@ k_build_list(int* __restrict__ nbr, int* __restrict__ nnb, int* __restrict__ fla
     int ti = (int)ri.w, bi = body[i], oi = perm[i];
     int e0 = 0, e1 = 0;
     if (nexcl) { e0 = exOff[oi]; e1 = exOff[oi + 1]; }
-    int xa = cx - 2, xb = cx + 2;
+    // Conservative culling: with f the fractional position of atom i inside
+    // its cell, the row of cells at offset (dy, dz) is at least (gy, gz) cells
+    // away, and inside that row only cells x with |x - (cx + fx)| <= e can
+    // hold an atom closer than the cutoff rcell (cell units).  The exact FP64
+    // test above still decides every scanned candidate; the margins absorb
+    // the rounding of this estimate, so the pair set is unchanged.
+    auto frac = [&](double v, int cc) {
+      double f = M * (v - floor(v)) - cc;
+      if (f > 1.5) f -= M;                  // the coordinate wrapped to cell 0
+      return fmin(fmax(f, 0.0), 1.0);
+    };
+    double fx = frac(ri.x, cx), fy = frac(ri.y, cy), fz = frac(ri.z, cz);
+    double rcell2 = xRc2 * (double)M * (double)M * (1.0 + 1e-6) + 1e-9;
+    auto scan_cells = [&](int base, int c0, int c1) {   // cells c0..c1 of one row, 0 <= c0 <= c1 < M
+      scan_range(start[base + c0], start[base + c1 + 1], i, ri.x, ri.y, ri.z, ti, bi, oi, Rs4, body, perm, interact,
+                 exItem, e0, e1, ntypes, xRc2, cap, Npad, nbr, cnt);
+    };
     for (int dz = -2; dz <= 2; dz++) {
+      double gz = dz > 0 ? dz - fz : (dz < 0 ? fz - dz - 1 : 0.0);
+      gz = fmax(gz, 0.0);
       int z = cz + dz; z = z < 0 ? z + M : (z >= M ? z - M : z);
       for (int dy = -2; dy <= 2; dy++) {
+        double gy = dy > 0 ? dy - fy : (dy < 0 ? fy - dy - 1 : 0.0);
+        gy = fmax(gy, 0.0);
+        double rem = rcell2 - gy * gy - gz * gz;
+        if (rem <= 0.0) continue;
+        double e = sqrt(rem) + 1e-6;
+        int xa = cx + max((int)floor(fx - e), -2), xb = cx + min((int)floor(fx + e), 2);
         int y = cy + dy; y = y < 0 ? y + M : (y >= M ? y - M : y);
         int base = M * (y + M * z);
-        if (xa >= 0 && xb < M) {
-          scan_range(start[base + xa], start[base + xb + 1], i, ri.x, ri.y, ri.z, ti, bi, oi, Rs4, body, perm, interact,
-                     exItem, e0, e1, ntypes, xRc2, cap, Npad, nbr, cnt);
-        } else {
-          int a0 = xa < 0 ? xa + M : xa, b1 = xb >= M ? xb - M : xb;
-          scan_range(start[base + a0], start[base + M], i, ri.x, ri.y, ri.z, ti, bi, oi, Rs4, body, perm, interact,
-                     exItem, e0, e1, ntypes, xRc2, cap, Npad, nbr, cnt);
-          scan_range(start[base], start[base + b1 + 1], i, ri.x, ri.y, ri.z, ti, bi, oi, Rs4, body, perm, interact,
-                     exItem, e0, e1, ntypes, xRc2, cap, Npad, nbr, cnt);
-        }
+        if (xa >= 0 && xb < M) scan_cells(base, xa, xb);
+        else if (xa < 0 && xb >= 0) { scan_cells(base, xa + M, M - 1); scan_cells(base, 0, xb); }
+        else if (xa < 0) scan_cells(base, xa + M, xb + M);
+        else if (xa < M) { scan_cells(base, xa, M - 1); scan_cells(base, 0, xb - M); }
+        else scan_cells(base, xa - M, xb - M);
       }
     }
     nnb[i] = cnt;
