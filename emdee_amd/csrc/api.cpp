@@ -45,6 +45,8 @@ struct System {
   std::vector<int> atomType;            // 1-based types, [N]
   std::vector<int> atomBody;            // reference body ids, [N]
   std::vector<int> freeList;            // 1-based free atoms
+  std::vector<int> bStart, bOrig;       // rigid bodies: member CSR, 0-based original indices
+  std::vector<double> bMemberMass, bodyMass;
   std::vector<double> mass, invMass, charge;
   std::vector<char> charged;
   Kiss rng;
@@ -83,7 +85,6 @@ double phi_fn(double x) {   // math.f90:230-237
 }
 
 void require_supported(System* me, const char* task) {
-  if (me->nbodies > 0) fatal(task, "rigid bodies are not available in this build of the engine");
   if (me->nbonds > 0 || me->nangles > 0) fatal(task, "bonds and angles are not available in this build of the engine");
   if (me->kspace_active) fatal(task, "kspace solvers are not available in this build of the engine");
 }
@@ -147,6 +148,7 @@ void upload_models(System* me) {
 // EmDeeData.f90:359-416
 void perform_initialization(System* me, tEmDee* md) {
   const char* task = "system initialization";
+  if (me->nbodies > 0) dev_bodies_update(me->dev, me->L);
   int bodyDoF = 6 * me->nbodies;
   md->RotDoF = bodyDoF - 3 * me->nbodies;
   md->DoF = 3 * me->nfree + bodyDoF - 3;
@@ -223,6 +225,10 @@ void compute_forces(tEmDee* md) {
   }
   E[PAIR_] = fs.Epair; E[COUL_] = fs.Ecoul; W[PAIR_] = fs.Wpair; W[COUL_] = fs.Wcoul;
   md->Virial = W[0] + W[1] + W[2] + W[3] + W[4];
+  if (me->nbodies != 0) {
+    md->BodyVirial = dev_bodies_virial(me->dev, l - 1);
+    md->Virial = md->Virial + md->BodyVirial;
+  }
   if (compute) {
     md->Energy.Dispersion = E[PAIR_];
     md->Energy.Coulomb = E[COUL_] + E[LONG_];
@@ -274,6 +280,16 @@ void allocate_rigid_bodies(System* me, const int* bodies) {
     me->nbodies = nb;
     me->freeList.clear();
     for (int k = 1; k <= N; k++) if (me->atomBody[k - 1] == 0) me->freeList.push_back(k);
+    // members of body b in increasing atom index (EmDeeData.f90:300-303)
+    std::vector<std::vector<int>> members(nb + 1);
+    for (int k = 1; k <= N; k++) if (me->atomBody[k - 1] > 0) members[me->atomBody[k - 1]].push_back(k - 1);
+    me->bStart.assign(1, 0);
+    for (int b = 1; b <= nb; b++) {
+      double m = 0;
+      for (int a : members[b]) { me->bOrig.push_back(a); me->bMemberMass.push_back(me->mass[a]); m += me->mass[a]; }
+      me->bodyMass.push_back(m);
+      me->bStart.push_back((int)me->bOrig.size());
+    }
     i = nb;
     for (int k = 0; k < N; k++) if (me->atomBody[k] == 0) me->atomBody[k] = ++i;
   } else {
@@ -353,6 +369,7 @@ tEmDee EmDee_system(int threads, int layers, double rc, double skin, int N, int*
   for (int i = 0; i < N; i++) { ds.type[i] = me->atomType[i] - 1; ds.body[i] = me->atomBody[i]; ds.freeFlag[i] = 0; }
   for (int k : me->freeList) ds.freeFlag[k - 1] = 1;
   me->dev = dev_create(ds);
+  if (me->nbodies > 0) dev_bodies_setup(me->dev, me->bStart, me->bOrig, me->bMemberMass, me->bodyMass);
   md.Builds = 0;
   md.DoF = 3 * (N - 1);
   md.RotDoF = 0;
@@ -569,6 +586,7 @@ void EmDee_download(tEmDee md, char* option, double* address) {
   System* me = S(md);
   std::string item(option);
   if (!address) fatal("download", "provided address is invalid");
+  int nb = me->nbodies;
   if (item == "box") *address = me->L;
   else if (item == "coordinates") {
     if (!me->hasR) fatal("download", "coordinates have not been allocated");
@@ -580,11 +598,28 @@ void EmDee_download(tEmDee md, char* option, double* address) {
   } else if (item == "centersOfMass") {
     std::vector<double> R(3 * (size_t)me->natoms);
     dev_download_positions(me->dev, R.data());
-    for (int j = 0; j < me->nfree; j++) for (int x = 0; x < 3; x++) address[3 * (j + me->nbodies) + x] = R[3 * (me->freeList[j] - 1) + x];
-  } else if (item == "quaternions" || item == "quatmom" || item == "quattau" || item == "angmom" || item == "bodycoord" ||
-             item == "bodymom" || item == "bodyforces" || item == "torques" || item == "inertia") {
-    // no rigid bodies exist in systems this build accepts
-  } else fatal("download", "invalid option");
+    dev_bodies_get(me->dev, BF_RCM, address);
+    for (int j = 0; j < me->nfree; j++) for (int x = 0; x < 3; x++) address[3 * (j + nb) + x] = R[3 * (me->freeList[j] - 1) + x];
+  } else if (item == "quaternions") dev_bodies_get(me->dev, BF_Q, address);
+  else if (item == "quatmom") dev_bodies_get(me->dev, BF_PI, address);
+  else if (item == "quattau") {
+    // C(q) (2 tau): EmDeeCode.f90:768
+    std::vector<double> q(4 * (size_t)nb), tau(3 * (size_t)nb);
+    dev_bodies_get(me->dev, BF_Q, q.data());
+    dev_bodies_get(me->dev, BF_TAU, tau.data());
+    for (int b = 0; b < nb; b++) {
+      const double* Q = &q[4 * b];
+      double t[3] = { 2.0 * tau[3 * b], 2.0 * tau[3 * b + 1], 2.0 * tau[3 * b + 2] };
+      double C[12] = { -Q[1], -Q[2], -Q[3], Q[0], Q[3], -Q[2], -Q[3], Q[0], Q[1], Q[2], -Q[1], Q[0] };
+      for (int r = 0; r < 4; r++) address[4 * b + r] = C[3 * r] * t[0] + C[3 * r + 1] * t[1] + C[3 * r + 2] * t[2];
+    }
+  } else if (item == "angmom") dev_bodies_get(me->dev, BF_OMEGA, address);
+  else if (item == "bodycoord") dev_bodies_get(me->dev, BF_RCM, address);
+  else if (item == "bodymom") dev_bodies_get(me->dev, BF_PCM, address);
+  else if (item == "bodyforces") dev_bodies_get(me->dev, BF_F, address);
+  else if (item == "torques") dev_bodies_get(me->dev, BF_TAU, address);
+  else if (item == "inertia") dev_bodies_get(me->dev, BF_MOI, address);
+  else fatal("download", "invalid option");
 }
 
 // EmDeeCode.f90:805-925
@@ -600,16 +635,20 @@ void EmDee_upload(tEmDee* md, char* option, double* address) {
   } else if (item == "coordinates") {
     dev_upload_positions(me->dev, address);
     me->hasR = true;
-    if (me->initialized) mark_stale(me, md);
-    else if (me->hasL) initialize_system(me, md);
+    if (me->initialized) {
+      mark_stale(me, md);
+      if (md->Options.AutoBodyUpdate && me->nbodies > 0) dev_bodies_update(me->dev, me->L);
+    } else if (me->hasL) initialize_system(me, md);
   } else if (item == "momenta") {
     if (!me->initialized) fatal("upload", "box and coordinates have not been defined");
     dev_upload_momenta(me->dev, address);
-    double t[3], r[3] = { 0, 0, 0 };
-    // assign_momenta, EmDeeData.f90:157-189: 2KE in free-atom order
-    for (int x = 0; x < 3; x++) t[x] = 0;
+    double kb[6];
+    dev_bodies_assign_momenta(me->dev, kb);
+    // assign_momenta, EmDeeData.f90:157-189: free atoms, then bodies
+    double t[3] = { 0, 0, 0 }, r[3];
     for (int j : me->freeList)
-      for (int x = 0; x < 3; x++) t[x] += me->invMass[j - 1] * address[3 * (j - 1) + x] * address[3 * (j - 1) + x];
+      for (int x = 0; x < 3; x++) t[x] += me->invMass[j - 1] * (address[3 * (j - 1) + x] * address[3 * (j - 1) + x]);
+    for (int x = 0; x < 3; x++) { t[x] += kb[x]; r[x] = kb[3 + x]; }
     set_kinetic(md, t, r);
   } else if (item == "forces") {
     if (!me->initialized) fatal("upload", "box and coordinates have not been defined");
@@ -624,40 +663,84 @@ void EmDee_upload(tEmDee* md, char* option, double* address) {
 // EmDeeCode.f90:929-946
 void EmDee_switch_model_layer(tEmDee* md, int layer) { switch_layer(S(*md), md, layer); }
 
-// EmDeeCode.f90:950-1020
+// pi = B(q) (2 I omega): tBody_assign_momenta, ArBee.f90:356-362
+static void body_pi_from_omega(const double* q, const double* MoI, const double* w, double* pi) {
+  double t[3] = { 2.0 * MoI[0] * w[0], 2.0 * MoI[1] * w[1], 2.0 * MoI[2] * w[2] };
+  double B[12] = { -q[1], -q[2], -q[3], q[0], -q[3], q[2], q[3], q[0], -q[1], -q[2], q[1], q[0] };
+  for (int r = 0; r < 4; r++) pi[r] = B[3 * r] * t[0] + B[3 * r + 1] * t[1] + B[3 * r + 2] * t[2];
+}
+
+// EmDeeCode.f90:950-1020.  The generator is serial (KISS + ziggurat), so the
+// draws run on the host in the reference order -- bodies (3 normals for the
+// centre-of-mass momentum, 3 for the angular velocity), then free atoms --
+// and the results are uploaded.
 void EmDee_random_momenta(tEmDee* md, double kT, bool adjust, int seed) {
   System* me = S(*md);
   if (me->rng.seeding_required) me->rng.setup(seed);
   double twoKEt[3] = { 0, 0, 0 }, twoKEr[3] = { 0, 0, 0 };
-  int N = me->natoms;
+  int N = me->natoms, nb = me->nbodies;
   std::vector<double> P(3 * (size_t)N, 0.0);
-  if (me->nbodies != 0 && !me->initialized) fatal("random_momenta", "coordinates have not defined");
+  std::vector<double> q(4 * (size_t)nb), MoI(3 * (size_t)nb), invMoI(3 * (size_t)nb), pcm(3 * (size_t)nb),
+      omega(3 * (size_t)nb), pi(4 * (size_t)nb);
+  if (nb != 0) {
+    if (!me->initialized) fatal("random_momenta", "coordinates have not defined");
+    dev_bodies_get(me->dev, BF_Q, q.data());
+    dev_bodies_get(me->dev, BF_MOI, MoI.data());
+    dev_bodies_get(me->dev, BF_INVMOI, invMoI.data());
+    for (int b = 0; b < nb; b++) {
+      double f = std::sqrt(me->bodyMass[b] * kT);
+      for (int x = 0; x < 3; x++) pcm[3 * b + x] = f * me->rng.normal();
+      for (int x = 0; x < 3; x++) omega[3 * b + x] = std::sqrt(invMoI[3 * b + x] * kT) * me->rng.normal();
+      body_pi_from_omega(&q[4 * b], &MoI[3 * b], &omega[3 * b], &pi[4 * b]);
+      double im = 1.0 / me->bodyMass[b];
+      for (int x = 0; x < 3; x++) {
+        twoKEt[x] += im * (pcm[3 * b + x] * pcm[3 * b + x]);
+        twoKEr[x] += MoI[3 * b + x] * omega[3 * b + x] * omega[3 * b + x];
+      }
+    }
+  }
   for (int i : me->freeList) {
     double f = std::sqrt(me->mass[i - 1] * kT);
     for (int x = 0; x < 3; x++) P[3 * (i - 1) + x] = f * me->rng.normal();
-    for (int x = 0; x < 3; x++) twoKEt[x] += me->invMass[i - 1] * P[3 * (i - 1) + x] * P[3 * (i - 1) + x];
+    for (int x = 0; x < 3; x++) twoKEt[x] += me->invMass[i - 1] * (P[3 * (i - 1) + x] * P[3 * (i - 1) + x]);
   }
   if (adjust) {   // adjust_momenta, EmDeeCode.f90:984-1018
     double vcm[3];
     for (int x = 0; x < 3; x++) {
-      double s1 = 0;
+      double s1 = 0, s2 = 0;
       for (int i : me->freeList) s1 += P[3 * (i - 1) + x];
-      double s2 = 0;
+      for (int b = 0; b < nb; b++) s2 += pcm[3 * b + x];
       vcm[x] = (s1 + s2) / me->totalMass;
     }
     for (int x = 0; x < 3; x++) twoKEt[x] = 0;
     for (int i : me->freeList)
       for (int x = 0; x < 3; x++) {
         P[3 * (i - 1) + x] = P[3 * (i - 1) + x] - me->mass[i - 1] * vcm[x];
-        twoKEt[x] += me->invMass[i - 1] * P[3 * (i - 1) + x] * P[3 * (i - 1) + x];
+        twoKEt[x] += me->invMass[i - 1] * (P[3 * (i - 1) + x] * P[3 * (i - 1) + x]);
       }
-    int dofsum = 6 * me->nbodies;
+    for (int b = 0; b < nb; b++)
+      for (int x = 0; x < 3; x++) {
+        pcm[3 * b + x] = pcm[3 * b + x] - me->bodyMass[b] * vcm[x];
+        twoKEt[x] += (1.0 / me->bodyMass[b]) * (pcm[3 * b + x] * pcm[3 * b + x]);
+      }
+    int dofsum = 6 * nb;
     double tot = (twoKEt[0] + twoKEr[0]) + (twoKEt[1] + twoKEr[1]) + (twoKEt[2] + twoKEr[2]);
     double factor = std::sqrt((double)(3 * me->nfree + dofsum - 3) * kT / tot);
     for (int i : me->freeList) for (int x = 0; x < 3; x++) P[3 * (i - 1) + x] *= factor;
+    for (int b = 0; b < nb; b++) {
+      double w[3];
+      for (int x = 0; x < 3; x++) { pcm[3 * b + x] = factor * pcm[3 * b + x]; w[x] = factor * omega[3 * b + x]; }
+      for (int x = 0; x < 3; x++) omega[3 * b + x] = w[x];
+      body_pi_from_omega(&q[4 * b], &MoI[3 * b], w, &pi[4 * b]);
+    }
     for (int x = 0; x < 3; x++) { twoKEt[x] = factor * factor * twoKEt[x]; twoKEr[x] = factor * factor * twoKEr[x]; }
   }
   dev_upload_momenta(me->dev, P.data());
+  if (nb != 0) {
+    dev_bodies_set(me->dev, BF_PCM, pcm.data());
+    dev_bodies_set(me->dev, BF_OMEGA, omega.data());
+    dev_bodies_set(me->dev, BF_PI, pi.data());
+  }
   set_kinetic(md, twoKEt, twoKEr);
 }
 
@@ -668,14 +751,15 @@ void EmDee_boost(tEmDee* md, double lambda, double alpha, double dt) {
   double CP = 1.0 - alpha * CF;
   CF = lambda * CF;
   if (lambda != 0.0 && !me->forcesUpToDate[me->layer]) compute_forces(md);
-  bool compute = md->Options.Compute, translate = md->Options.Translate;
-  double twoKEt[3] = { 0, 0, 0 };
+  bool compute = md->Options.Compute, translate = md->Options.Translate, rotate = md->Options.Rotate;
+  double twoKEt[3] = { 0, 0, 0 }, kb[6] = { 0, 0, 0, 0, 0, 0 };
+  dev_bodies_boost(me->dev, me->layer - 1, CP, CF, translate, rotate, compute ? kb : nullptr);
   dev_boost(me->dev, me->layer - 1, CP, CF, translate, compute ? twoKEt : nullptr);
   if (compute) {
-    if (translate) for (int x = 0; x < 3; x++) md->Kinetic.TransPart[x] = 0.5 * twoKEt[x];
-    if (md->Options.Rotate) {
-      for (int x = 0; x < 3; x++) md->Kinetic.RotPart[x] = 0.0;
-      md->Kinetic.Rotational = 0.0;
+    if (translate) for (int x = 0; x < 3; x++) md->Kinetic.TransPart[x] = 0.5 * (kb[x] + twoKEt[x]);
+    if (rotate) {
+      for (int x = 0; x < 3; x++) md->Kinetic.RotPart[x] = 0.5 * kb[3 + x];
+      md->Kinetic.Rotational = md->Kinetic.RotPart[0] + md->Kinetic.RotPart[1] + md->Kinetic.RotPart[2];
     }
     md->Kinetic.Kinetic = (md->Kinetic.TransPart[0] + md->Kinetic.TransPart[1] + md->Kinetic.TransPart[2]) + md->Kinetic.Rotational;
   }
@@ -696,6 +780,7 @@ static void displace(tEmDee* md, double lambda, double alpha, double dt) {
     CR = 1.0;
   }
   CP = lambda * CP;
+  dev_bodies_move(me->dev, CR, CP, dt, md->Options.Translate, md->Options.Rotate, md->Options.RotationMode);
   if (md->Options.Translate) dev_displace(me->dev, CR, CP);
   mark_stale(me, md);
   md->Time.Motion += wtime();

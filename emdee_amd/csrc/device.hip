@@ -26,113 +26,11 @@
 #include <cstring>
 #include <vector>
 
+#include "device_state.h"
 #include "engine.h"
 #include "pair_eval.h"
 
 namespace emdee {
-
-#define HIP_CHECK(x)                                                             \
-  do {                                                                           \
-    hipError_t e_ = (x);                                                         \
-    if (e_ != hipSuccess) fatal("HIP runtime", std::string(#x) + ": " + hipGetErrorString(e_)); \
-  } while (0)
-
-constexpr int BLOCK = 256;
-constexpr int NXCD = 8;
-
-static inline int div_up(long a, long b) { return (int)((a + b - 1) / b); }
-
-// Blocks are dispatched round-robin over the 8 XCDs (block b runs on XCD
-// b % 8).  Remap so that XCD x processes one contiguous range of slots: the
-// neighbor positions a wave gathers then mostly live in that XCD's L2.
-__device__ __forceinline__ int xcd_block() {
-  int b = blockIdx.x, nb = gridDim.x;
-  int q = nb / NXCD, rem = nb % NXCD, x = b % NXCD, r = b / NXCD;
-  return x * q + (x < rem ? x : rem) + r;
-}
-
-template <typename T>
-__device__ __forceinline__ T wave_sum(T v) {
-  for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
-  return v;
-}
-
-// Sums NV values over the block; thread 0 of the block gets the totals.
-template <int NV>
-__device__ __forceinline__ void block_sum(double (&v)[NV]) {
-  __shared__ double red[BLOCK / 64][NV];
-  int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
-#pragma unroll
-  for (int k = 0; k < NV; k++) v[k] = wave_sum(v[k]);
-  if (lane == 0) {
-#pragma unroll
-    for (int k = 0; k < NV; k++) red[wid][k] = v[k];
-  }
-  __syncthreads();
-  if (threadIdx.x == 0) {
-#pragma unroll
-    for (int k = 0; k < NV; k++) {
-      double s = 0;
-      for (int w = 0; w < BLOCK / 64; w++) s += red[w][k];
-      v[k] = s;
-    }
-  }
-}
-
-// ------------------------------------------------------------ device state
-
-template <typename T>
-struct DBuf {
-  T* p = nullptr;
-  size_t n = 0;
-  void alloc(size_t count) {
-    if (count <= n && p) return;
-    if (p) HIP_CHECK(hipFree(p));
-    HIP_CHECK(hipMalloc(&p, std::max<size_t>(count, 1) * sizeof(T)));
-    n = count;
-  }
-  void release() { if (p) (void)hipFree(p); p = nullptr; n = 0; }
-};
-
-struct Device {
-  int N = 0, Npad = 0, ntypes = 0, nlayers = 0;
-  hipStream_t stream = nullptr;
-  // per-atom SoA arrays (sorted order)
-  DBuf<double> R, P, R0, invMass, charge, F;   // F holds nlayers * 3 * Npad
-  DBuf<double4> Rs4;      // {x/L, y/L, z/L, type}: one 32-byte gather per neighbor
-  DBuf<int> type, body, perm, freeFlag, chargedFlag;
-  DBuf<double> tmp3, tmpF, tmp1;   // permutation scratch (swapped with the arrays)
-  DBuf<int> tmpI;
-  // transfers
-  DBuf<double> stage;     // 3N AoS, original order
-  double* hstage = nullptr;  // pinned host mirror
-  // cells
-  int M = 0, ncells = 0;
-  DBuf<int> key, cellCount, cellStart, order, blockSums;   // cellStart has ncells+1 entries
-  // neighbor list
-  int cap = 0;
-  DBuf<int> nbr, nnb;
-  DBuf<int> flags;        // [0] overflow max count
-  bool built = false;
-  // models
-  DBuf<DevPair> pairs;    // nlayers * nt * nt
-  DBuf<DevCoul> couls;    // nlayers
-  DBuf<uint8_t> interact;
-  DBuf<int> exOff, exItem;
-  int nexcl = 0;
-  std::vector<int> hostType;
-  // reductions
-  DBuf<double> partials, result;
-  double* hresult = nullptr;
-  DBuf<double> d2;
-  DBuf<double> trig;      // [0] max, [1] first original index holding it, [2] max before it
-  // profiling
-  bool profiling = false;
-  hipEvent_t ev0 = nullptr, ev1 = nullptr;
-  double pairMs = 0, listMs = 0;
-  long pairLaunches = 0, listLaunches = 0;
-  double listEntries = 0;
-};
 
 [[noreturn]] void fatal(const std::string& task, const std::string& msg) {
   fprintf(stderr, "Error in %s: %s.\n", task.c_str(), msg.c_str());
@@ -146,6 +44,11 @@ void warning(const std::string& msg) { fprintf(stderr, "WARNING: %s.\n", msg.c_s
 __global__ void k_iota(int* p, int n) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) p[i] = i;
+}
+// iperm[perm[s]] = s
+__global__ void k_invert_perm(int* __restrict__ iperm, const int* __restrict__ perm, int n) {
+  int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s < n) iperm[perm[s]] = s;
 }
 
 // AoS (original order) -> SoA (sorted order)
@@ -623,7 +526,7 @@ __global__ void k_boost(double* __restrict__ P, double* __restrict__ partials, c
     for (int x = 0; x < 3; x++) {
       double p = P[x * Npad + s];
       if (translate) { p = CP * p + CF * F[x * Npad + s]; P[x * Npad + s] = p; }
-      if (KE) v[x] = im * p * p;
+      if (KE) v[x] = im * (p * p);
     }
   }
   if (KE) {
@@ -643,7 +546,7 @@ __global__ void k_displace(double* __restrict__ R, const double* __restrict__ P,
 
 // ------------------------------------------------------------ host side
 
-static void launch_check(const char* what) {
+void launch_check(const char* what) {
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) fatal("HIP kernel launch", std::string(what) + ": " + hipGetErrorString(e));
 }
@@ -679,7 +582,11 @@ Device* dev_create(const DeviceSetup& s) {
   HIP_CHECK(hipMemsetAsync(d->nnb.p, 0, sizeof(int) * Np, d->stream));
   HIP_CHECK(hipMemsetAsync(d->freeFlag.p, 0, sizeof(int) * Np, d->stream));
   int g = div_up(s.N, BLOCK);
-  if (s.N > 0) hipLaunchKernelGGL(k_iota, dim3(g), dim3(BLOCK), 0, d->stream, d->perm.p, s.N);
+  d->iperm.alloc(Np);
+  if (s.N > 0) {
+    hipLaunchKernelGGL(k_iota, dim3(g), dim3(BLOCK), 0, d->stream, d->perm.p, s.N);
+    hipLaunchKernelGGL(k_iota, dim3(g), dim3(BLOCK), 0, d->stream, d->iperm.p, s.N);
+  }
   launch_check("iota");
   // static per-atom data, initial order = original order
   std::vector<double> im(s.N), q(s.N);
@@ -707,7 +614,7 @@ void dev_destroy(Device* d) {
   (void)hipStreamSynchronize(d->stream);
   for (DBuf<double>* b : { &d->R, &d->P, &d->R0, &d->invMass, &d->charge, &d->F, &d->tmp3, &d->tmpF, &d->tmp1, &d->stage, &d->partials, &d->result, &d->d2, &d->trig }) b->release();
   d->Rs4.release();
-  for (DBuf<int>* b : { &d->type, &d->body, &d->perm, &d->freeFlag, &d->chargedFlag, &d->tmpI, &d->key, &d->cellCount, &d->cellStart, &d->order, &d->blockSums, &d->nbr, &d->nnb, &d->flags, &d->exOff, &d->exItem }) b->release();
+  for (DBuf<int>* b : { &d->type, &d->body, &d->perm, &d->freeFlag, &d->chargedFlag, &d->tmpI, &d->key, &d->cellCount, &d->cellStart, &d->order, &d->blockSums, &d->nbr, &d->nnb, &d->flags, &d->exOff, &d->exItem, &d->iperm }) b->release();
   d->pairs.release(); d->couls.release(); d->interact.release();
   if (d->hstage) (void)hipHostFree(d->hstage);
   if (d->hresult) (void)hipHostFree(d->hresult);
@@ -775,7 +682,16 @@ static void download_aos(Device* d, double* host, const double* src) {
 void dev_upload_positions(Device* d, const double* R) { if (d->N) upload_aos(d, d->R.p, R, nullptr); }
 void dev_download_positions(Device* d, double* R) { if (d->N) download_aos(d, R, d->R.p); }
 void dev_upload_momenta(Device* d, const double* P) { if (d->N) upload_aos(d, d->P.p, P, d->freeFlag.p); }
-void dev_download_momenta(Device* d, double* P) { if (d->N) download_aos(d, P, d->P.p); }
+void dev_download_momenta(Device* d, double* P) {
+  int N = d->N;
+  if (!N) return;
+  hipLaunchKernelGGL(k_scatter_aos, dim3(div_up(N, BLOCK)), dim3(BLOCK), 0, d->stream, d->stage.p, d->P.p, d->perm.p, N, d->Npad);
+  launch_check("scatter_aos");
+  dev_bodies_particle_momenta(d);
+  HIP_CHECK(hipMemcpyAsync(d->hstage, d->stage.p, 3 * sizeof(double) * N, hipMemcpyDeviceToHost, d->stream));
+  HIP_CHECK(hipStreamSynchronize(d->stream));
+  memcpy(P, d->hstage, 3 * sizeof(double) * N);
+}
 void dev_upload_forces(Device* d, int layer, const double* F) {
   if (d->N) upload_aos(d, d->F.p + 3 * (size_t)d->Npad * layer, F, nullptr);
 }
@@ -855,6 +771,7 @@ void dev_rebuild(Device* d, double L, double xRc, double xInRc) {
   permN(d->invMass, d->tmp1, 1);
   permN(d->charge, d->tmp1, 1);
   permI(d->type); permI(d->body); permI(d->perm); permI(d->freeFlag); permI(d->chargedFlag);
+  hipLaunchKernelGGL(k_invert_perm, dim3(g), dim3(BLOCK), 0, d->stream, d->iperm.p, d->perm.p, N);
   launch_check("permute");
   // scaled positions and keys in the new order (Rs4 is a function of R, type)
   hipLaunchKernelGGL(k_scale4, dim3(g), dim3(BLOCK), 0, d->stream, d->Rs4.p, d->R.p, d->type.p, L, N, Np);

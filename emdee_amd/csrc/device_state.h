@@ -1,0 +1,134 @@
+// device_state.h -- HBM state of one system, shared by the kernel files
+// (device.hip: neighbor lists, pair loop, integrator of free atoms;
+// rigid.hip: rigid bodies).
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <string>
+#include <vector>
+
+#include "engine.h"
+
+namespace emdee {
+
+#define HIP_CHECK(x)                                                             \
+  do {                                                                           \
+    hipError_t e_ = (x);                                                         \
+    if (e_ != hipSuccess) fatal("HIP runtime", std::string(#x) + ": " + hipGetErrorString(e_)); \
+  } while (0)
+
+constexpr int BLOCK = 256;
+constexpr int NXCD = 8;
+
+static inline int div_up(long a, long b) { return (int)((a + b - 1) / b); }
+
+// Blocks are dispatched round-robin over the 8 XCDs (block b runs on XCD
+// b % 8).  Remap so that XCD x processes one contiguous range of slots: the
+// neighbor positions a wave gathers then mostly live in that XCD's L2.
+__device__ __forceinline__ int xcd_block() {
+  int b = blockIdx.x, nb = gridDim.x;
+  int q = nb / NXCD, rem = nb % NXCD, x = b % NXCD, r = b / NXCD;
+  return x * q + (x < rem ? x : rem) + r;
+}
+
+template <typename T>
+__device__ __forceinline__ T wave_sum(T v) {
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
+  return v;
+}
+
+// Sums NV values over the block; thread 0 of the block gets the totals.
+template <int NV>
+__device__ __forceinline__ void block_sum(double (&v)[NV]) {
+  __shared__ double red[BLOCK / 64][NV];
+  int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+#pragma unroll
+  for (int k = 0; k < NV; k++) v[k] = wave_sum(v[k]);
+  if (lane == 0) {
+#pragma unroll
+    for (int k = 0; k < NV; k++) red[wid][k] = v[k];
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+#pragma unroll
+    for (int k = 0; k < NV; k++) {
+      double s = 0;
+      for (int w = 0; w < BLOCK / 64; w++) s += red[w][k];
+      v[k] = s;
+    }
+  }
+}
+
+// ------------------------------------------------------------ device state
+
+template <typename T>
+struct DBuf {
+  T* p = nullptr;
+  size_t n = 0;
+  void alloc(size_t count) {
+    if (count <= n && p) return;
+    if (p) HIP_CHECK(hipFree(p));
+    HIP_CHECK(hipMalloc(&p, std::max<size_t>(count, 1) * sizeof(T)));
+    n = count;
+  }
+  void release() { if (p) (void)hipFree(p); p = nullptr; n = 0; }
+};
+
+// Rigid bodies (ArBee.f90 tBody), SoA over bodies; members in CSR order.
+struct BodyState {
+  int nb = 0, nm = 0;
+  DBuf<int> start, orig;        // member range of body b: start[b]..start[b+1]-1; original atom index
+  DBuf<double> mMass, d, delta; // member mass; body-frame and space-frame member offsets [3][nm]
+  DBuf<double> mass, invMass, MoI, invMoI, coef;   // coef: I113, I313, I223, I212, m312 [5][nb]
+  DBuf<double> rcm, pcm, q, pi, omega, F, tau;     // [3][nb] or [4][nb]
+  DBuf<double> part;            // reduction partials
+};
+
+struct Device {
+  int N = 0, Npad = 0, ntypes = 0, nlayers = 0;
+  hipStream_t stream = nullptr;
+  // per-atom SoA arrays (sorted order)
+  DBuf<double> R, P, R0, invMass, charge, F;   // F holds nlayers * 3 * Npad
+  DBuf<double4> Rs4;      // {x/L, y/L, z/L, type}: one 32-byte gather per neighbor
+  DBuf<int> type, body, perm, freeFlag, chargedFlag;
+  DBuf<double> tmp3, tmpF, tmp1;   // permutation scratch (swapped with the arrays)
+  DBuf<int> tmpI;
+  // transfers
+  DBuf<double> stage;     // 3N AoS, original order
+  double* hstage = nullptr;  // pinned host mirror
+  // cells
+  int M = 0, ncells = 0;
+  DBuf<int> key, cellCount, cellStart, order, blockSums;   // cellStart has ncells+1 entries
+  // neighbor list
+  int cap = 0;
+  DBuf<int> nbr, nnb;
+  DBuf<int> flags;        // [0] overflow max count
+  bool built = false;
+  // models
+  DBuf<DevPair> pairs;    // nlayers * nt * nt
+  DBuf<DevCoul> couls;    // nlayers
+  DBuf<uint8_t> interact;
+  DBuf<int> exOff, exItem;
+  int nexcl = 0;
+  std::vector<int> hostType;
+  // reductions
+  DBuf<double> partials, result;
+  double* hresult = nullptr;
+  DBuf<double> d2;
+  DBuf<double> trig;      // [0] max, [1] first original index holding it, [2] max before it
+  // rigid bodies; iperm: original atom -> slot
+  BodyState bodies;
+  DBuf<int> iperm;
+  // profiling
+  bool profiling = false;
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  double pairMs = 0, listMs = 0;
+  long pairLaunches = 0, listLaunches = 0;
+  double listEntries = 0;
+};
+
+
+void launch_check(const char* what);
+
+}  // namespace emdee

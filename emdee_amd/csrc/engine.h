@@ -107,7 +107,7 @@ void dev_set_models(Device* d, int layer, const std::vector<DevPair>& pairs, con
 // Host <-> device transfers of (3,N) arrays in original atom order.
 void dev_upload_positions(Device* d, const double* R);
 void dev_download_positions(Device* d, double* R);
-void dev_upload_momenta(Device* d, const double* P);      // free atoms only
+void dev_upload_momenta(Device* d, const double* P);      // free atoms; bodies via dev_bodies_assign_momenta
 void dev_download_momenta(Device* d, double* P);
 void dev_upload_forces(Device* d, int layer, const double* F);
 void dev_download_forces(Device* d, int layer, double* F);
@@ -126,6 +126,20 @@ void dev_add_forces(Device* d, int layer, const double* F);
 void dev_boost(Device* d, int layer, double CP, double CF, bool translate, double* twoKE);
 void dev_displace(Device* d, double CR, double CP);
 void dev_kinetic(Device* d, double* twoKE);
+
+// Rigid bodies (members in CSR order, original atom indices, 0-based).
+enum BodyField { BF_MASS, BF_MOI, BF_INVMOI, BF_RCM, BF_PCM, BF_Q, BF_PI, BF_OMEGA, BF_F, BF_TAU };
+void dev_bodies_setup(Device* d, const std::vector<int>& start, const std::vector<int>& orig,
+                      const std::vector<double>& mMass, const std::vector<double>& bodyMass);
+void dev_bodies_update(Device* d, double L);          // update_rigid_bodies (unwraps R, principal axes)
+void dev_bodies_boost(Device* d, int layer, double CP, double CF, bool translate, bool rotate, double* twoKE6);
+void dev_bodies_move(Device* d, double CR, double CP, double dt, bool translate, bool rotate, int mode);
+double dev_bodies_virial(Device* d, int layer);
+void dev_bodies_assign_momenta(Device* d, double* twoKE6);   // from momenta staged by dev_upload_momenta
+void dev_bodies_particle_momenta(Device* d);                 // into the staging buffer
+void dev_bodies_place(Device* d);                            // member positions from rcm, q
+void dev_bodies_get(Device* d, int field, double* host);     // host layout [nb][ncomp]
+void dev_bodies_set(Device* d, int field, const double* host);
 
 // Profiling (HIP events around the pair kernel).
 void dev_set_profiling(Device* d, bool on);

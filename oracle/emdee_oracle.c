@@ -2347,7 +2347,7 @@ static void assign_momenta(odata_t *me, const double *P, double *twoKEt, double 
     int i = me->free_[j];
     for (int x = 0; x < 3; x++) {
       me->P[3 * (i - 1) + x] = P[3 * (i - 1) + x];
-      twoKEt[x] += me->invMass[i] * P[3 * (i - 1) + x] * P[3 * (i - 1) + x];
+      twoKEt[x] += me->invMass[i] * (P[3 * (i - 1) + x] * P[3 * (i - 1) + x]);
     }
   }
   for (int ib = 1; ib <= me->nbodies; ib++) {
@@ -2361,14 +2361,14 @@ static void assign_momenta(odata_t *me, const double *P, double *twoKEt, double 
       cross3(b->delta + 3 * j, Pj, cr);
       for (int x = 0; x < 3; x++) Lv[x] += cr[x];
     }
-    for (int x = 0; x < 3; x++) twoKEt[x] += b->invMass * b->pcm[x] * b->pcm[x];
+    for (int x = 0; x < 3; x++) twoKEt[x] += b->invMass * (b->pcm[x] * b->pcm[x]);
     double C[12], B[12], t[3] = { TWO * Lv[0], TWO * Lv[1], TWO * Lv[2] }, s[3];
     matC(b->q, C);
     mul43(C, t, b->pi);
     matB(b->q, B);
     mul34t(B, b->pi, s);
     for (int x = 0; x < 3; x++) b->omega[x] = HALF * b->invMoI[x] * s[x];
-    for (int x = 0; x < 3; x++) twoKEr[x] += b->MoI[x] * b->omega[x] * b->omega[x];
+    for (int x = 0; x < 3; x++) twoKEr[x] += b->MoI[x] * (b->omega[x] * b->omega[x]);
   }
 }
 
@@ -2499,14 +2499,14 @@ void EmDee_random_momenta(tEmDee *md, double kT, bool adjust, int seed) {
       double w[3];
       for (int x = 0; x < 3; x++) w[x] = sqrt(b->invMoI[x] * kT) * rng_normal(rng);
       body_assign_momenta(b, w, 3);
-      for (int x = 0; x < 3; x++) { twoKEt[x] += b->invMass * b->pcm[x] * b->pcm[x]; twoKEr[x] += b->MoI[x] * b->omega[x] * b->omega[x]; }
+      for (int x = 0; x < 3; x++) { twoKEt[x] += b->invMass * (b->pcm[x] * b->pcm[x]); twoKEr[x] += b->MoI[x] * (b->omega[x] * b->omega[x]); }
     }
   }
   for (int j = 1; j <= me->nfree; j++) {
     int i = me->free_[j];
     double f = sqrt(me->mass[i] * kT);
     for (int x = 0; x < 3; x++) me->P[3 * (i - 1) + x] = f * rng_normal(rng);
-    for (int x = 0; x < 3; x++) twoKEt[x] += me->invMass[i] * me->P[3 * (i - 1) + x] * me->P[3 * (i - 1) + x];
+    for (int x = 0; x < 3; x++) twoKEt[x] += me->invMass[i] * (me->P[3 * (i - 1) + x] * me->P[3 * (i - 1) + x]);
   }
   if (adjust) {   /* adjust_momenta: EmDeeCode.f90:984-1018 */
     double vcm[3];
@@ -2521,14 +2521,14 @@ void EmDee_random_momenta(tEmDee *md, double kT, bool adjust, int seed) {
       int i = me->free_[j];
       for (int x = 0; x < 3; x++) {
         me->P[3 * (i - 1) + x] = me->P[3 * (i - 1) + x] - me->mass[i] * vcm[x];
-        twoKEt[x] += me->invMass[i] * me->P[3 * (i - 1) + x] * me->P[3 * (i - 1) + x];
+        twoKEt[x] += me->invMass[i] * (me->P[3 * (i - 1) + x] * me->P[3 * (i - 1) + x]);
       }
     }
     for (int b = 1; b <= me->nbodies; b++) {
       obody_t *bd = &me->body[b];
       for (int x = 0; x < 3; x++) {
         bd->pcm[x] = bd->pcm[x] - bd->mass * vcm[x];
-        twoKEt[x] += bd->invMass * bd->pcm[x] * bd->pcm[x];
+        twoKEt[x] += bd->invMass * (bd->pcm[x] * bd->pcm[x]);
       }
     }
     int dofsum = 0;
@@ -2583,7 +2583,7 @@ static void kinetic_energies(odata_t *me, int translate, int rotate, double *two
     for (int i = 1; i <= me->nbodies; i++) for (int x = 0; x < 3; x++) twoKEt[x] += me->body[i].invMass * me->body[i].pcm[x] * me->body[i].pcm[x];
     for (int j = 1; j <= me->nfree; j++) {
       int i = me->free_[j];
-      for (int x = 0; x < 3; x++) twoKEt[x] += me->invMass[i] * me->P[3 * (i - 1) + x] * me->P[3 * (i - 1) + x];
+      for (int x = 0; x < 3; x++) twoKEt[x] += me->invMass[i] * (me->P[3 * (i - 1) + x] * me->P[3 * (i - 1) + x]);
     }
   }
   if (rotate) {
@@ -2712,7 +2712,7 @@ void EmDee_verlet_step(tEmDee *md, double dt) {
       mul43(C, b->tau, c4);
       mul34t(B, c4, tb);
       Us += b->invMass * (b->F[0] * b->F[0] + b->F[1] * b->F[1] + b->F[2] * b->F[2]) +
-            (b->invMoI[0] * tb[0] * tb[0] + b->invMoI[1] * tb[1] * tb[1] + b->invMoI[2] * tb[2] * tb[2]);
+            (b->invMoI[0] * (tb[0] * tb[0]) + b->invMoI[1] * (tb[1] * tb[1]) + b->invMoI[2] * (tb[2] * tb[2]));
     }
     for (int i = 1; i <= nf; i++) {
       int j = me->free_[i];
