@@ -24,7 +24,11 @@ def test_setup_and_forces(gpu, oracle):
         a, b = gpu.download(mg, view, shape), oracle.download(mo, view, shape)
         assert np.max(np.abs(a - b)) <= tol * np.max(np.abs(b)), view
     qg, qo = gpu.download(mg, "quaternions", (nb, 4)), oracle.download(mo, "quaternions", (nb, 4))
-    assert np.max(np.abs(qg - qo)) < 1e-12
+    # principal axes come from an eigensolver: a rounding difference in the
+    # inertia tensor moves an eigenvector by about eps/(relative eigengap), so
+    # nearly symmetric bodies set the bound, not the typical 1e-15 agreement
+    assert np.max(np.abs(qg - qo)) < 1e-10
+    assert np.median(np.abs(qg - qo)) < 1e-14
     Fg, Fo = gpu.download(mg, "forces", (N, 3)), oracle.download(mo, "forces", (N, 3))
     assert np.max(np.abs(Fg - Fo)) <= 1e-12 * np.max(np.abs(Fo))
     assert mg.Energy.Potential == pytest.approx(mo.Energy.Potential, rel=1e-12)
@@ -75,5 +79,7 @@ def test_momenta_roundtrip(gpu, oracle):
     for lib, md in ((gpu, mg), (oracle, mo)):
         lib.upload(md, "momenta", P)
     assert mg.Kinetic.Total == pytest.approx(mo.Kinetic.Total, rel=1e-12)
+    # the download rebuilds member momenta from the body's omega and principal
+    # axes, so it inherits the eigenvector bound of the quaternions
     a, b = gpu.download(mg, "momenta", (N, 3)), oracle.download(mo, "momenta", (N, 3))
-    assert np.max(np.abs(a - b)) <= 1e-12 * np.max(np.abs(b))
+    assert np.max(np.abs(a - b)) <= 1e-10 * np.max(np.abs(b))
