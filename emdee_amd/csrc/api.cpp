@@ -17,6 +17,7 @@
 #include "../../include/emdee.h"
 #include "../../include/emdee_gpu.h"
 #include "engine.h"
+#include "kspace.h"
 #include "models.h"
 
 using namespace emdee;
@@ -61,6 +62,8 @@ struct System {
   std::vector<VirialRec> layerVirial;
   bool multilayer_coulomb = false;
   int nbonds = 0, nangles = 0;
+  std::vector<DevStruct> bonds, angles;     // structs.f90 lists, 0-based atoms
+  EwaldHost ewald;
   int precision = 0;
   Device* dev = nullptr;
   std::vector<double> mirror;                // host buffer for EmDee_memory_address
@@ -82,11 +85,6 @@ bool ranged(std::initializer_list<int> v, int imax) {
 double phi_fn(double x) {   // math.f90:230-237
   if (std::fabs(x) > 1e-4) return (1.0 - std::exp(-x)) / x;
   return 1.0 + 0.5 * x * ((1.0 / 3.0) * x * (1.0 - 0.25 * x) - 1.0);
-}
-
-void require_supported(System* me, const char* task) {
-  if (me->nbonds > 0 || me->nangles > 0) fatal(task, "bonds and angles are not available in this build of the engine");
-  if (me->kspace_active) fatal(task, "kspace solvers are not available in this build of the engine");
 }
 
 void mark_stale(System* me, tEmDee* md) {
@@ -166,7 +164,17 @@ void perform_initialization(System* me, tEmDee* md) {
     if (me->kspace_active) me->kspace_active = false;
     else fatal(task, "a kspace solver is required, but has not been defined");
   }
-  require_supported(me, task);
+  if (me->kspace_active) {
+    std::vector<double> R(3 * (size_t)me->natoms);
+    dev_download_positions(me->dev, R.data());
+    auto kCoul = [me](int i, int j, int l) { return me->P(i, j, l).kCoul; };
+    me->ewald = ewald_initialize(me->kspace.accuracy, kspaceRc, me->L, me->nlayers, me->atomType, me->charge,
+                                 me->bStart, me->bOrig, R.data(), kCoul);
+    dev_ewald_setup(me->dev, me->ewald.dev);
+    for (int l = 1; l <= me->nlayers; l++)
+      if (me->coul[l - 1].requires_kspace) coul_kspace_setup(me->coul[l - 1], me->ewald.alpha);
+  }
+  if (me->nbonds + me->nangles > 0) dev_structs_setup(me->dev, me->bonds, me->angles);
   upload_models(me);
   me->initialized = true;
 }
@@ -224,6 +232,19 @@ void compute_forces(tEmDee* md) {
     dev_upload_forces(me->dev, l - 1, zero.data());
   }
   E[PAIR_] = fs.Epair; E[COUL_] = fs.Ecoul; W[PAIR_] = fs.Wpair; W[COUL_] = fs.Wcoul;
+  // compute_bonds / compute_angles, EmDeeData.f90:443-550
+  bool kspace = me->coul[l - 1].requires_kspace;
+  if (me->nbonds + me->nangles > 0 && (me->bonded[l] || kspace)) {
+    double o[5];
+    dev_structs_compute(me->dev, l - 1, me->L, me->ntypes, me->bonded[l], kspace, me->ewald.alpha, me->ewald.beta, o);
+    E[BOND_] = o[0]; W[BOND_] = o[1]; E[ANGLE_] = o[2]; W[ANGLE_] = o[3];
+    E[COUL_] += o[4];
+  }
+  // compute_kspace, EmDeeData.f90:689-700; virial EmDeeCode.f90:1248-1252
+  if (kspace) {
+    E[LONG_] = dev_ewald_compute(me->dev, l - 1, me->L) + me->ewald.rigidEnergy[l - 1];
+    W[LONG_] = E[COUL_] + E[LONG_] - W[COUL_];
+  }
   md->Virial = W[0] + W[1] + W[2] + W[3] + W[4];
   if (me->nbodies != 0) {
     md->BodyVirial = dev_bodies_virial(me->dev, l - 1);
@@ -554,6 +575,7 @@ void EmDee_add_bond(tEmDee md, int i, int j, void* model) {
   const Model& m = *static_cast<Model*>(model);
   if (m.kind != BOND_HARMONIC && m.kind != BOND_NONE) fatal("add_bond", "the provided model must be a bond model");
   me->nbonds++;
+  me->bonds.push_back(DevStruct{ i - 1, j - 1, -1, m.kind, m.half_k, m.minus_k, m.x0, 0.0 });
   EmDee_ignore_pair(md, i, j);
 }
 
@@ -565,6 +587,7 @@ void EmDee_add_angle(tEmDee md, int i, int j, int k, void* model) {
   const Model& m = *static_cast<Model*>(model);
   if (m.kind != ANGLE_HARMONIC && m.kind != ANGLE_NONE) fatal("add_angle", "the provided model must be an angle model");
   me->nangles++;
+  me->angles.push_back(DevStruct{ i - 1, j - 1, k - 1, m.kind, m.half_k, m.minus_k, m.x0, 0.0 });
   EmDee_ignore_pair(md, i, j);
   EmDee_ignore_pair(md, i, k);
   EmDee_ignore_pair(md, j, k);

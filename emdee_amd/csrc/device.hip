@@ -493,23 +493,6 @@ k_pairs(double* __restrict__ F, double* __restrict__ partials,
   }
 }
 
-// Deterministic final reduction of per-block partials (fixed order).
-template <int NV>
-__global__ void k_reduce_partials(double* __restrict__ out, const double* __restrict__ partials, int nblocks) {
-  double v[NV];
-#pragma unroll
-  for (int k = 0; k < NV; k++) v[k] = 0;
-  for (int b = threadIdx.x; b < nblocks; b += BLOCK) {
-#pragma unroll
-    for (int k = 0; k < NV; k++) v[k] += partials[NV * b + k];
-  }
-  block_sum<NV>(v);
-  if (threadIdx.x == 0) {
-#pragma unroll
-    for (int k = 0; k < NV; k++) out[k] = v[k];
-  }
-}
-
 // ---- integrator -------------------------------------------------------------
 // boost, EmDeeData.f90:864-896 (free atoms): P = CP*P + CF*F;
 // kinetic_energies, EmDeeData.f90:900-922: 2KE_x = sum invMass*P_x^2.
@@ -616,6 +599,8 @@ void dev_destroy(Device* d) {
   d->Rs4.release();
   for (DBuf<int>* b : { &d->type, &d->body, &d->perm, &d->freeFlag, &d->chargedFlag, &d->tmpI, &d->key, &d->cellCount, &d->cellStart, &d->order, &d->blockSums, &d->nbr, &d->nnb, &d->flags, &d->exOff, &d->exItem, &d->iperm }) b->release();
   d->pairs.release(); d->couls.release(); d->interact.release();
+  dev_ewald_release(d);
+  dev_structs_release(d);
   if (d->hstage) (void)hipHostFree(d->hstage);
   if (d->hresult) (void)hipHostFree(d->hresult);
   (void)hipEventDestroy(d->ev0); (void)hipEventDestroy(d->ev1);

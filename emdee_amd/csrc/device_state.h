@@ -60,6 +60,23 @@ __device__ __forceinline__ void block_sum(double (&v)[NV]) {
   }
 }
 
+// Deterministic final reduction of per-block partials (fixed order).
+template <int NV>
+__global__ void k_reduce_partials(double* __restrict__ out, const double* __restrict__ partials, int nblocks) {
+  double v[NV];
+#pragma unroll
+  for (int k = 0; k < NV; k++) v[k] = 0;
+  for (int b = threadIdx.x; b < nblocks; b += BLOCK) {
+#pragma unroll
+    for (int k = 0; k < NV; k++) v[k] += partials[NV * b + k];
+  }
+  block_sum<NV>(v);
+  if (threadIdx.x == 0) {
+#pragma unroll
+    for (int k = 0; k < NV; k++) out[k] = v[k];
+  }
+}
+
 // ------------------------------------------------------------ device state
 
 template <typename T>
@@ -84,6 +101,9 @@ struct BodyState {
   DBuf<double> rcm, pcm, q, pi, omega, F, tau;     // [3][nb] or [4][nb]
   DBuf<double> part;            // reduction partials
 };
+
+struct EwaldState;
+struct StructState;
 
 struct Device {
   int N = 0, Npad = 0, ntypes = 0, nlayers = 0;
@@ -120,6 +140,9 @@ struct Device {
   // rigid bodies; iperm: original atom -> slot
   BodyState bodies;
   DBuf<int> iperm;
+  // k-space (ewald.hip) and bonded terms (bonded.hip), created on demand
+  EwaldState* ewald = nullptr;
+  StructState* structs = nullptr;
   // profiling
   bool profiling = false;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
@@ -130,5 +153,7 @@ struct Device {
 
 
 void launch_check(const char* what);
+void dev_ewald_release(Device* d);
+void dev_structs_release(Device* d);
 
 }  // namespace emdee

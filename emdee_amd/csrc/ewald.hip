@@ -1,0 +1,303 @@
+// ewald.hip -- reciprocal-space Ewald sum on the GPU (kspace_ewald.f90:188-314).
+//
+// The reference stores qeikr(k, j) = q_j e^{i k.r_j} for every k-vector and
+// charge (nvecs x Ncharged complex FP64: 1.4 GB for 12,288 charges and 7,073
+// vectors) and reads it twice.  Here nothing of that size exists:
+//   * k_ew_tables: per charge, the powers e^{2 pi i p s_x}, p = 0..nmax, for
+//     each axis by the reference's recursion (kspace_ewald.f90:232-246):
+//     3 (nmax+1) complex numbers per charge, SoA over charges.
+//   * the vectors come in rows along n3 (kspace.h).  A thread owns a charge;
+//     at a row start it forms q G1(n1) G2(n2) G3(n3) from the tables, inside
+//     a row it multiplies by e^{2 pi i s_z}.  The k data are the same for all
+//     lanes of a wavefront, so they are scalar (SGPR) loads.
+//   * k_ew_structure: S_I(k) = sum over charges of type I; each wavefront
+//     sums 4 x 64 charges of one type per k (registers, then a shuffle tree)
+//     and writes one partial per k; k_ew_sigma adds the partials in a fixed
+//     order (deterministic), forms sigma = S lambda and the energy per k.
+//   * k_ew_forces: F_j = sum_k vec_k (sigma_r(k) Im q_j e^{ik.r_j} -
+//     Re q_j e^{ik.r_j} sigma_i(k)) (kspace_ewald.f90:297), with the k range
+//     split over blocks for occupancy; k_ew_force_add adds the partial
+//     forces in a fixed order into F (cell-sorted slots).
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+
+#include "device_state.h"
+#include "kspace.h"
+
+namespace emdee {
+
+namespace {
+const double TWO_PI = 2.0 * 3.14159265358979324;   // 2 pi with global.f90:38
+constexpr int EW_BLOCK = 64;                        // one wavefront per block
+constexpr int EW_ATOMS = EWALD_GROUP / EW_BLOCK;    // charges per thread in k_ew_structure
+constexpr int EW_TARGET_WAVES = 4096;               // k-range split target
+}  // namespace
+
+struct EwaldState {
+  int nm = 0, nvecs = 0, nt = 0, nl = 0, nslots = 0, ngroups = 0;
+  int ksS = 1, kchS = 0, ksF = 1, kchF = 0;
+  DBuf<int4> kn;
+  DBuf<double4> kv;
+  DBuf<int> item, groupType, typeGroups;
+  DBuf<double> q, lambda, Fpart, part, result;
+  DBuf<double2> T, Spart, S, sigma;
+  std::vector<int> hostTypeGroups;
+};
+
+static EwaldState* ewald_of(Device* d) {
+  if (!d->ewald) d->ewald = new EwaldState();
+  return d->ewald;
+}
+
+void dev_ewald_release(Device* d) {
+  EwaldState* e = d->ewald;
+  if (!e) return;
+  e->kn.release(); e->kv.release(); e->item.release(); e->groupType.release(); e->typeGroups.release();
+  e->q.release(); e->lambda.release(); e->Fpart.release(); e->part.release(); e->result.release();
+  e->T.release(); e->Spart.release(); e->S.release(); e->sigma.release();
+  delete e;
+  d->ewald = nullptr;
+}
+
+__device__ __forceinline__ double2 cmul(double2 a, double2 b) {
+  return make_double2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+
+// e^{2 pi i p s} for p = 0..nm on each axis, kspace_ewald.f90:209-246.
+// T[(x*(nm+1) + p)*nslots + slot].
+__global__ void k_ew_tables(double2* __restrict__ T, const int* __restrict__ item, const int* __restrict__ iperm,
+                            const double* __restrict__ R, double L, int nm, int nslots, int Npad) {
+  int j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= nslots) return;
+  int i = item[j];
+  int s = i >= 0 ? iperm[i] : -1;
+  for (int x = 0; x < 3; x++) {
+    double2 Z = make_double2(1.0, 0.0);
+    if (s >= 0) {
+      double th = TWO_PI * (R[x * Npad + s] / L);
+      double sn, cs;
+      sincos(th, &sn, &cs);
+      Z = make_double2(cs, sn);
+    }
+    double2* Tx = T + (size_t)x * (nm + 1) * nslots + j;
+    Tx[0] = make_double2(1.0, 0.0);
+    double2 b = Z;
+    for (int p = 1; p <= nm; p++) {
+      Tx[(size_t)p * nslots] = b;
+      b = cmul(b, Z);
+    }
+  }
+}
+
+// q G1(n1) G2(n2) G3(n3), with G(-p) = conj(G(p)); kspace_ewald.f90:216-219.
+__device__ __forceinline__ double2 row_start(const double2* __restrict__ T, double q, int4 n, int j, int nm,
+                                             int nslots) {
+  size_t stride = (size_t)(nm + 1) * nslots;
+  double2 g1 = T[(size_t)n.x * nslots + j];
+  double2 g2 = T[stride + (size_t)abs(n.y) * nslots + j];
+  double2 g3 = T[2 * stride + (size_t)abs(n.z) * nslots + j];
+  if (n.y < 0) g2.y = -g2.y;
+  if (n.z < 0) g3.y = -g3.y;
+  double2 v = make_double2(q * g1.x, q * g1.y);
+  v = cmul(v, g2);
+  return cmul(v, g3);
+}
+
+// Partial structure factors: Spart[g*nvecs + k] = sum over the EWALD_GROUP
+// slots of group g of q e^{ik.r}.  Grid (ngroups, ksS), one wavefront each.
+__global__ void __launch_bounds__(EW_BLOCK) k_ew_structure(double2* __restrict__ Spart, const double2* __restrict__ T,
+                                                           const double* __restrict__ q, const int4* __restrict__ kn,
+                                                           int nm, int nslots, int nvecs, int kch) {
+  int g = blockIdx.x, lane = threadIdx.x;
+  int k0 = blockIdx.y * kch, k1 = min(nvecs, k0 + kch);
+  int j[EW_ATOMS];
+  double qa[EW_ATOMS];
+  double2 z[EW_ATOMS], v[EW_ATOMS];
+  size_t zoff = (2 * (size_t)(nm + 1) + 1) * nslots;   // G3(1) = e^{2 pi i s_z}
+#pragma unroll
+  for (int a = 0; a < EW_ATOMS; a++) {
+    j[a] = g * EWALD_GROUP + a * EW_BLOCK + lane;
+    qa[a] = q[j[a]];
+    z[a] = T[zoff + j[a]];
+    v[a] = make_double2(0.0, 0.0);
+  }
+  for (int k = k0; k < k1; k++) {
+    int4 n = kn[k];
+    if (k == k0 || n.w) {
+#pragma unroll
+      for (int a = 0; a < EW_ATOMS; a++) v[a] = row_start(T, qa[a], n, j[a], nm, nslots);
+    } else {
+#pragma unroll
+      for (int a = 0; a < EW_ATOMS; a++) v[a] = cmul(v[a], z[a]);
+    }
+    double sr = 0.0, si = 0.0;
+#pragma unroll
+    for (int a = 0; a < EW_ATOMS; a++) { sr += v[a].x; si += v[a].y; }
+    sr = wave_sum(sr);
+    si = wave_sum(si);
+    if (lane == 0) Spart[(size_t)g * nvecs + k] = make_double2(sr, si);
+  }
+}
+
+// S_I(k) (kspace_ewald.f90:226-228), sigma(k) = S(k) lambda (:284) and the
+// energy prefac * sum_I S_I . sigma_I (:287), one thread per k.
+__global__ void __launch_bounds__(BLOCK) k_ew_sigma(double2* __restrict__ S, double2* __restrict__ sigma,
+                                                    double* __restrict__ part, const double2* __restrict__ Spart,
+                                                    const int* __restrict__ typeGroups, const double* __restrict__ lam,
+                                                    const double4* __restrict__ kv, int nt, int nvecs) {
+  int k = blockIdx.x * blockDim.x + threadIdx.x;
+  double e[1] = { 0.0 };
+  if (k < nvecs) {
+    for (int t = 0; t < nt; t++) {
+      double sr = 0.0, si = 0.0;
+      for (int g = typeGroups[t]; g < typeGroups[t + 1]; g++) {
+        double2 p = Spart[(size_t)g * nvecs + k];
+        sr += p.x;
+        si += p.y;
+      }
+      S[(size_t)t * nvecs + k] = make_double2(sr, si);
+    }
+    double s = 0.0;
+    for (int t2 = 0; t2 < nt; t2++) {
+      double sr = 0.0, si = 0.0;
+      for (int t1 = 0; t1 < nt; t1++) {
+        double2 a = S[(size_t)t1 * nvecs + k];
+        sr += a.x * lam[t1 + nt * t2];
+        si += a.y * lam[t1 + nt * t2];
+      }
+      sigma[(size_t)t2 * nvecs + k] = make_double2(sr, si);
+      double2 b = S[(size_t)t2 * nvecs + k];
+      s += b.x * sr + b.y * si;
+    }
+    e[0] = kv[k].w * s;
+  }
+  block_sum<1>(e);
+  if (threadIdx.x == 0) part[blockIdx.x] = e[0];
+}
+
+// Partial k-space forces over the k range of blockIdx.y:
+// Fpart[(y*3 + x)*nslots + j].  Every wavefront's slots hold one charged type.
+__global__ void __launch_bounds__(EW_BLOCK) k_ew_forces(double* __restrict__ Fpart, const double2* __restrict__ T,
+                                                        const double* __restrict__ q, const int4* __restrict__ kn,
+                                                        const double4* __restrict__ kv,
+                                                        const double2* __restrict__ sigma,
+                                                        const int* __restrict__ groupType, int nm, int nslots,
+                                                        int nvecs, int kch) {
+  int j = blockIdx.x * EW_BLOCK + threadIdx.x;
+  int k0 = blockIdx.y * kch, k1 = min(nvecs, k0 + kch);
+  int t = groupType[(blockIdx.x * EW_BLOCK) / EWALD_GROUP];
+  const double2* sg = sigma + (size_t)t * nvecs;
+  double qj = q[j];
+  size_t zoff = (2 * (size_t)(nm + 1) + 1) * nslots;   // G3(1) = e^{2 pi i s_z}
+  double2 z = T[zoff + j], v = make_double2(0.0, 0.0);
+  double fx = 0.0, fy = 0.0, fz = 0.0;
+  for (int k = k0; k < k1; k++) {
+    int4 n = kn[k];
+    v = (k == k0 || n.w) ? row_start(T, qj, n, j, nm, nslots) : cmul(v, z);
+    double2 s = sg[k];
+    double4 w = kv[k];
+    double c = s.x * v.y - v.x * s.y;
+    fx += w.x * c;
+    fy += w.y * c;
+    fz += w.z * c;
+  }
+  size_t o = (size_t)blockIdx.y * 3 * nslots + j;
+  Fpart[o] = fx;
+  Fpart[o + nslots] = fy;
+  Fpart[o + 2 * (size_t)nslots] = fz;
+}
+
+// F(slot of atom) += sum over k ranges (fixed order), kspace_ewald.f90:297.
+__global__ void k_ew_force_add(double* __restrict__ F, const double* __restrict__ Fpart, const int* __restrict__ item,
+                               const int* __restrict__ iperm, int nslots, int ks, int Npad) {
+  int j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= nslots) return;
+  int i = item[j];
+  if (i < 0) return;
+  double f[3] = { 0.0, 0.0, 0.0 };
+  for (int y = 0; y < ks; y++)
+    for (int x = 0; x < 3; x++) f[x] += Fpart[((size_t)y * 3 + x) * nslots + j];
+  int s = iperm[i];
+  for (int x = 0; x < 3; x++) F[x * (size_t)Npad + s] += f[x];
+}
+
+void dev_ewald_setup(Device* d, const EwaldSetup& s) {
+  EwaldState* e = ewald_of(d);
+  e->nm = s.nm; e->nvecs = s.nvecs; e->nt = s.nt; e->nl = s.nl; e->nslots = s.nslots;
+  e->ngroups = s.nslots / EWALD_GROUP;
+  e->hostTypeGroups = s.typeGroups;
+  auto put = [&](auto& buf, const auto& host) {
+    buf.alloc(host.size());
+    HIP_CHECK(hipMemcpy(buf.p, host.data(), host.size() * sizeof(host[0]), hipMemcpyHostToDevice));
+  };
+  for (int k = 0; k < s.nvecs; k++)
+    if (s.kn[4 * k] < 0 || s.kn[4 * k] > s.nm || std::abs(s.kn[4 * k + 1]) > s.nm || std::abs(s.kn[4 * k + 2]) > s.nm)
+      fatal("kspace set-up", "k-vector outside the half space of the table");
+  if (s.nslots % EWALD_GROUP || (int)s.groupType.size() != s.nslots / EWALD_GROUP)
+    fatal("kspace set-up", "charge slots are not grouped");
+  std::vector<int4> kn(s.nvecs);
+  std::vector<double4> kv(s.nvecs);
+  for (int k = 0; k < s.nvecs; k++) {
+    kn[k] = make_int4(s.kn[4 * k], s.kn[4 * k + 1], s.kn[4 * k + 2], s.kn[4 * k + 3]);
+    kv[k] = make_double4(s.kv[4 * k], s.kv[4 * k + 1], s.kv[4 * k + 2], s.kv[4 * k + 3]);
+  }
+  put(e->kn, kn);
+  put(e->kv, kv);
+  put(e->item, s.slotItem);
+  put(e->q, s.slotQ);
+  put(e->groupType, s.groupType);
+  put(e->typeGroups, s.typeGroups);
+  put(e->lambda, s.lambda);
+  // k-range splits: enough wavefronts to fill the chip
+  e->ksS = std::max(1, std::min(s.nvecs, EW_TARGET_WAVES / std::max(1, e->ngroups)));
+  e->kchS = div_up(s.nvecs, e->ksS);
+  e->ksS = div_up(s.nvecs, e->kchS);
+  int nwF = s.nslots / EW_BLOCK;
+  e->ksF = std::max(1, std::min(s.nvecs, EW_TARGET_WAVES / std::max(1, nwF)));
+  e->kchF = div_up(s.nvecs, e->ksF);
+  e->ksF = div_up(s.nvecs, e->kchF);
+  e->T.alloc(3 * (size_t)(s.nm + 1) * s.nslots);
+  e->Spart.alloc((size_t)e->ngroups * s.nvecs);
+  e->S.alloc((size_t)s.nt * s.nvecs);
+  e->sigma.alloc((size_t)s.nt * s.nvecs);
+  e->Fpart.alloc((size_t)e->ksF * 3 * s.nslots);
+  e->part.alloc(div_up(s.nvecs, BLOCK));
+  e->result.alloc(1);
+}
+
+double dev_ewald_compute(Device* d, int layer, double L) {
+  EwaldState* e = d->ewald;
+  if (!e || e->nvecs == 0 || e->nslots == 0) return 0.0;
+  // shapes the kernels assume
+  if (e->nslots % EWALD_GROUP || e->ngroups * EWALD_GROUP != e->nslots || e->kchS * e->ksS < e->nvecs ||
+      e->kchF * e->ksF < e->nvecs || e->nm < 1)
+    fatal("kspace computation", "inconsistent k-space set-up");
+  int Np = d->Npad;
+  double* F = d->F.p + 3 * (size_t)Np * layer;
+  hipLaunchKernelGGL(k_ew_tables, dim3(div_up(e->nslots, BLOCK)), dim3(BLOCK), 0, d->stream, e->T.p, e->item.p,
+                     d->iperm.p, d->R.p, L, e->nm, e->nslots, Np);
+  launch_check("ewald tables");
+  hipLaunchKernelGGL(k_ew_structure, dim3(e->ngroups, e->ksS), dim3(EW_BLOCK), 0, d->stream, e->Spart.p, e->T.p,
+                     e->q.p, e->kn.p, e->nm, e->nslots, e->nvecs, e->kchS);
+  launch_check("ewald structure factors");
+  int gk = div_up(e->nvecs, BLOCK);
+  hipLaunchKernelGGL(k_ew_sigma, dim3(gk), dim3(BLOCK), 0, d->stream, e->S.p, e->sigma.p, e->part.p, e->Spart.p,
+                     e->typeGroups.p, e->lambda.p + (size_t)layer * e->nt * e->nt, e->kv.p, e->nt, e->nvecs);
+  launch_check("ewald sigma");
+  hipLaunchKernelGGL(k_reduce_partials<1>, dim3(1), dim3(BLOCK), 0, d->stream, e->result.p, e->part.p, gk);
+  hipLaunchKernelGGL(k_ew_forces, dim3(e->nslots / EW_BLOCK, e->ksF), dim3(EW_BLOCK), 0, d->stream, e->Fpart.p,
+                     e->T.p, e->q.p, e->kn.p, e->kv.p, e->sigma.p, e->groupType.p, e->nm, e->nslots, e->nvecs,
+                     e->kchF);
+  launch_check("ewald forces");
+  hipLaunchKernelGGL(k_ew_force_add, dim3(div_up(e->nslots, BLOCK)), dim3(BLOCK), 0, d->stream, F, e->Fpart.p,
+                     e->item.p, d->iperm.p, e->nslots, e->ksF, Np);
+  launch_check("ewald force add");
+  double E = 0;
+  HIP_CHECK(hipMemcpyAsync(d->hresult, e->result.p, sizeof(double), hipMemcpyDeviceToHost, d->stream));
+  HIP_CHECK(hipStreamSynchronize(d->stream));
+  E = d->hresult[0];
+  return E;
+}
+
+}  // namespace emdee

@@ -1,0 +1,67 @@
+// kspace.h -- host-side Ewald set-up (modelClass_kspace.f90, kspace_ewald.f90)
+// and the device inputs it produces.
+#pragma once
+#include <functional>
+#include <vector>
+
+#include "engine.h"
+
+namespace emdee {
+
+// Everything the k-space kernels need, in the device's order.
+//   * k-vectors in the reference order (kspace_ewald.f90:115-138): n3 runs
+//     fastest, so consecutive vectors that differ only by n3 += 1 form a
+//     "row" along which e^{ik.r} advances by one factor of e^{2 pi i z/L}.
+//   * charged atoms in "slots": grouped by charged type (ascending type,
+//     ascending atom index, modelClass_kspace.f90:130-150), each type padded
+//     with empty slots (item -1, charge 0) to a multiple of EWALD_GROUP so
+//     that every wavefront group holds one type only.
+constexpr int EWALD_GROUP = 256;
+
+struct EwaldSetup {
+  int nm = 0, nvecs = 0, nt = 0, nl = 0, nslots = 0;
+  std::vector<int> kn;         // [nvecs][4]: n1, n2, n3, row-start flag
+  std::vector<double> kv;      // [nvecs][4]: 2 prefac k (x, y, z), prefac
+  std::vector<int> slotItem;   // [nslots]: original atom index (0-based) or -1
+  std::vector<double> slotQ;   // [nslots]
+  std::vector<int> groupType;  // [nslots / EWALD_GROUP]: local charged type (0-based)
+  std::vector<int> typeGroups; // [nt + 1]: group range of each charged type
+  std::vector<double> lambda;  // [nl][nt][nt]: kCoul of the type pair in each layer
+};
+
+struct EwaldHost {
+  double alpha = 0, beta = 0, kmax = 0, volume = 0;
+  std::vector<double> rigidEnergy;   // per layer: sum over type pairs of lambda * Erigid
+  EwaldSetup dev;
+};
+
+// modelClass_kspace.f90:111-261 with kspace_ewald.f90:82-184.
+//   R: coordinates (3,N) in original order, used for the intrabody pairs;
+//   kCoul(i, j, l): pair(i,j,l)%kCoul with 1-based types and layer.
+EwaldHost ewald_initialize(double accuracy, double Rc, double L, int nlayers, const std::vector<int>& atomType,
+                           const std::vector<double>& charge, const std::vector<int>& bStart,
+                           const std::vector<int>& bOrig, const double* R,
+                           const std::function<double(int, int, int)>& kCoul);
+
+// math.f90:642-657
+double inverse_of_x_plus_ln_x(double y);
+
+// Device side (ewald.hip): set-up, and one evaluation that adds the k-space
+// forces to F(layer) and returns the reciprocal-space energy plus the self
+// energy sum (kspace_ewald.f90:188-314).
+void dev_ewald_setup(Device* d, const EwaldSetup& s);
+double dev_ewald_compute(Device* d, int layer, double L);
+
+// Bonded terms (bonded.hip): harmonic bonds and angles, EmDeeData.f90:443-550.
+struct DevStruct {
+  int i, j, k, kind;          // 0-based original atoms; kind of the model
+  double half_k, minus_k, x0, pad;
+};
+void dev_structs_setup(Device* d, const std::vector<DevStruct>& bonds, const std::vector<DevStruct>& angles);
+// Adds the bonded forces (and, when kspace is on, the erf discounts of the
+// excluded 1-2 and 1-3 pairs) to F(layer); out = {Ebond, Wbond, Eangle,
+// Wangle, Ecoul discount}.
+void dev_structs_compute(Device* d, int layer, double L, int ntypes, bool bonded, bool kspace, double alpha,
+                         double beta, double* out);
+
+}  // namespace emdee
