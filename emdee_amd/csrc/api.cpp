@@ -813,8 +813,34 @@ void EmDee_move(tEmDee* md, double lambda, double alpha, double dt) { displace(m
 
 // EmDeeCode.f90:1107-1211
 void EmDee_verlet_step(tEmDee* md, double dt) {
-  (void)md; (void)dt;
-  fatal("verlet step", "not available in this build of the engine");
+  System* me = S(*md);
+  double dt_2 = 0.5 * dt;
+  bool compute = md->Options.Compute;
+  int mode = md->Options.RotationMode, l = me->layer;
+  // pre_force: shadow bookkeeping, boost by dt/2, move by dt (translation and
+  // rotation regardless of the options, as the reference)
+  if (compute) dev_shadow_pre(me->dev, l - 1, dt, mode);
+  dev_bodies_boost(me->dev, l - 1, 1.0, dt_2, true, true, nullptr);
+  dev_boost(me->dev, l - 1, 1.0, dt_2, true, nullptr);
+  dev_bodies_move(me->dev, 1.0, dt, dt, true, true, mode);
+  dev_displace(me->dev, 1.0, dt);
+  compute_forces(md);
+  // post_force
+  double twoKEt[3] = { 0, 0, 0 }, kb[6] = { 0, 0, 0, 0, 0, 0 };
+  dev_bodies_boost(me->dev, l - 1, 1.0, dt_2, true, true, compute ? kb : nullptr);
+  dev_boost(me->dev, l - 1, 1.0, dt_2, true, compute ? twoKEt : nullptr);
+  if (compute) {
+    double sums[3];
+    dev_shadow_post(me->dev, l - 1, dt, mode, sums);
+    double Ks_t = sums[0], Ks_r = sums[1], Us = sums[2];
+    for (int x = 0; x < 3; x++) { md->Kinetic.TransPart[x] = 0.5 * (kb[x] + twoKEt[x]); md->Kinetic.RotPart[x] = 0.5 * kb[3 + x]; }
+    md->Kinetic.Rotational = md->Kinetic.RotPart[0] + md->Kinetic.RotPart[1] + md->Kinetic.RotPart[2];
+    md->Kinetic.Kinetic = (md->Kinetic.TransPart[0] + md->Kinetic.TransPart[1] + md->Kinetic.TransPart[2]) + md->Kinetic.Rotational;
+    md->Energy.ShadowPotential = md->Energy.ShadowPotential - dt * dt * Us / 24.0;
+    md->Kinetic.ShadowRotational = Ks_r / (6.0 * dt);
+    md->Kinetic.ShadowKinetic = (Ks_t + Ks_r) / (6.0 * dt);
+  }
+  md->Kinetic.UpToDate = compute;
 }
 
 // EmDeeCode.f90:1215-1277
@@ -822,8 +848,33 @@ void EmDee_compute_forces(tEmDee* md) { compute_forces(md); }
 
 // EmDeeCode.f90:1281-1395 (analysis observable, not part of the force engine)
 void EmDee_rdf(tEmDee md, int bins, double Rc, int pairs, int itype[], int jtype[], double g[]) {
-  (void)md; (void)bins; (void)Rc; (void)pairs; (void)itype; (void)jtype; (void)g;
-  fatal("radial distribution calculation", "not available in this build of the engine");
+  const char* task = "radial distribution calculation";
+  const double Pi4_3 = 4.188790204786391;
+  System* me = S(md);
+  int nt = me->ntypes;
+  for (int p = 0; p < pairs; p++)
+    if (!ranged({ itype[p], jtype[p] }, nt)) fatal(task, "at least one provided type index is out of range");
+  std::vector<uint8_t> pairOn((size_t)(nt + 1) * (nt + 1), 0);
+  int maxtype = 0;
+  for (int p = 0; p < pairs; p++) {
+    pairOn[itype[p] * (nt + 1) + jtype[p]] = pairOn[jtype[p] * (nt + 1) + itype[p]] = 1;
+    maxtype = std::max(maxtype, std::max(itype[p], jtype[p]));
+  }
+  auto symm1D = [](int i, int j) { int x = std::min(i, j) - 1, y = std::max(i, j) - 1; return x + (y + 1) * y / 2 + 1; };
+  int npt = maxtype > 0 ? symm1D(maxtype, maxtype) : 0;
+  std::vector<long long> count;
+  if (npt > 0) dev_rdf(me->dev, bins, Rc, me->L, pairOn, npt, count);
+  double invL = 1.0 / me->L;
+  double norm = Pi4_3 * std::pow(Rc * invL / bins, 3);
+  for (int p = 0; p < pairs; p++) {
+    int i = itype[p], j = jtype[p];
+    long Ni = 0, Nj = 0;
+    for (int a = 0; a < me->natoms; a++) { Ni += me->atomType[a] == i; Nj += me->atomType[a] == j; }
+    for (int b = 0; b < bins; b++) {
+      double r = count[(size_t)b + (size_t)bins * (symm1D(i, j) - 1)] / norm / (3 * (b + 1) * b + 1);
+      g[(size_t)b + (size_t)bins * p] = (i == j ? 2.0 * r : r) / ((double)Ni * (double)Nj);
+    }
+  }
 }
 
 // ---- extensions (include/emdee_gpu.h) --------------------------------------

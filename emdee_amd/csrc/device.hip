@@ -527,6 +527,78 @@ __global__ void k_displace(double* __restrict__ R, const double* __restrict__ P,
   for (int x = 0; x < 3; x++) R[x * Npad + s] = CR * R[x * Npad + s] + CP * P[x * Npad + s] * im;
 }
 
+// ---- shadow terms of EmDee_verlet_step, free atoms (EmDeeCode.f90:1167-1204)
+// s0 = 2.5 R + dt/2 invMass (P - dt/2 F), stored by ORIGINAL atom index so
+// that a list rebuild (which permutes the slots) inside the step keeps it.
+__global__ void k_free_shadow_pre(double* __restrict__ s0, const double* __restrict__ R, const double* __restrict__ P,
+                                  const double* __restrict__ F, const double* __restrict__ invMass,
+                                  const int* __restrict__ freeFlag, const int* __restrict__ perm, double dt, int N,
+                                  int Npad) {
+  int s = blockIdx.x * BLOCK + threadIdx.x;
+  if (s >= N || !freeFlag[s]) return;
+  double dt_2 = 0.5 * dt, im = invMass[s];
+  int a = perm[s];
+  for (int x = 0; x < 3; x++)
+    s0[x * (size_t)N + a] = 2.5 * R[x * Npad + s] + dt_2 * im * (P[x * Npad + s] - dt_2 * F[x * Npad + s]);
+}
+
+// partials of {Ks_t, 0, Us}
+__global__ void k_free_shadow_post(double* __restrict__ part, const double* __restrict__ s0,
+                                   const double* __restrict__ R, const double* __restrict__ P,
+                                   const double* __restrict__ F, const double* __restrict__ invMass,
+                                   const int* __restrict__ freeFlag, const int* __restrict__ perm, double dt, int N,
+                                   int Npad) {
+  int s = blockIdx.x * BLOCK + threadIdx.x;
+  double v[3] = { 0.0, 0.0, 0.0 };
+  if (s < N && freeFlag[s]) {
+    double dt_2 = 0.5 * dt, im = invMass[s];
+    int a = perm[s];
+    double rdot[3], p[3], f[3];
+    for (int x = 0; x < 3; x++) {
+      p[x] = P[x * Npad + s];
+      f[x] = F[x * Npad + s];
+      rdot[x] = 2.5 * R[x * Npad + s] + dt * im * (p[x] + dt_2 * f[x]) - s0[x * (size_t)N + a];
+    }
+    v[0] = rdot[0] * p[0] + rdot[1] * p[1] + rdot[2] * p[2];
+    v[2] = im * (f[0] * f[0] + f[1] * f[1] + f[2] * f[2]);
+  }
+  block_sum<3>(v);
+  if (threadIdx.x == 0) for (int k = 0; k < 3; k++) part[3 * blockIdx.x + k] = v[k];
+}
+
+// ---- EmDee_rdf (EmDeeCode.f90:1281-1395): pair counts per bin over the
+// current neighbor lists, each pair once (original index a < b), with the
+// reference's distance arithmetic (Rs = invL*R, d - anint(d), no FMA so that
+// the bin of every pair is the reference's).  Integer atomics: exact counts.
+__global__ void k_rdf(unsigned long long* __restrict__ count, const double* __restrict__ R,
+                      const int* __restrict__ nbr, const int* __restrict__ nnb, const int* __restrict__ perm,
+                      const int* __restrict__ type, const uint8_t* __restrict__ pairOn, int nt, double invL,
+                      double Rc2, double binsByRc, int bins, int N, int Npad) {
+  int s = blockIdx.x * BLOCK + threadIdx.x;
+  if (s >= N) return;
+  int a = perm[s], it = type[s] + 1;
+  double ri[3] = { __dmul_rn(invL, R[s]), __dmul_rn(invL, R[Npad + s]), __dmul_rn(invL, R[2 * Npad + s]) };
+  int n = nnb[s];
+  for (int k = 0; k < n; k++) {
+    int t = nbr[(size_t)k * Npad + s];
+    if (perm[t] <= a) continue;
+    int jt = type[t] + 1;
+    if (!pairOn[it * (nt + 1) + jt]) continue;
+    double r2 = 0.0;
+    for (int x = 0; x < 3; x++) {
+      double d = __dsub_rn(ri[x], __dmul_rn(invL, R[x * Npad + t]));
+      d = __dsub_rn(d, round(d));
+      r2 = __dadd_rn(r2, __dmul_rn(d, d));
+    }
+    if (r2 < Rc2) {
+      int bin = (int)__dmul_rn(sqrt(r2), binsByRc);
+      int lo = min(it, jt) - 1, hi = max(it, jt) - 1;
+      int sp = lo + (hi + 1) * hi / 2;   // symm1D - 1
+      atomicAdd(&count[bin + (size_t)bins * sp], 1ULL);
+    }
+  }
+}
+
 // ------------------------------------------------------------ host side
 
 void launch_check(const char* what) {
@@ -595,7 +667,7 @@ Device* dev_create(const DeviceSetup& s) {
 void dev_destroy(Device* d) {
   if (!d) return;
   (void)hipStreamSynchronize(d->stream);
-  for (DBuf<double>* b : { &d->R, &d->P, &d->R0, &d->invMass, &d->charge, &d->F, &d->tmp3, &d->tmpF, &d->tmp1, &d->stage, &d->partials, &d->result, &d->d2, &d->trig }) b->release();
+  for (DBuf<double>* b : { &d->R, &d->P, &d->R0, &d->invMass, &d->charge, &d->F, &d->tmp3, &d->tmpF, &d->tmp1, &d->stage, &d->partials, &d->result, &d->d2, &d->trig, &d->shadow }) b->release();
   d->Rs4.release();
   for (DBuf<int>* b : { &d->type, &d->body, &d->perm, &d->freeFlag, &d->chargedFlag, &d->tmpI, &d->key, &d->cellCount, &d->cellStart, &d->order, &d->blockSums, &d->nbr, &d->nnb, &d->flags, &d->exOff, &d->exItem, &d->iperm }) b->release();
   d->pairs.release(); d->couls.release(); d->interact.release();
@@ -876,6 +948,59 @@ void dev_displace(Device* d, double CR, double CP) {
   if (N == 0) return;
   hipLaunchKernelGGL(k_displace, dim3(div_up(N, BLOCK)), dim3(BLOCK), 0, d->stream, d->R.p, d->P.p, d->invMass.p, d->freeFlag.p, CR, CP, N, d->Npad);
   launch_check("displace");
+}
+
+void dev_shadow_pre(Device* d, int layer, double dt, int mode) {
+  int N = d->N, Np = d->Npad;
+  int nb = d->bodies.nb;
+  d->shadow.alloc(3 * (size_t)std::max(N, 1) + 7 * (size_t)nb);
+  if (N > 0)
+    hipLaunchKernelGGL(k_free_shadow_pre, dim3(div_up(N, BLOCK)), dim3(BLOCK), 0, d->stream, d->shadow.p, d->R.p, d->P.p,
+                       d->F.p + 3 * (size_t)Np * layer, d->invMass.p, d->freeFlag.p, d->perm.p, dt, N, Np);
+  launch_check("shadow pre");
+  dev_bodies_shadow_pre(d, d->shadow.p + 3 * (size_t)std::max(N, 1), dt, mode);
+}
+
+void dev_shadow_post(Device* d, int layer, double dt, int mode, double* out3) {
+  int N = d->N, Np = d->Npad;
+  double bsum[3];
+  dev_bodies_shadow_post(d, d->shadow.p + 3 * (size_t)std::max(N, 1), dt, mode, bsum);
+  double fsum[3] = { 0.0, 0.0, 0.0 };
+  if (N > 0) {
+    int g = div_up(N, BLOCK);
+    hipLaunchKernelGGL(k_free_shadow_post, dim3(g), dim3(BLOCK), 0, d->stream, d->partials.p, d->shadow.p, d->R.p,
+                       d->P.p, d->F.p + 3 * (size_t)Np * layer, d->invMass.p, d->freeFlag.p, d->perm.p, dt, N, Np);
+    launch_check("shadow post");
+    hipLaunchKernelGGL(k_reduce_partials<3>, dim3(1), dim3(BLOCK), 0, d->stream, d->result.p + 40, d->partials.p, g);
+    HIP_CHECK(hipMemcpyAsync(d->hresult + 40, d->result.p + 40, 3 * sizeof(double), hipMemcpyDeviceToHost, d->stream));
+    HIP_CHECK(hipStreamSynchronize(d->stream));
+    for (int k = 0; k < 3; k++) fsum[k] = d->hresult[40 + k];
+  }
+  // body terms first, then free atoms, as the per-thread sums of the reference
+  for (int k = 0; k < 3; k++) out3[k] = bsum[k] + fsum[k];
+}
+
+void dev_rdf(Device* d, int bins, double Rc, double L, const std::vector<uint8_t>& pairOn, int npt,
+             std::vector<long long>& count) {
+  int N = d->N;
+  count.assign((size_t)bins * npt, 0);
+  if (N == 0 || !d->built) return;
+  DBuf<unsigned long long> c;
+  DBuf<uint8_t> on;
+  c.alloc((size_t)bins * npt);
+  on.alloc(pairOn.size());
+  HIP_CHECK(hipMemsetAsync(c.p, 0, sizeof(unsigned long long) * bins * npt, d->stream));
+  HIP_CHECK(hipMemcpyAsync(on.p, pairOn.data(), pairOn.size(), hipMemcpyHostToDevice, d->stream));
+  double invL = 1.0 / L, invL2 = invL * invL, Rc2 = Rc * Rc * invL2, binsByRc = bins / (Rc * invL);
+  hipLaunchKernelGGL(k_rdf, dim3(div_up(N, BLOCK)), dim3(BLOCK), 0, d->stream, c.p, d->R.p, d->nbr.p, d->nnb.p,
+                     d->perm.p, d->type.p, on.p, d->ntypes, invL, Rc2, binsByRc, bins, N, d->Npad);
+  launch_check("rdf");
+  std::vector<unsigned long long> h((size_t)bins * npt);
+  HIP_CHECK(hipMemcpyAsync(h.data(), c.p, sizeof(unsigned long long) * h.size(), hipMemcpyDeviceToHost, d->stream));
+  HIP_CHECK(hipStreamSynchronize(d->stream));
+  for (size_t k = 0; k < h.size(); k++) count[k] = (long long)h[k];
+  c.release();
+  on.release();
 }
 
 void dev_set_profiling(Device* d, bool on) { d->profiling = on; d->pairMs = d->listMs = 0; d->pairLaunches = d->listLaunches = 0; }

@@ -141,6 +141,7 @@ struct Device {
   BodyState bodies;
   DBuf<int> iperm;
   // k-space (ewald.hip) and bonded terms (bonded.hip), created on demand
+  DBuf<double> shadow;    // EmDee_verlet_step: s0 [3][N] (original order), then r0 [3][nb], q0 [4][nb]
   EwaldState* ewald = nullptr;
   StructState* structs = nullptr;
   // profiling
@@ -155,5 +156,7 @@ struct Device {
 void launch_check(const char* what);
 void dev_ewald_release(Device* d);
 void dev_structs_release(Device* d);
+void dev_bodies_shadow_pre(Device* d, double* sh, double dt, int mode);
+void dev_bodies_shadow_post(Device* d, const double* sh, double dt, int mode, double* out3);
 
 }  // namespace emdee

@@ -141,6 +141,16 @@ void dev_bodies_place(Device* d);                            // member positions
 void dev_bodies_get(Device* d, int field, double* host);     // host layout [nb][ncomp]
 void dev_bodies_set(Device* d, int field, const double* host);
 
+// Shadow terms of EmDee_verlet_step (EmDeeCode.f90:1107-1211): pre stores
+// s0, r0, q0 from the current state; post returns {Ks_t, Ks_r, Us}.
+void dev_shadow_pre(Device* d, int layer, double dt, int mode);
+void dev_shadow_post(Device* d, int layer, double dt, int mode, double* out3);
+
+// EmDee_rdf pair counts over the current lists: count[bin + bins*(symm1D-1)];
+// pairOn is the (nt+1)^2 table of requested type pairs (1-based types).
+void dev_rdf(Device* d, int bins, double Rc, double L, const std::vector<uint8_t>& pairOn, int npt,
+             std::vector<long long>& count);
+
 // Profiling (HIP events around the pair kernel).
 void dev_set_profiling(Device* d, bool on);
 void dev_kernel_stats(Device* d, double* out, int n);

@@ -612,6 +612,68 @@ __global__ void k_body_reduce(double* __restrict__ out, const double* __restrict
   if (threadIdx.x == 0) for (int k = 0; k < NV; k++) out[k] = v[k];
 }
 
+// ---- shadow terms of EmDee_verlet_step (EmDeeCode.f90:1107-1211) ----------
+
+// virtual_rotation (EmDeeCode.f90:1150-1163): a copy of body b given the
+// quaternion momenta pi + C(q) dt tau and rotated by dt; returns its q.
+__device__ void virtual_rotation(const BodyState& B, int b, double dt, int mode, double* qout) {
+  BodyReg r;
+  load_body(B, b, r);
+  int nb = B.nb;
+  double C[12], t[3], c4[4], in[4];
+  for (int x = 0; x < 3; x++) t[x] = dt * B.tau.p[x * nb + b];
+  matC(r.q, C);
+  mul43(C, t, c4);
+  for (int k = 0; k < 4; k++) in[k] = r.pi[k] + c4[k];
+  assign_pi(r, in);
+  if (mode != 0) rotate_no_squish(r, dt, mode); else rotate_exact(r, dt);
+  for (int k = 0; k < 4; k++) qout[k] = r.q[k];
+}
+
+// pre_force, body part: r0 = 2.5 rcm + dt/2 invMass (pcm - dt/2 F);
+// q0 = q(-dt)/2 - 3 q.  sh holds [3][nb] r0 then [4][nb] q0.
+__global__ void k_body_shadow_pre(BodyState B, double* __restrict__ sh, double dt, int mode) {
+  int b = blockIdx.x * blockDim.x + threadIdx.x;
+  if (b >= B.nb) return;
+  int nb = B.nb;
+  double dt_2 = 0.5 * dt, im = B.invMass.p[b], qv[4];
+  for (int x = 0; x < 3; x++)
+    sh[x * nb + b] = 2.5 * B.rcm.p[x * nb + b] + dt_2 * im * (B.pcm.p[x * nb + b] - dt_2 * B.F.p[x * nb + b]);
+  virtual_rotation(B, b, -dt, mode, qv);
+  for (int k = 0; k < 4; k++) sh[(3 + k) * nb + b] = 0.5 * qv[k] - 3.0 * B.q.p[k * nb + b];
+}
+
+// post_force, body part: partials of {Ks_t, Ks_r, Us}.
+__global__ void k_body_shadow_post(BodyState B, const double* __restrict__ sh, double dt, int mode) {
+  int b = blockIdx.x * blockDim.x + threadIdx.x;
+  double v[3] = { 0.0, 0.0, 0.0 };
+  if (b < B.nb) {
+    int nb = B.nb;
+    double dt_2 = 0.5 * dt, im = B.invMass.p[b];
+    double q[4], pi[4], F[3], tau[3], pcm[3];
+    for (int k = 0; k < 4; k++) { q[k] = B.q.p[k * nb + b]; pi[k] = B.pi.p[k * nb + b]; }
+    for (int x = 0; x < 3; x++) { F[x] = B.F.p[x * nb + b]; tau[x] = B.tau.p[x * nb + b]; pcm[x] = B.pcm.p[x * nb + b]; }
+    double rdot[3];
+    for (int x = 0; x < 3; x++) rdot[x] = 2.5 * B.rcm.p[x * nb + b] + dt * im * (pcm[x] + dt_2 * F[x]) - sh[x * nb + b];
+    v[0] = rdot[0] * pcm[0] + rdot[1] * pcm[1] + rdot[2] * pcm[2];
+    double qv[4], qdot[4];
+    virtual_rotation(B, b, dt, mode, qv);
+    for (int k = 0; k < 4; k++) qdot[k] = sh[(3 + k) * nb + b] + 1.5 * q[k] + qv[k];
+    double qq = qdot[0] * q[0] + qdot[1] * q[1] + qdot[2] * q[2] + qdot[3] * q[3], s = 0.0;
+    for (int k = 0; k < 4; k++) s += (qdot[k] - qq * q[k]) * pi[k];
+    v[1] = s;
+    double Bm[12], C[12], c4[4], tb[3];
+    matB(q, Bm);
+    matC(q, C);
+    mul43(C, tau, c4);
+    mul34t(Bm, c4, tb);
+    v[2] = im * (F[0] * F[0] + F[1] * F[1] + F[2] * F[2]) +
+           (B.invMoI.p[b] * (tb[0] * tb[0]) + B.invMoI.p[nb + b] * (tb[1] * tb[1]) + B.invMoI.p[2 * nb + b] * (tb[2] * tb[2]));
+  }
+  block_sum<3>(v);
+  if (threadIdx.x == 0) for (int k = 0; k < 3; k++) B.part.p[3 * blockIdx.x + k] = v[k];
+}
+
 // ------------------------------------------------------------ host side
 
 static DBuf<double>* field(BodyState& B, int f, int& ncomp) {
@@ -744,6 +806,26 @@ void dev_bodies_set(Device* d, int f, const double* host) {
   for (int b = 0; b < B.nb; b++) for (int k = 0; k < nc; k++) t[(size_t)k * B.nb + b] = host[(size_t)nc * b + k];
   HIP_CHECK(hipMemcpyAsync(a->p, t.data(), sizeof(double) * t.size(), hipMemcpyHostToDevice, d->stream));
   HIP_CHECK(hipStreamSynchronize(d->stream));
+}
+
+void dev_bodies_shadow_pre(Device* d, double* sh, double dt, int mode) {
+  BodyState& B = d->bodies;
+  if (B.nb == 0) return;
+  hipLaunchKernelGGL(k_body_shadow_pre, dim3(div_up(B.nb, BLOCK)), dim3(BLOCK), 0, d->stream, B, sh, dt, mode);
+  launch_check("body shadow pre");
+}
+
+void dev_bodies_shadow_post(Device* d, const double* sh, double dt, int mode, double* out3) {
+  BodyState& B = d->bodies;
+  out3[0] = out3[1] = out3[2] = 0.0;
+  if (B.nb == 0) return;
+  int g = div_up(B.nb, BLOCK);
+  hipLaunchKernelGGL(k_body_shadow_post, dim3(g), dim3(BLOCK), 0, d->stream, B, sh, dt, mode);
+  launch_check("body shadow post");
+  hipLaunchKernelGGL(k_body_reduce<3>, dim3(1), dim3(BLOCK), 0, d->stream, d->result.p + 32, B.part.p, g);
+  HIP_CHECK(hipMemcpyAsync(d->hresult + 32, d->result.p + 32, 3 * sizeof(double), hipMemcpyDeviceToHost, d->stream));
+  HIP_CHECK(hipStreamSynchronize(d->stream));
+  for (int k = 0; k < 3; k++) out3[k] = d->hresult[32 + k];
 }
 
 }  // namespace emdee
