@@ -363,6 +363,10 @@ k_build_list(int* __restrict__ nbr, int* __restrict__ nnb, int* __restrict__ fla
       }
     }
     nnb[i] = cnt;
+    // pad the list to a multiple of 4 with the atom itself (zero separation):
+    // the mixed-form kernel reads whole groups of four without a bound test
+    if (cnt <= cap)
+      for (int k = cnt; k < ((cnt + 3) & ~3); k++) nbr[(size_t)k * Npad + i] = i;
     if (cnt > cap) atomicMax(&flags[0], cnt);
   }
   double v[1] = { (double)cnt };
@@ -391,13 +395,27 @@ __device__ __forceinline__ double fast_rcp(double x) {
 
 struct PairAcc { double fx, fy, fz, Ep, Ec, Wp, Wc; };
 
-template <typename T, bool ENERGY>
+// LJ: the model is pair_lj_cut with no modifier or an energy shift, and no
+// pair is charged -- the single-model systems of the LJ workloads, compiled
+// without the generic dispatch so that the kernel needs fewer registers.
+template <typename T, bool ENERGY, bool LJ>
 __device__ __forceinline__ void pair_body(const DevPair& p, const DevCoul& cm, bool ijcharged, double qiqj,
                                           double dx, double dy, double dz, double r2, double invL2, PairAcc& a) {
   T invR2;
   if (std::is_same<T, float>::value) invR2 = T(invL2) * rcp<T>(T(r2));
   else invR2 = T(invL2 * fast_rcp(r2));
   T E, W;
+  if (LJ) {
+    T sr2 = T(p.c2) * invR2, sr6 = sr2 * sr2 * sr2, sr12 = sr6 * sr6;
+    W = T(p.c1) * (sr12 + sr12 - sr6);
+    if (ENERGY) a.Ep += (double)(T(p.c0) * (sr12 - sr6) + T(p.eshift));
+    a.Wp += (double)W;
+    T g = W * invR2;
+    a.fx += (double)(g * T(dx));
+    a.fy += (double)(g * T(dy));
+    a.fz += (double)(g * T(dz));
+    return;
+  }
   pair_model<T>(p, invR2, E, W);
   if (p.modifier != MOD_NONE) {
     T invR = (p.modifier == MOD_SHIFTED_FORCE || p.modifier == MOD_SMOOTHED || p.modifier == MOD_SHIFTED_SMOOTHED)
@@ -423,7 +441,7 @@ __device__ __forceinline__ void pair_body(const DevPair& p, const DevCoul& cm, b
 }
 
 // UNIFORM: a single atom type, so the model record is loaded once.
-template <bool ENERGY, bool MIXED, bool UNIFORM>
+template <bool ENERGY, bool MIXED, bool UNIFORM, bool LJ>
 __global__ void __launch_bounds__(BLOCK)
 k_pairs(double* __restrict__ F, double* __restrict__ partials,
         const double4* __restrict__ Rs4, const int* __restrict__ nbr, const int* __restrict__ nnb,
@@ -442,7 +460,7 @@ k_pairs(double* __restrict__ F, double* __restrict__ partials,
     bool ich = charged[i] != 0;
     const DevPair* prow = pairs + ti * ntypes;
     DevPair pu;
-    if (UNIFORM) pu = prow[0];
+    if (UNIFORM) pu = pairs[0];   // uniform address: scalar loads
     const int n = nnb[i];
     const int* __restrict__ col = nbr + i;
     auto one = [&](int j, const double4& rj) {
@@ -454,36 +472,244 @@ k_pairs(double* __restrict__ F, double* __restrict__ partials,
         r2 = dx * dx + dy * dy + dz * dz;
       }
       if (r2 < Rc2) {
-        if (UNIFORM) {
+        if (LJ) {
+          pair_body<T, ENERGY, true>(pu, cm, false, 0.0, dx, dy, dz, r2, invL2, a);
+        } else if (UNIFORM) {
           bool jc = ich && pu.coulomb && charged[j];
           double qiqj = jc ? pu.kCoul * qi * charge[j] : 0.0;
-          pair_body<T, ENERGY>(pu, cm, jc, qiqj, dx, dy, dz, r2, invL2, a);
+          pair_body<T, ENERGY, false>(pu, cm, jc, qiqj, dx, dy, dz, r2, invL2, a);
         } else {
           const DevPair p = prow[(int)rj.w];
           bool jc = ich && p.coulomb && charged[j];
           double qiqj = jc ? p.kCoul * qi * charge[j] : 0.0;
-          pair_body<T, ENERGY>(p, cm, jc, qiqj, dx, dy, dz, r2, invL2, a);
+          pair_body<T, ENERGY, false>(p, cm, jc, qiqj, dx, dy, dz, r2, invL2, a);
         }
       }
     };
-    int k = 0;
-    for (; k + 4 <= n; k += 4) {
-      int j0 = __builtin_nontemporal_load(col + (size_t)(k + 0) * Npad);
-      int j1 = __builtin_nontemporal_load(col + (size_t)(k + 1) * Npad);
-      int j2 = __builtin_nontemporal_load(col + (size_t)(k + 2) * Npad);
-      int j3 = __builtin_nontemporal_load(col + (size_t)(k + 3) * Npad);
-      double4 r0 = Rs4[j0], r1 = Rs4[j1], r2 = Rs4[j2], r3 = Rs4[j3];
-      one(j0, r0); one(j1, r1); one(j2, r2); one(j3, r3);
-    }
-    for (; k < n; k++) {
-      int j = __builtin_nontemporal_load(col + (size_t)k * Npad);
-      one(j, Rs4[j]);
+    {
+      int k = 0;
+      for (; k + 4 <= n; k += 4) {
+        int j0 = __builtin_nontemporal_load(col + (size_t)(k + 0) * Npad);
+        int j1 = __builtin_nontemporal_load(col + (size_t)(k + 1) * Npad);
+        int j2 = __builtin_nontemporal_load(col + (size_t)(k + 2) * Npad);
+        int j3 = __builtin_nontemporal_load(col + (size_t)(k + 3) * Npad);
+        double4 r0 = Rs4[j0], r1 = Rs4[j1], r2 = Rs4[j2], r3 = Rs4[j3];
+        one(j0, r0); one(j1, r1); one(j2, r2); one(j3, r3);
+      }
+      for (; k < n; k++) {
+        int j = __builtin_nontemporal_load(col + (size_t)k * Npad);
+        one(j, Rs4[j]);
+      }
     }
     F[i] = L * a.fx;
     F[Npad + i] = L * a.fy;
     F[2 * Npad + i] = L * a.fz;
   }
   double v[4] = { a.Ep, a.Ec, a.Wp, a.Wc };
+  block_sum<4>(v);
+  if (threadIdx.x == 0) {
+    partials[4 * blk + 0] = 0.5 * v[0];
+    partials[4 * blk + 1] = 0.5 * v[1];
+    partials[4 * blk + 2] = 0.5 * v[2];
+    partials[4 * blk + 3] = 0.5 * v[3];
+  }
+}
+
+// ---- mixed form: FP32 pair arithmetic, FP64 accumulation -----------------
+// Positions are also kept as 32-bit fixed point of the scaled coordinate,
+// u = round(2^32 (s - floor(s))) mod 2^32.  The wrapped integer difference
+// (int)(u_i - u_j) is the minimum-image separation in units of L/2^32, so a
+// neighbor costs one 16-byte gather {ux, uy, uz, type} and three integer
+// subtractions, with no rounding-to-image step.  Converted to FP32 it carries
+// a relative error below 1e-7.  Pairs whose FP32 r^2 falls within 2e-6 of the
+// cutoff are decided by the FP64 test of the plain form on the FP64 scaled
+// positions, so both forms keep exactly the same set of interacting pairs.
+// Pair forces are summed in FP32 over groups of four neighbors, and each
+// group is added to FP64 accumulators.
+struct MixedCut {
+  float K;          // (2^32 / L)^2: invR2 (length^-2) = K / r2 (fixed-point units)
+  float lo, hi;     // FP32 band around the cutoff in fixed-point units^2
+  double Rc2s;      // FP64 cutoff on scaled coordinates (the plain form's)
+  double unit;      // L / 2^32
+};
+
+__global__ void k_scale4q(double4* __restrict__ Rs4, int4* __restrict__ Q4, float* __restrict__ qf,
+                          const double* __restrict__ R, const int* __restrict__ type, const double* __restrict__ charge,
+                          const int* __restrict__ charged, double L, int N, int Npad) {
+  int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= N) return;
+  double v[3] = { R[s] / L, R[Npad + s] / L, R[2 * Npad + s] / L };
+  unsigned int u[3];
+  for (int x = 0; x < 3; x++) {
+    double f = v[x] - floor(v[x]);
+    u[x] = (unsigned int)(unsigned long long)llrint(f * 4294967296.0);
+  }
+  Rs4[s] = make_double4(v[0], v[1], v[2], (double)type[s]);
+  Q4[s] = make_int4((int)u[0], (int)u[1], (int)u[2], type[s]);
+  qf[s] = charged[s] ? (float)charge[s] : 0.0f;
+}
+
+__device__ __noinline__ bool exact_inside(const double4* __restrict__ Rs4, int i, int j, double Rc2s) {
+  double4 a = Rs4[i], b = Rs4[j];
+  double dx = a.x - b.x, dy = a.y - b.y, dz = a.z - b.z;
+  dx = dx - rint(dx); dy = dy - rint(dy); dz = dz - rint(dz);
+  double r2;
+  {
+#pragma clang fp contract(off)
+    r2 = dx * dx + dy * dy + dz * dz;
+  }
+  return r2 < Rc2s;
+}
+
+struct MixedAcc { float fx, fy, fz, Ep, Ec, Wp, Wc; };
+
+template <bool ENERGY, bool LJ>
+__device__ __forceinline__ void pair_body_mx(const DevPair& p, const DevCoul& cm, bool jc, float qiqj, float dx,
+                                             float dy, float dz, float invR2, MixedAcc& a) {
+  float E, W;
+  if (LJ) {   // see pair_body
+    float sr2 = float(p.c2) * invR2, sr6 = sr2 * sr2 * sr2;
+    W = float(p.c1) * sr6 * (sr6 + sr6 - 1.0f);
+    if (ENERGY) a.Ep += float(p.c0) * sr6 * (sr6 - 1.0f) + float(p.eshift);
+    a.Wp += W;
+    float g = W * invR2;
+    a.fx += g * dx;
+    a.fy += g * dy;
+    a.fz += g * dz;
+    return;
+  }
+  pair_model<float>(p, invR2, E, W);
+  if (p.modifier != MOD_NONE) {
+    float invR = (p.modifier == MOD_SHIFTED_FORCE || p.modifier == MOD_SMOOTHED || p.modifier == MOD_SHIFTED_SMOOTHED)
+                     ? sqrtf(invR2) : 0.0f;
+    apply_modifier<float>(p.modifier, float(p.eshift), float(p.fshift), float(p.factor), float(p.Rm2fac), invR,
+                          invR2, E, W);
+  }
+  if (ENERGY) a.Ep += E;
+  a.Wp += W;
+  float Wsum = W;
+  if (jc) {
+    float invR = sqrtf(invR2);
+    coul_model<float, ENERGY>(cm, invR, invR2, E, W);
+    apply_modifier<float>(cm.modifier, float(cm.eshift), float(cm.fshift), float(cm.factor), float(cm.Rm2fac), invR,
+                          invR2, E, W);
+    if (ENERGY) a.Ec += qiqj * E;
+    float Wq = qiqj * W;
+    a.Wc += Wq;
+    Wsum = Wsum + Wq;
+  }
+  float g = Wsum * invR2;
+  a.fx += g * dx;
+  a.fy += g * dy;
+  a.fz += g * dz;
+}
+
+template <bool ENERGY, bool UNIFORM, bool LJ>
+__global__ void __launch_bounds__(BLOCK)
+k_pairs_mx(double* __restrict__ F, double* __restrict__ partials, const int4* __restrict__ Q4,
+           const double4* __restrict__ Rs4, const int* __restrict__ nbr, const int* __restrict__ nnb,
+           const float* __restrict__ qf, const DevPair* __restrict__ pairs, const DevCoul* __restrict__ coulp,
+           int ntypes, MixedCut mc, int N, int Npad) {
+  int blk = xcd_block();
+  int i = blk * BLOCK + threadIdx.x;
+  double fx = 0, fy = 0, fz = 0, Ep = 0, Ec = 0, Wp = 0, Wc = 0;
+  if (i < N) {
+    const DevCoul cm = *coulp;
+    const int4 qi = Q4[i];
+    const float qqi = qf[i];
+    const DevPair* prow = pairs + qi.w * ntypes;
+    DevPair pu;
+    if (UNIFORM) pu = pairs[0];   // uniform address: scalar loads, no VGPRs
+    const int n = nnb[i];
+    const int* __restrict__ col = nbr + i;
+    auto one = [&](int j, const int4& qj, MixedAcc& a) {
+      float dx = (float)(int)((unsigned int)qi.x - (unsigned int)qj.x);
+      float dy = (float)(int)((unsigned int)qi.y - (unsigned int)qj.y);
+      float dz = (float)(int)((unsigned int)qi.z - (unsigned int)qj.z);
+      float r2 = dx * dx + dy * dy + dz * dz;
+      bool in = r2 < mc.lo;
+      if (!in && r2 < mc.hi) in = exact_inside(Rs4, i, j, mc.Rc2s);
+      if (in) {
+        float invR2 = mc.K * __builtin_amdgcn_rcpf(r2);
+        if (LJ) {
+          pair_body_mx<ENERGY, true>(pu, cm, false, 0.0f, dx, dy, dz, invR2, a);
+        } else if (UNIFORM) {
+          bool jc = qqi != 0.0f && pu.coulomb && qf[j] != 0.0f;
+          float qiqj = jc ? float(pu.kCoul) * qqi * qf[j] : 0.0f;
+          pair_body_mx<ENERGY, false>(pu, cm, jc, qiqj, dx, dy, dz, invR2, a);
+        } else {
+          const DevPair p = prow[qj.w];
+          bool jc = qqi != 0.0f && p.coulomb && qf[j] != 0.0f;
+          float qiqj = jc ? float(p.kCoul) * qqi * qf[j] : 0.0f;
+          pair_body_mx<ENERGY, false>(p, cm, jc, qiqj, dx, dy, dz, invR2, a);
+        }
+      }
+    };
+    auto flush = [&](const MixedAcc& a) {
+      fx += (double)a.fx; fy += (double)a.fy; fz += (double)a.fz;
+      Wp += (double)a.Wp; Wc += (double)a.Wc;
+      if (ENERGY) { Ep += (double)a.Ep; Ec += (double)a.Ec; }
+    };
+    if (LJ) {
+      // Branch-free groups of four: every entry is evaluated and masked, and
+      // only a group holding an entry in the cutoff band takes the FP64 test.
+      const float c0 = float(pu.c0), c1 = float(pu.c1), c2 = float(pu.c2), es = float(pu.eshift);
+      // lists are padded to a multiple of 4 with the atom itself; its zero
+      // separation marks the padding
+      const int n4 = (n + 3) & ~3;
+      for (int k = 0; k < n4; k += 4) {
+        int j[4];
+        int4 q[4];
+#pragma unroll
+        for (int u = 0; u < 4; u++) j[u] = __builtin_nontemporal_load(col + (size_t)(k + u) * Npad);
+#pragma unroll
+        for (int u = 0; u < 4; u++) q[u] = Q4[j[u]];
+        MixedAcc a = { 0, 0, 0, 0, 0, 0, 0 };
+#pragma unroll
+        for (int u = 0; u < 4; u++) {
+          float dx = (float)(int)((unsigned int)qi.x - (unsigned int)q[u].x);
+          float dy = (float)(int)((unsigned int)qi.y - (unsigned int)q[u].y);
+          float dz = (float)(int)((unsigned int)qi.z - (unsigned int)q[u].z);
+          float r2 = dx * dx + dy * dy + dz * dz;
+          if (r2 == 0.0f) r2 = mc.hi;
+          bool in = r2 < mc.lo;
+          if (r2 >= mc.lo && r2 < mc.hi) in = exact_inside(Rs4, i, j[u], mc.Rc2s);
+          float invR2 = mc.K * __builtin_amdgcn_rcpf(r2);
+          float sr2 = c2 * invR2, sr6 = sr2 * sr2 * sr2;
+          float W = c1 * sr6 * (sr6 + sr6 - 1.0f);
+          if (ENERGY) a.Ep += in ? c0 * sr6 * (sr6 - 1.0f) + es : 0.0f;
+          a.Wp += in ? W : 0.0f;
+          float g = in ? W * invR2 : 0.0f;
+          a.fx += g * dx;
+          a.fy += g * dy;
+          a.fz += g * dz;
+        }
+        flush(a);
+      }
+    } else {
+      int k = 0;
+      for (; k + 4 <= n; k += 4) {
+        int j0 = __builtin_nontemporal_load(col + (size_t)(k + 0) * Npad);
+        int j1 = __builtin_nontemporal_load(col + (size_t)(k + 1) * Npad);
+        int j2 = __builtin_nontemporal_load(col + (size_t)(k + 2) * Npad);
+        int j3 = __builtin_nontemporal_load(col + (size_t)(k + 3) * Npad);
+        int4 q0 = Q4[j0], q1 = Q4[j1], q2 = Q4[j2], q3 = Q4[j3];
+        MixedAcc a = { 0, 0, 0, 0, 0, 0, 0 };
+        one(j0, q0, a); one(j1, q1, a); one(j2, q2, a); one(j3, q3, a);
+        flush(a);
+      }
+      MixedAcc a = { 0, 0, 0, 0, 0, 0, 0 };
+      for (; k < n; k++) {
+        int j = __builtin_nontemporal_load(col + (size_t)k * Npad);
+        one(j, Q4[j], a);
+      }
+      flush(a);
+    }
+    F[i] = mc.unit * fx;
+    F[Npad + i] = mc.unit * fy;
+    F[2 * Npad + i] = mc.unit * fz;
+  }
+  double v[4] = { Ep, Ec, Wp, Wc };
   block_sum<4>(v);
   if (threadIdx.x == 0) {
     partials[4 * blk + 0] = 0.5 * v[0];
@@ -668,7 +894,7 @@ void dev_destroy(Device* d) {
   if (!d) return;
   (void)hipStreamSynchronize(d->stream);
   for (DBuf<double>* b : { &d->R, &d->P, &d->R0, &d->invMass, &d->charge, &d->F, &d->tmp3, &d->tmpF, &d->tmp1, &d->stage, &d->partials, &d->result, &d->d2, &d->trig, &d->shadow }) b->release();
-  d->Rs4.release();
+  d->Rs4.release(); d->Q4.release(); d->qf.release();
   for (DBuf<int>* b : { &d->type, &d->body, &d->perm, &d->freeFlag, &d->chargedFlag, &d->tmpI, &d->key, &d->cellCount, &d->cellStart, &d->order, &d->blockSums, &d->nbr, &d->nnb, &d->flags, &d->exOff, &d->exItem, &d->iperm }) b->release();
   d->pairs.release(); d->couls.release(); d->interact.release();
   dev_ewald_release(d);
@@ -684,6 +910,7 @@ void dev_set_charges(Device* d, const std::vector<double>& q) {
   int N = d->N;
   std::vector<int> chg(N);
   for (int i = 0; i < N; i++) chg[i] = std::fabs(q[i]) > 2.220446049250313e-16;
+  d->anyCharged = std::any_of(chg.begin(), chg.end(), [](int c) { return c != 0; });
   // original order -> sorted order through perm
   DBuf<double> tq; tq.alloc(N);
   DBuf<int> tc; tc.alloc(N);
@@ -714,6 +941,8 @@ void dev_set_interact(Device* d, const std::vector<uint8_t>& table) {
 
 void dev_set_models(Device* d, int layer, const std::vector<DevPair>& pairs, const DevCoul& coul) {
   size_t nt2 = (size_t)d->ntypes * d->ntypes;
+  d->hostPairs.resize(nt2 * d->nlayers);
+  std::copy(pairs.begin(), pairs.begin() + nt2, d->hostPairs.begin() + nt2 * layer);
   HIP_CHECK(hipMemcpyAsync(d->pairs.p + nt2 * layer, pairs.data(), sizeof(DevPair) * nt2, hipMemcpyHostToDevice, d->stream));
   HIP_CHECK(hipMemcpyAsync(d->couls.p + layer, &coul, sizeof(DevCoul), hipMemcpyHostToDevice, d->stream));
   HIP_CHECK(hipStreamSynchronize(d->stream));
@@ -892,22 +1121,56 @@ void dev_compute_pairs(Device* d, const PairCall& c, ForceSums* out) {
   double* F = d->F.p + 3 * (size_t)Np * c.layer;
   if (N == 0) { *out = ForceSums{ 0, 0, 0, 0 }; return; }
   int g = div_up(N, BLOCK);
-  hipLaunchKernelGGL(k_scale4, dim3(g), dim3(BLOCK), 0, d->stream, d->Rs4.p, d->R.p, d->type.p, c.L, N, Np);
+  if (c.mixed) {
+    d->Q4.alloc(Np);
+    d->qf.alloc(Np);
+    hipLaunchKernelGGL(k_scale4q, dim3(g), dim3(BLOCK), 0, d->stream, d->Rs4.p, d->Q4.p, d->qf.p, d->R.p, d->type.p,
+                       d->charge.p, d->chargedFlag.p, c.L, N, Np);
+  } else {
+    hipLaunchKernelGGL(k_scale4, dim3(g), dim3(BLOCK), 0, d->stream, d->Rs4.p, d->R.p, d->type.p, c.L, N, Np);
+  }
   const DevPair* pt = d->pairs.p + (size_t)c.layer * c.ntypes * c.ntypes;
   const DevCoul* cp = d->couls.p + c.layer;
   if (d->profiling) HIP_CHECK(hipEventRecord(d->ev0, d->stream));
-#define LAUNCH_PAIRS(E, MX, U)                                                                          \
-  hipLaunchKernelGGL((k_pairs<E, MX, U>), dim3(g), dim3(BLOCK), 0, d->stream, F, d->partials.p, d->Rs4.p, \
-                     d->nbr.p, d->nnb.p, d->charge.p, d->chargedFlag.p, pt, cp, c.ntypes, c.L, c.invL2, c.Rc2, N, Np)
   bool U = c.ntypes == 1;
-  if (c.energy) {
-    if (c.mixed) { if (U) LAUNCH_PAIRS(true, true, true); else LAUNCH_PAIRS(true, true, false); }
-    else { if (U) LAUNCH_PAIRS(true, false, true); else LAUNCH_PAIRS(true, false, false); }
+  const DevPair& hp = d->hostPairs[(size_t)c.layer * c.ntypes * c.ntypes];
+  bool LJ = U && !d->anyCharged && hp.kind == PAIR_LJ_CUT && (hp.modifier == MOD_NONE || hp.modifier == MOD_SHIFTED);
+  int variant = (c.energy ? 4 : 0) + (U ? 2 : 0) + (LJ ? 1 : 0);
+  if (c.mixed) {
+    // cutoff in fixed-point units: Rc2 (scaled) * 2^64
+    MixedCut mc;
+    double two32 = 4294967296.0, rc2u = c.Rc2 * two32 * two32;
+    mc.K = (float)(two32 * two32 / (c.L * c.L));
+    mc.lo = (float)(rc2u * (1.0 - 2e-6));
+    mc.hi = (float)(rc2u * (1.0 + 2e-6));
+    mc.Rc2s = c.Rc2;
+    mc.unit = c.L / two32;
+#define LAUNCH_MX(E, U, J)                                                                                       \
+  hipLaunchKernelGGL((k_pairs_mx<E, U, J>), dim3(g), dim3(BLOCK), 0, d->stream, F, d->partials.p, d->Q4.p, d->Rs4.p, \
+                     d->nbr.p, d->nnb.p, d->qf.p, pt, cp, c.ntypes, mc, N, Np)
+    switch (variant) {
+      case 7: LAUNCH_MX(true, true, true); break;
+      case 6: LAUNCH_MX(true, true, false); break;
+      case 4: LAUNCH_MX(true, false, false); break;
+      case 3: LAUNCH_MX(false, true, true); break;
+      case 2: LAUNCH_MX(false, true, false); break;
+      default: LAUNCH_MX(false, false, false); break;
+    }
+#undef LAUNCH_MX
   } else {
-    if (c.mixed) { if (U) LAUNCH_PAIRS(false, true, true); else LAUNCH_PAIRS(false, true, false); }
-    else { if (U) LAUNCH_PAIRS(false, false, true); else LAUNCH_PAIRS(false, false, false); }
-  }
+#define LAUNCH_PAIRS(E, U, J)                                                                                     \
+  hipLaunchKernelGGL((k_pairs<E, false, U, J>), dim3(g), dim3(BLOCK), 0, d->stream, F, d->partials.p, d->Rs4.p,   \
+                     d->nbr.p, d->nnb.p, d->charge.p, d->chargedFlag.p, pt, cp, c.ntypes, c.L, c.invL2, c.Rc2, N, Np)
+    switch (variant) {
+      case 7: LAUNCH_PAIRS(true, true, true); break;
+      case 6: LAUNCH_PAIRS(true, true, false); break;
+      case 4: LAUNCH_PAIRS(true, false, false); break;
+      case 3: LAUNCH_PAIRS(false, true, true); break;
+      case 2: LAUNCH_PAIRS(false, true, false); break;
+      default: LAUNCH_PAIRS(false, false, false); break;
+    }
 #undef LAUNCH_PAIRS
+  }
   launch_check("pairs");
   if (d->profiling) HIP_CHECK(hipEventRecord(d->ev1, d->stream));
   hipLaunchKernelGGL(k_reduce_partials<4>, dim3(1), dim3(BLOCK), 0, d->stream, d->result.p, d->partials.p, g);

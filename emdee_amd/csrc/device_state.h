@@ -111,6 +111,8 @@ struct Device {
   // per-atom SoA arrays (sorted order)
   DBuf<double> R, P, R0, invMass, charge, F;   // F holds nlayers * 3 * Npad
   DBuf<double4> Rs4;      // {x/L, y/L, z/L, type}: one 32-byte gather per neighbor
+  DBuf<int4> Q4;          // mixed form: 32-bit fixed-point scaled coordinates and type
+  DBuf<float> qf;         // mixed form: FP32 charge (0 when uncharged)
   DBuf<int> type, body, perm, freeFlag, chargedFlag;
   DBuf<double> tmp3, tmpF, tmp1;   // permutation scratch (swapped with the arrays)
   DBuf<int> tmpI;
@@ -127,6 +129,8 @@ struct Device {
   bool built = false;
   // models
   DBuf<DevPair> pairs;    // nlayers * nt * nt
+  std::vector<DevPair> hostPairs;   // host copy of the pair table
+  bool anyCharged = false;
   DBuf<DevCoul> couls;    // nlayers
   DBuf<uint8_t> interact;
   DBuf<int> exOff, exItem;
