@@ -112,6 +112,15 @@ GPU_SIGNATURES = {
     "EmDeeGPU_kernel_stats": (None, [tEmDee, _dp]),
     "EmDeeGPU_synchronize": (None, [tEmDee]),
     "EmDeeGPU_destroy": (None, [POINTER(tEmDee)]),
+    "EmDeeHMC_create": (c_void_p, [c_int, _ip, _dp, _dp, _dp, c_double, c_double, c_int, ctypes.c_ulonglong, c_int]),
+    "EmDeeHMC_tune": (c_int, [c_void_p]),
+    "EmDeeHMC_variant": (None, [c_void_p, _ip, _ip, _dp]),
+    "EmDeeHMC_run": (None, [c_void_p, c_int, c_int]),
+    "EmDeeHMC_synchronize": (None, [c_void_p]),
+    "EmDeeHMC_get": (None, [c_void_p, _dp, _dp, POINTER(ctypes.c_longlong), _dp]),
+    "EmDeeHMC_destroy": (None, [c_void_p]),
+    "EmDeeHMC_momenta": (None, [ctypes.c_ulonglong, c_int, c_int, c_int, c_double, _dp]),
+    "EmDeeHMC_uniform": (c_double, [ctypes.c_ulonglong, c_int, c_int]),
 }
 
 

@@ -35,6 +35,35 @@ int EmDeeGPU_neighbor_pairs( tEmDee md, int max, int* pairs );
 void EmDeeGPU_profile( tEmDee md, int on );
 void EmDeeGPU_kernel_stats( tEmDee md, double* stats );
 
+/* Batched hybrid Monte Carlo of independent Lennard-Jones replicas
+   (pair_lj_cut, epsilon = sigma = mass = 1, cutoff rc) in one persistent
+   launch per graph replay.  Each cycle of a replica draws momenta at its kT,
+   runs nsteps velocity-Verlet steps of size dt (boost dt/2, displace dt,
+   boost dt/2, as EmDee_boost/EmDee_displace) and accepts the trajectory with
+   probability min(1, exp(-dH/kT)), restoring the coordinates otherwise.
+   R holds 3 * sum(natoms) coordinates, replica after replica.  precision:
+   0 = FP64 pair arithmetic, 1 = FP32 pair arithmetic with FP64 accumulation.
+   No counterpart in the reference: a reference host loops over one tEmDee
+   per replica. */
+typedef struct EmDeeHMC EmDeeHMC;
+EmDeeHMC* EmDeeHMC_create( int nrep, const int* natoms, const double* box, const double* kT, const double* R,
+                           double rc, double dt, int nsteps, unsigned long long seed, int precision );
+/* Times one cycle of every launch shape (workgroup size x cell division)
+   from the same state, restores the state and keeps the fastest; returns
+   its index.  EmDeeHMC_variant reports the choice and the six timings (ms,
+   -1 for shapes that do not fit). */
+int  EmDeeHMC_tune( EmDeeHMC* h );
+void EmDeeHMC_variant( EmDeeHMC* h, int* blockSize, int* cellDivision, double* tuneMs );
+/* Replays `launches` times a captured launch of `cycles` cycles per replica. */
+void EmDeeHMC_run( EmDeeHMC* h, int launches, int cycles );
+void EmDeeHMC_synchronize( EmDeeHMC* h );
+void EmDeeHMC_get( EmDeeHMC* h, double* R, double* U, long long* accepted, double* lastDH );
+void EmDeeHMC_destroy( EmDeeHMC* h );
+/* The run's random numbers: momenta (3n, unit mass) and Metropolis uniform
+   of a (replica, cycle). */
+void   EmDeeHMC_momenta( unsigned long long seed, int replica, int cycle, int n, double kT, double* P );
+double EmDeeHMC_uniform( unsigned long long seed, int replica, int cycle );
+
 /* Waits for all device work of the system. */
 void EmDeeGPU_synchronize( tEmDee md );
 

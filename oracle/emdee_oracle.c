@@ -476,6 +476,39 @@ void oracle_jacobi(double u, double m, double *jac) { jacobi(u, m, jac); }
 void oracle_diagonalization(const double *matrix, double *q, double *w) { diagonalization(matrix, q, w); }
 double oracle_inverse_x_plus_ln_x(double y) { return inverse_of_x_plus_ln_x(y); }
 double oracle_phi(double x) { return phi_fn(x); }
+/* Counter-based generator of the batched hybrid Monte Carlo driver (no
+   reference counterpart; used by the tests that replay batched HMC runs on
+   the reference API).  splitmix64's output function (Steele, Lea & Flood,
+   "Fast splittable pseudorandom number generators", OOPSLA 2014) keyed by
+   (seed, stream, counter); Gaussians by the Box-Muller transform. */
+static uint64_t sm64_mix(uint64_t z) {
+  z = (z ^ (z >> 30)) * 0xbf58476d1ce4e5b9ULL;
+  z = (z ^ (z >> 27)) * 0x94d049bb133111ebULL;
+  return z ^ (z >> 31);
+}
+/* the classic sequential generator: state += golden gamma, output mix */
+uint64_t oracle_splitmix64_next(uint64_t *state) {
+  *state += 0x9E3779B97F4A7C15ULL;
+  return sm64_mix(*state);
+}
+double oracle_hmc_uniform(uint64_t seed, uint64_t stream, uint64_t counter) {
+  uint64_t x = sm64_mix(seed ^ sm64_mix(stream * 0x9E3779B97F4A7C15ULL + counter));
+  return ((double)(x >> 11) + 0.5) / 9007199254740992.0;
+}
+static uint64_t hmc_stream_of(int replica, int cycle) { return ((uint64_t)(uint32_t)replica << 32) | (uint32_t)cycle; }
+void oracle_hmc_momenta(uint64_t seed, int replica, int cycle, int n, double kT, double *P) {
+  uint64_t st = hmc_stream_of(replica, cycle);
+  double sq = sqrt(kT);
+  for (int i = 0; i < n; i++)
+    for (int x = 0; x < 3; x++) {
+      double u1 = oracle_hmc_uniform(seed, st, 6ULL * i + 2 * x), u2 = oracle_hmc_uniform(seed, st, 6ULL * i + 2 * x + 1);
+      P[3 * i + x] = sq * (sqrt(-2.0 * log(u1)) * cos(6.283185307179586 * u2));
+    }
+}
+double oracle_hmc_accept_uniform(uint64_t seed, int replica, int cycle) {
+  return oracle_hmc_uniform(seed, hmc_stream_of(replica, cycle), 1ULL << 40);
+}
+
 /* First n normals of a freshly seeded generator. */
 void oracle_normals(int seed, int n, double *out) {
   okiss_t g; rng_setup(&g, seed);
