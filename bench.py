@@ -104,6 +104,52 @@ def cpu_baseline(R, L, steps, dt, threads):
     return R.shape[0] * steps / t, t
 
 
+def hmc_bench(lib, nrep, cycles, warm, seed=1):
+    """Batched HMC workload (512 LJ replicas of 256-8192 atoms, densities
+    0.7-0.9, kT 0.8-2.0, rc 2.5, 10-step trajectories of dt 0.005, FP32 pair
+    arithmetic): graph replays of one cycle each, potential-energy series per
+    replica for the effective sample count."""
+    from emdee_amd.hmc import BatchedHMC, integrated_autocorrelation, replica_set
+    N, L, T, Rs = replica_set(nrep, seed=seed)
+    h = BatchedHMC(lib, N, L, T, Rs, dt=0.005, nsteps=10, seed=seed + 6, precision=1)
+    tune = h.tune()
+    h.run(launches=warm, cycles=1)
+    h.synchronize()
+    _, acc0 = h.energies()
+    series = []
+    t0 = time.perf_counter()
+    for _ in range(cycles):
+        h.run(launches=1, cycles=1)
+        U, acc = h.energies()
+        series.append(U)
+    t = time.perf_counter() - t0
+    h.close()
+    series = np.array(series)
+    taus = np.array([integrated_autocorrelation(series[:, r]) for r in range(nrep)])
+    ess = float(np.sum(cycles / taus))
+    accepted = int(np.sum(acc - acc0))
+    atom_steps = float(np.sum(N)) * 10 * cycles
+    return dict(replicas=nrep, atoms=int(np.sum(N)), cycles=cycles, warmup_cycles=warm, seconds=t,
+                atom_steps_per_s=atom_steps / t, accepted_per_s=accepted / t,
+                acceptance=accepted / (cycles * nrep), effective_samples_per_s=ess / t,
+                median_tau_cycles=float(np.median(taus)), launch_shape=tune), (N, L, T, Rs)
+
+
+def hmc_cpu_baseline(replicas, sample, threads, seed=1):
+    """Reference-API replay (one tEmDee per replica) on the oracle, for the
+    first `sample` replicas and one cycle."""
+    from emdee_amd.emdee import EmDeeLibrary
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import hmc_ref
+    lib = EmDeeLibrary(os.path.join(ROOT, "oracle", "liboracle_emdee_fast.so"))
+    N, L, T, Rs = replicas
+    idx = list(range(sample))
+    t0 = time.perf_counter()
+    hmc_ref.replay(lib, lib, N[idx], L[idx], T[idx], [Rs[i] for i in idx], 1, seed=seed + 6, threads=threads)
+    t = time.perf_counter() - t0
+    return float(np.sum(N[idx])) * 10 / t, t
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -112,6 +158,9 @@ def main():
     ap.add_argument("--natoms", type=int, default=1048576)
     ap.add_argument("--cpu-steps", type=int, default=20)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--hmc-replicas", type=int, default=512)
+    ap.add_argument("--hmc-cycles", type=int, default=60)
+    ap.add_argument("--no-hmc", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build measures one GPU; --gpus {args.gpus} is not supported "
@@ -150,6 +199,18 @@ def main():
                "build": "oracle/liboracle_emdee_fast.so (-Ofast)",
                "sample": f"{N}-atom LJ, {args.cpu_steps} steps after 1 warm-up step ({ct:.1f} s), same set-up"}
 
+    hmc = None
+    if not args.no_hmc:
+        print(f"# batched HMC: {args.hmc_replicas} replicas, {args.hmc_cycles} cycles", file=sys.stderr, flush=True)
+        hmc, reps = hmc_bench(lib, args.hmc_replicas, args.hmc_cycles, warm=20)
+        print(f"# hmc: {hmc}", file=sys.stderr, flush=True)
+        if not args.no_cpu:
+            threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+            hv, ht = hmc_cpu_baseline(reps, len(reps[0]), threads)
+            hmc["cpu_baseline"] = {"value": hv, "unit": "atom-steps/s", "cores": threads, "kind": "port",
+                                   "sample": f"reference-API replay of all {len(reps[0])} replicas, one 10-step "
+                                             f"cycle each ({ht:.1f} s)"}
+
     line = {
         "metric": METRIC, "value": value, "unit": "atom-steps/s", "n_gpus": 1,
         "steps": args.steps, "warmup": args.warmup,
@@ -164,6 +225,7 @@ def main():
                      "frac": achieved / HBM_PEAK_GBS, "traffic": None,
                      "kernel": "k_pairs", "kernel_ms": best["pair_ms"], "bytes_per_launch": bytes_per_launch},
         "cpu_baseline": cpu,
+        "hmc": hmc,
     }
     print(json.dumps(line), flush=True)
 

@@ -441,7 +441,17 @@ __device__ __forceinline__ void pair_body(const DevPair& p, const DevCoul& cm, b
 }
 
 // UNIFORM: a single atom type, so the model record is loaded once.
-template <bool ENERGY, bool MIXED, bool UNIFORM, bool LJ>
+// TPA threads per atom (consecutive lanes) split an atom's list for small
+// systems that would not fill the chip with one thread per atom; their
+// partial forces are combined by a fixed butterfly (deterministic).
+template <int TPA>
+__device__ __forceinline__ double group_sum(double v) {
+#pragma unroll
+  for (int off = TPA / 2; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+  return v;
+}
+
+template <bool ENERGY, bool MIXED, bool UNIFORM, bool LJ, int TPA>
 __global__ void __launch_bounds__(BLOCK)
 k_pairs(double* __restrict__ F, double* __restrict__ partials,
         const double4* __restrict__ Rs4, const int* __restrict__ nbr, const int* __restrict__ nnb,
@@ -450,7 +460,8 @@ k_pairs(double* __restrict__ F, double* __restrict__ partials,
         int ntypes, double L, double invL2, double Rc2, int N, int Npad) {
   using T = typename std::conditional<MIXED, float, double>::type;
   int blk = xcd_block();
-  int i = blk * BLOCK + threadIdx.x;
+  int gt = blk * BLOCK + threadIdx.x;
+  int i = TPA == 1 ? gt : gt / TPA, sub = TPA == 1 ? 0 : gt % TPA;
   PairAcc a = { 0, 0, 0, 0, 0, 0, 0 };
   if (i < N) {
     const DevCoul cm = *coulp;
@@ -486,7 +497,7 @@ k_pairs(double* __restrict__ F, double* __restrict__ partials,
         }
       }
     };
-    {
+    if (TPA == 1) {
       int k = 0;
       for (; k + 4 <= n; k += 4) {
         int j0 = __builtin_nontemporal_load(col + (size_t)(k + 0) * Npad);
@@ -500,7 +511,22 @@ k_pairs(double* __restrict__ F, double* __restrict__ partials,
         int j = __builtin_nontemporal_load(col + (size_t)k * Npad);
         one(j, Rs4[j]);
       }
+    } else {
+      int k = sub;
+      for (; k + TPA < n; k += 2 * TPA) {
+        int j0 = col[(size_t)k * Npad], j1 = col[(size_t)(k + TPA) * Npad];
+        double4 r0 = Rs4[j0], r1 = Rs4[j1];
+        one(j0, r0); one(j1, r1);
+      }
+      if (k < n) { int j = col[(size_t)k * Npad]; one(j, Rs4[j]); }
     }
+  }
+  if (TPA > 1) {
+    a.fx = group_sum<TPA>(a.fx);
+    a.fy = group_sum<TPA>(a.fy);
+    a.fz = group_sum<TPA>(a.fz);
+  }
+  if (i < N && sub == 0) {
     F[i] = L * a.fx;
     F[Npad + i] = L * a.fy;
     F[2 * Npad + i] = L * a.fz;
@@ -604,14 +630,15 @@ __device__ __forceinline__ void pair_body_mx(const DevPair& p, const DevCoul& cm
   a.fz += g * dz;
 }
 
-template <bool ENERGY, bool UNIFORM, bool LJ>
+template <bool ENERGY, bool UNIFORM, bool LJ, int TPA>
 __global__ void __launch_bounds__(BLOCK)
 k_pairs_mx(double* __restrict__ F, double* __restrict__ partials, const int4* __restrict__ Q4,
            const double4* __restrict__ Rs4, const int* __restrict__ nbr, const int* __restrict__ nnb,
            const float* __restrict__ qf, const DevPair* __restrict__ pairs, const DevCoul* __restrict__ coulp,
            int ntypes, MixedCut mc, int N, int Npad) {
   int blk = xcd_block();
-  int i = blk * BLOCK + threadIdx.x;
+  int gt = blk * BLOCK + threadIdx.x;
+  int i = TPA == 1 ? gt : gt / TPA, sub = TPA == 1 ? 0 : gt % TPA;
   double fx = 0, fy = 0, fz = 0, Ep = 0, Ec = 0, Wp = 0, Wc = 0;
   if (i < N) {
     const DevCoul cm = *coulp;
@@ -686,7 +713,7 @@ k_pairs_mx(double* __restrict__ F, double* __restrict__ partials, const int4* __
         }
         flush(a);
       }
-    } else {
+    } else if (TPA == 1) {
       int k = 0;
       for (; k + 4 <= n; k += 4) {
         int j0 = __builtin_nontemporal_load(col + (size_t)(k + 0) * Npad);
@@ -704,7 +731,30 @@ k_pairs_mx(double* __restrict__ F, double* __restrict__ partials, const int4* __
         one(j, Q4[j], a);
       }
       flush(a);
+    } else {
+      int k = sub;
+      for (; k + 3 * TPA < n; k += 4 * TPA) {
+        int j0 = col[(size_t)k * Npad], j1 = col[(size_t)(k + TPA) * Npad];
+        int j2 = col[(size_t)(k + 2 * TPA) * Npad], j3 = col[(size_t)(k + 3 * TPA) * Npad];
+        int4 q0 = Q4[j0], q1 = Q4[j1], q2 = Q4[j2], q3 = Q4[j3];
+        MixedAcc a = { 0, 0, 0, 0, 0, 0, 0 };
+        one(j0, q0, a); one(j1, q1, a); one(j2, q2, a); one(j3, q3, a);
+        flush(a);
+      }
+      MixedAcc a = { 0, 0, 0, 0, 0, 0, 0 };
+      for (; k < n; k += TPA) {
+        int j = col[(size_t)k * Npad];
+        one(j, Q4[j], a);
+      }
+      flush(a);
     }
+  }
+  if (TPA > 1) {
+    fx = group_sum<TPA>(fx);
+    fy = group_sum<TPA>(fy);
+    fz = group_sum<TPA>(fz);
+  }
+  if (i < N && sub == 0) {
     F[i] = mc.unit * fx;
     F[Npad + i] = mc.unit * fy;
     F[2 * Npad + i] = mc.unit * fz;
@@ -851,7 +901,7 @@ Device* dev_create(const DeviceSetup& s) {
   HIP_CHECK(hipHostMalloc(&d->hstage, 3 * sizeof(double) * std::max(s.N, 1), hipHostMallocDefault));
   HIP_CHECK(hipHostMalloc(&d->hresult, 64 * sizeof(double), hipHostMallocDefault));
   d->result.alloc(64);
-  d->partials.alloc(8 * (size_t)(Np / BLOCK + 1));
+  d->partials.alloc(64 * (size_t)(Np / BLOCK + 1));   // up to 16 threads per atom x 4 sums
   d->d2.alloc(Np);
   d->trig.alloc(4);
   HIP_CHECK(hipMemsetAsync(d->Rs4.p, 0, sizeof(double4) * Np, d->stream));
@@ -1135,6 +1185,13 @@ void dev_compute_pairs(Device* d, const PairCall& c, ForceSums* out) {
   bool U = c.ntypes == 1;
   const DevPair& hp = d->hostPairs[(size_t)c.layer * c.ntypes * c.ntypes];
   bool LJ = U && !d->anyCharged && hp.kind == PAIR_LJ_CUT && (hp.modifier == MOD_NONE || hp.modifier == MOD_SHIFTED);
+  // threads per atom: enough wavefronts to fill the chip (about 2^18 lanes)
+  int tpa = 1;
+  if (!LJ) {
+    long want = (1L << 18) / std::max(N, 1);
+    tpa = want >= 16 ? 16 : (want >= 4 ? 4 : 1);
+  }
+  int gp = div_up((long)N * tpa, BLOCK);
   int variant = (c.energy ? 4 : 0) + (U ? 2 : 0) + (LJ ? 1 : 0);
   if (c.mixed) {
     // cutoff in fixed-point units: Rc2 (scaled) * 2^64
@@ -1145,35 +1202,47 @@ void dev_compute_pairs(Device* d, const PairCall& c, ForceSums* out) {
     mc.hi = (float)(rc2u * (1.0 + 2e-6));
     mc.Rc2s = c.Rc2;
     mc.unit = c.L / two32;
-#define LAUNCH_MX(E, U, J)                                                                                       \
-  hipLaunchKernelGGL((k_pairs_mx<E, U, J>), dim3(g), dim3(BLOCK), 0, d->stream, F, d->partials.p, d->Q4.p, d->Rs4.p, \
-                     d->nbr.p, d->nnb.p, d->qf.p, pt, cp, c.ntypes, mc, N, Np)
+#define LAUNCH_MX(E, U, J, T)                                                                                      \
+  hipLaunchKernelGGL((k_pairs_mx<E, U, J, T>), dim3(gp), dim3(BLOCK), 0, d->stream, F, d->partials.p, d->Q4.p,     \
+                     d->Rs4.p, d->nbr.p, d->nnb.p, d->qf.p, pt, cp, c.ntypes, mc, N, Np)
+#define GENERIC_MX(E, U)                                                                                           \
+  do {                                                                                                            \
+    if (tpa == 16) LAUNCH_MX(E, U, false, 16); else if (tpa == 4) LAUNCH_MX(E, U, false, 4);                      \
+    else LAUNCH_MX(E, U, false, 1);                                                                               \
+  } while (0)
     switch (variant) {
-      case 7: LAUNCH_MX(true, true, true); break;
-      case 6: LAUNCH_MX(true, true, false); break;
-      case 4: LAUNCH_MX(true, false, false); break;
-      case 3: LAUNCH_MX(false, true, true); break;
-      case 2: LAUNCH_MX(false, true, false); break;
-      default: LAUNCH_MX(false, false, false); break;
+      case 7: LAUNCH_MX(true, true, true, 1); break;
+      case 6: GENERIC_MX(true, true); break;
+      case 4: GENERIC_MX(true, false); break;
+      case 3: LAUNCH_MX(false, true, true, 1); break;
+      case 2: GENERIC_MX(false, true); break;
+      default: GENERIC_MX(false, false); break;
     }
+#undef GENERIC_MX
 #undef LAUNCH_MX
   } else {
-#define LAUNCH_PAIRS(E, U, J)                                                                                     \
-  hipLaunchKernelGGL((k_pairs<E, false, U, J>), dim3(g), dim3(BLOCK), 0, d->stream, F, d->partials.p, d->Rs4.p,   \
+#define LAUNCH_PAIRS(E, U, J, T)                                                                                    \
+  hipLaunchKernelGGL((k_pairs<E, false, U, J, T>), dim3(gp), dim3(BLOCK), 0, d->stream, F, d->partials.p, d->Rs4.p, \
                      d->nbr.p, d->nnb.p, d->charge.p, d->chargedFlag.p, pt, cp, c.ntypes, c.L, c.invL2, c.Rc2, N, Np)
+#define GENERIC(E, U)                                                                                              \
+  do {                                                                                                            \
+    if (tpa == 16) LAUNCH_PAIRS(E, U, false, 16); else if (tpa == 4) LAUNCH_PAIRS(E, U, false, 4);                \
+    else LAUNCH_PAIRS(E, U, false, 1);                                                                            \
+  } while (0)
     switch (variant) {
-      case 7: LAUNCH_PAIRS(true, true, true); break;
-      case 6: LAUNCH_PAIRS(true, true, false); break;
-      case 4: LAUNCH_PAIRS(true, false, false); break;
-      case 3: LAUNCH_PAIRS(false, true, true); break;
-      case 2: LAUNCH_PAIRS(false, true, false); break;
-      default: LAUNCH_PAIRS(false, false, false); break;
+      case 7: LAUNCH_PAIRS(true, true, true, 1); break;
+      case 6: GENERIC(true, true); break;
+      case 4: GENERIC(true, false); break;
+      case 3: LAUNCH_PAIRS(false, true, true, 1); break;
+      case 2: GENERIC(false, true); break;
+      default: GENERIC(false, false); break;
     }
+#undef GENERIC
 #undef LAUNCH_PAIRS
   }
   launch_check("pairs");
   if (d->profiling) HIP_CHECK(hipEventRecord(d->ev1, d->stream));
-  hipLaunchKernelGGL(k_reduce_partials<4>, dim3(1), dim3(BLOCK), 0, d->stream, d->result.p, d->partials.p, g);
+  hipLaunchKernelGGL(k_reduce_partials<4>, dim3(1), dim3(BLOCK), 0, d->stream, d->result.p, d->partials.p, gp);
   launch_check("reduce");
   HIP_CHECK(hipMemcpyAsync(d->hresult, d->result.p, 4 * sizeof(double), hipMemcpyDeviceToHost, d->stream));
   HIP_CHECK(hipStreamSynchronize(d->stream));

@@ -72,7 +72,34 @@ class BatchedHMC:
                               acc.ctypes.data_as(ctypes.POINTER(ctypes.c_longlong)), dH.ctypes.data_as(_dp))
         return [R[a:b] for a, b in zip(self.offsets[:-1], self.offsets[1:])], U, acc, dH
 
+    def energies(self):
+        """Potential energies and accepted counts only (no coordinate copy)."""
+        U = np.zeros(len(self.N))
+        acc = np.zeros(len(self.N), dtype=np.int64)
+        self.lib.EmDeeHMC_get(self.h, None, U.ctypes.data_as(_dp),
+                              acc.ctypes.data_as(ctypes.POINTER(ctypes.c_longlong)), None)
+        return U, acc
+
     def close(self):
         if self.h:
             self.lib.EmDeeHMC_destroy(self.h)
             self.h = None
+
+
+def integrated_autocorrelation(x):
+    """Integrated autocorrelation time of a series by Geyer's initial
+    positive sequence estimator (Geyer, Stat. Sci. 7, 1992)."""
+    x = np.asarray(x, dtype=np.float64)
+    n = len(x)
+    x = x - x.mean()
+    var = np.dot(x, x) / n
+    if var == 0.0 or n < 4:
+        return 1.0
+    acf = np.array([np.dot(x[: n - k], x[k:]) / (n * var) for k in range(n // 2)])
+    tau = -1.0
+    for m in range(0, len(acf) - 1, 2):
+        g = acf[m] + acf[m + 1]
+        if g <= 0.0:
+            break
+        tau += 2.0 * g
+    return max(tau, 1.0)

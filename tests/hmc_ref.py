@@ -29,13 +29,13 @@ def accept_uniform(oracle, seed, replica, cycle):
                [ctypes.c_uint64, ctypes.c_int, ctypes.c_int])(seed, replica, cycle)
 
 
-def replay(lib, oracle, N, L, T, Rs, cycles, rc=2.5, dt=0.005, nsteps=10, seed=2024, skin=0.3):
+def replay(lib, oracle, N, L, T, Rs, cycles, rc=2.5, dt=0.005, nsteps=10, seed=2024, skin=0.3, threads=1):
     """Runs `cycles` HMC cycles of every replica on `lib` (any library with
     the EmDee ABI); returns final coordinates, potential energies, accepted
     counts and the energy changes of every cycle ([replica][cycle])."""
     outR, outU, outA, outD = [], [], [], []
     for r, (n, box, kT, R0) in enumerate(zip(N, L, T, Rs)):
-        md = lib.system(1, 1, rc, skin, int(n), None, None, None)
+        md = lib.system(threads, 1, rc, skin, int(n), None, None, None)
         lib.EmDee_set_pair_model(md, 1, 1, lib.EmDee_pair_lj_cut(1.0, 1.0), 0.0)
         lib.upload(md, "box", np.array([box]))
         lib.upload(md, "coordinates", R0)
