@@ -19,6 +19,7 @@
 #include "engine.h"
 #include "kspace.h"
 #include "models.h"
+#include "surrogate.h"
 
 using namespace emdee;
 
@@ -64,6 +65,9 @@ struct System {
   int nbonds = 0, nangles = 0;
   std::vector<DevStruct> bonds, angles;     // structs.f90 lists, 0-based atoms
   EwaldHost ewald;
+  EmDeeSurrogate* surrogate = nullptr;       // learned model added to one layer
+  int surrogateLayer = 0;
+  bool atomEnergies = false;                 // next force evaluation writes atom energies
   int precision = 0;
   Device* dev = nullptr;
   std::vector<double> mirror;                // host buffer for EmDee_memory_address
@@ -214,7 +218,8 @@ void compute_forces(tEmDee* md) {
   }
   md->Time.Neighbor += wtime();
   md->Time.Pair -= wtime();
-  bool compute = md->Options.Compute;
+  dev_set_atom_energies(me->dev, me->atomEnergies);
+  bool compute = md->Options.Compute || me->atomEnergies;
   int l = me->layer;
   PairCall c;
   c.layer = l - 1;
@@ -244,6 +249,12 @@ void compute_forces(tEmDee* md) {
   if (kspace) {
     E[LONG_] = dev_ewald_compute(me->dev, l - 1, me->L) + me->ewald.rigidEnergy[l - 1];
     W[LONG_] = E[COUL_] + E[LONG_] - W[COUL_];
+  }
+  // learned model standing in for a costly term of this layer (no reference counterpart)
+  if (me->surrogate && me->surrogateLayer == l) {
+    double Ws = 0, Es = dev_surrogate_apply(me->dev, me->surrogate, l - 1, me->L, &Ws);
+    E[COUL_] += Es;
+    W[COUL_] += Ws;
   }
   md->Virial = W[0] + W[1] + W[2] + W[3] + W[4];
   if (me->nbodies != 0) {
@@ -889,6 +900,27 @@ int EmDeeGPU_neighbor_pairs(tEmDee md, int max, int* pairs) { return dev_neighbo
 void EmDeeGPU_profile(tEmDee md, int on) { dev_set_profiling(S(md)->dev, on != 0); }
 void EmDeeGPU_kernel_stats(tEmDee md, double* stats) { dev_kernel_stats(S(md)->dev, stats, 7); }
 void EmDeeGPU_synchronize(tEmDee md) { dev_sync(S(md)->dev); }
+void EmDeeGPU_atom_energies(tEmDee* md, double* e) {
+  System* me = S(*md);
+  if (!me->initialized) fatal("atom energies", "box and coordinates have not been defined");
+  me->atomEnergies = true;
+  compute_forces(md);
+  me->atomEnergies = false;
+  dev_get_atom_energies(me->dev, e);
+  dev_set_atom_energies(me->dev, false);
+}
+void EmDeeGPU_attach_surrogate(tEmDee md, EmDeeSurrogate* s, int layer) {
+  System* me = S(md);
+  if (s && (layer < 1 || layer > me->nlayers)) fatal("surrogate attachment", "selected layer is out of range");
+  me->surrogate = s;
+  me->surrogateLayer = s ? layer : 0;
+  for (int l = 1; l <= me->nlayers; l++) me->forcesUpToDate[l] = 0;
+}
+void EmDeeGPU_surrogate_sample(tEmDee md, EmDeeSurrogate* s, const double* labels) {
+  System* me = S(md);
+  if (!me->initialized) fatal("surrogate sampling", "box and coordinates have not been defined");
+  dev_surrogate_add_sample(me->dev, s, me->L, labels);
+}
 void EmDeeGPU_destroy(tEmDee* md) {
   if (!md || !md->Data) return;
   System* me = S(*md);

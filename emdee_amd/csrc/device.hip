@@ -457,7 +457,7 @@ k_pairs(double* __restrict__ F, double* __restrict__ partials,
         const double4* __restrict__ Rs4, const int* __restrict__ nbr, const int* __restrict__ nnb,
         const double* __restrict__ charge, const int* __restrict__ charged,
         const DevPair* __restrict__ pairs, const DevCoul* __restrict__ coulp,
-        int ntypes, double L, double invL2, double Rc2, int N, int Npad) {
+        int ntypes, double L, double invL2, double Rc2, int N, int Npad, double* __restrict__ eatom) {
   using T = typename std::conditional<MIXED, float, double>::type;
   int blk = xcd_block();
   int gt = blk * BLOCK + threadIdx.x;
@@ -530,6 +530,11 @@ k_pairs(double* __restrict__ F, double* __restrict__ partials,
     F[i] = L * a.fx;
     F[Npad + i] = L * a.fy;
     F[2 * Npad + i] = L * a.fz;
+  }
+  if (ENERGY && eatom) {   // atom energies: half of every pair term of the atom
+    double e = a.Ep + a.Ec;
+    if (TPA > 1) e = group_sum<TPA>(e);
+    if (i < N && sub == 0) eatom[i] = 0.5 * e;
   }
   double v[4] = { a.Ep, a.Ec, a.Wp, a.Wc };
   block_sum<4>(v);
@@ -635,7 +640,7 @@ __global__ void __launch_bounds__(BLOCK)
 k_pairs_mx(double* __restrict__ F, double* __restrict__ partials, const int4* __restrict__ Q4,
            const double4* __restrict__ Rs4, const int* __restrict__ nbr, const int* __restrict__ nnb,
            const float* __restrict__ qf, const DevPair* __restrict__ pairs, const DevCoul* __restrict__ coulp,
-           int ntypes, MixedCut mc, int N, int Npad) {
+           int ntypes, MixedCut mc, int N, int Npad, double* __restrict__ eatom) {
   int blk = xcd_block();
   int gt = blk * BLOCK + threadIdx.x;
   int i = TPA == 1 ? gt : gt / TPA, sub = TPA == 1 ? 0 : gt % TPA;
@@ -758,6 +763,11 @@ k_pairs_mx(double* __restrict__ F, double* __restrict__ partials, const int4* __
     F[i] = mc.unit * fx;
     F[Npad + i] = mc.unit * fy;
     F[2 * Npad + i] = mc.unit * fz;
+  }
+  if (ENERGY && eatom) {   // atom energies: half of every pair term of the atom
+    double e = Ep + Ec;
+    if (TPA > 1) e = group_sum<TPA>(e);
+    if (i < N && sub == 0) eatom[i] = 0.5 * e;
   }
   double v[4] = { Ep, Ec, Wp, Wc };
   block_sum<4>(v);
@@ -944,7 +954,7 @@ void dev_destroy(Device* d) {
   if (!d) return;
   (void)hipStreamSynchronize(d->stream);
   for (DBuf<double>* b : { &d->R, &d->P, &d->R0, &d->invMass, &d->charge, &d->F, &d->tmp3, &d->tmpF, &d->tmp1, &d->stage, &d->partials, &d->result, &d->d2, &d->trig, &d->shadow }) b->release();
-  d->Rs4.release(); d->Q4.release(); d->qf.release();
+  d->Rs4.release(); d->Q4.release(); d->qf.release(); d->eatom.release();
   for (DBuf<int>* b : { &d->type, &d->body, &d->perm, &d->freeFlag, &d->chargedFlag, &d->tmpI, &d->key, &d->cellCount, &d->cellStart, &d->order, &d->blockSums, &d->nbr, &d->nnb, &d->flags, &d->exOff, &d->exItem, &d->iperm }) b->release();
   d->pairs.release(); d->couls.release(); d->interact.release();
   dev_ewald_release(d);
@@ -1192,6 +1202,7 @@ void dev_compute_pairs(Device* d, const PairCall& c, ForceSums* out) {
     tpa = want >= 16 ? 16 : (want >= 4 ? 4 : 1);
   }
   int gp = div_up((long)N * tpa, BLOCK);
+  double* ea = d->wantAtomE ? d->eatom.p : nullptr;
   int variant = (c.energy ? 4 : 0) + (U ? 2 : 0) + (LJ ? 1 : 0);
   if (c.mixed) {
     // cutoff in fixed-point units: Rc2 (scaled) * 2^64
@@ -1204,7 +1215,7 @@ void dev_compute_pairs(Device* d, const PairCall& c, ForceSums* out) {
     mc.unit = c.L / two32;
 #define LAUNCH_MX(E, U, J, T)                                                                                      \
   hipLaunchKernelGGL((k_pairs_mx<E, U, J, T>), dim3(gp), dim3(BLOCK), 0, d->stream, F, d->partials.p, d->Q4.p,     \
-                     d->Rs4.p, d->nbr.p, d->nnb.p, d->qf.p, pt, cp, c.ntypes, mc, N, Np)
+                     d->Rs4.p, d->nbr.p, d->nnb.p, d->qf.p, pt, cp, c.ntypes, mc, N, Np, ea)
 #define GENERIC_MX(E, U)                                                                                           \
   do {                                                                                                            \
     if (tpa == 16) LAUNCH_MX(E, U, false, 16); else if (tpa == 4) LAUNCH_MX(E, U, false, 4);                      \
@@ -1223,7 +1234,7 @@ void dev_compute_pairs(Device* d, const PairCall& c, ForceSums* out) {
   } else {
 #define LAUNCH_PAIRS(E, U, J, T)                                                                                    \
   hipLaunchKernelGGL((k_pairs<E, false, U, J, T>), dim3(gp), dim3(BLOCK), 0, d->stream, F, d->partials.p, d->Rs4.p, \
-                     d->nbr.p, d->nnb.p, d->charge.p, d->chargedFlag.p, pt, cp, c.ntypes, c.L, c.invL2, c.Rc2, N, Np)
+                     d->nbr.p, d->nnb.p, d->charge.p, d->chargedFlag.p, pt, cp, c.ntypes, c.L, c.invL2, c.Rc2, N, Np, ea)
 #define GENERIC(E, U)                                                                                              \
   do {                                                                                                            \
     if (tpa == 16) LAUNCH_PAIRS(E, U, false, 16); else if (tpa == 4) LAUNCH_PAIRS(E, U, false, 4);                \
@@ -1333,6 +1344,23 @@ void dev_rdf(Device* d, int bins, double Rc, double L, const std::vector<uint8_t
   for (size_t k = 0; k < h.size(); k++) count[k] = (long long)h[k];
   c.release();
   on.release();
+}
+
+void dev_set_atom_energies(Device* d, bool on) {
+  d->wantAtomE = on;
+  if (on) {
+    d->eatom.alloc(d->Npad);
+    HIP_CHECK(hipMemsetAsync(d->eatom.p, 0, sizeof(double) * d->Npad, d->stream));
+  }
+}
+void dev_get_atom_energies(Device* d, double* host) {
+  int N = d->N;
+  std::vector<double> e(N);
+  std::vector<int> perm(N);
+  HIP_CHECK(hipMemcpyAsync(e.data(), d->eatom.p, sizeof(double) * N, hipMemcpyDeviceToHost, d->stream));
+  HIP_CHECK(hipMemcpyAsync(perm.data(), d->perm.p, sizeof(int) * N, hipMemcpyDeviceToHost, d->stream));
+  HIP_CHECK(hipStreamSynchronize(d->stream));
+  for (int s = 0; s < N; s++) host[perm[s]] = e[s];
 }
 
 void dev_set_profiling(Device* d, bool on) { d->profiling = on; d->pairMs = d->listMs = 0; d->pairLaunches = d->listLaunches = 0; }

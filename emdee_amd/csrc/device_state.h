@@ -145,6 +145,8 @@ struct Device {
   BodyState bodies;
   DBuf<int> iperm;
   // k-space (ewald.hip) and bonded terms (bonded.hip), created on demand
+  DBuf<double> eatom;     // atom energies (slot order) when wantAtomE
+  bool wantAtomE = false;
   DBuf<double> shadow;    // EmDee_verlet_step: s0 [3][N] (original order), then r0 [3][nb], q0 [4][nb]
   EwaldState* ewald = nullptr;
   StructState* structs = nullptr;

@@ -151,6 +151,12 @@ void dev_shadow_post(Device* d, int layer, double dt, int mode, double* out3);
 void dev_rdf(Device* d, int bins, double Rc, double L, const std::vector<uint8_t>& pairOn, int npt,
              std::vector<long long>& count);
 
+// Atom energies: while on, force evaluations also write each atom's share of
+// the potential energy (half of each of its pair terms, its k-space energy
+// and self energy); get returns them in original atom order.
+void dev_set_atom_energies(Device* d, bool on);
+void dev_get_atom_energies(Device* d, double* host);
+
 // Profiling (HIP events around the pair kernel).
 void dev_set_profiling(Device* d, bool on);
 void dev_kernel_stats(Device* d, double* out, int n);

@@ -40,7 +40,7 @@ struct EwaldState {
   DBuf<int4> kn;
   DBuf<double4> kv;
   DBuf<int> item, groupType, typeGroups;
-  DBuf<double> q, lambda, Fpart, part, result;
+  DBuf<double> q, lambda, Fpart, part, result, selfc;
   DBuf<double2> T, Spart, S, sigma;
   std::vector<int> hostTypeGroups;
 };
@@ -54,7 +54,7 @@ void dev_ewald_release(Device* d) {
   EwaldState* e = d->ewald;
   if (!e) return;
   e->kn.release(); e->kv.release(); e->item.release(); e->groupType.release(); e->typeGroups.release();
-  e->q.release(); e->lambda.release(); e->Fpart.release(); e->part.release(); e->result.release();
+  e->q.release(); e->lambda.release(); e->Fpart.release(); e->part.release(); e->result.release(); e->selfc.release();
   e->T.release(); e->Spart.release(); e->S.release(); e->sigma.release();
   delete e;
   d->ewald = nullptr;
@@ -177,7 +177,10 @@ __global__ void __launch_bounds__(BLOCK) k_ew_sigma(double2* __restrict__ S, dou
 }
 
 // Partial k-space forces over the k range of blockIdx.y:
-// Fpart[(y*3 + x)*nslots + j].  Every wavefront's slots hold one charged type.
+// Fpart[(y*4 + x)*nslots + j], x = 0..2; x = 3 holds the atom's share of the
+// reciprocal energy, sum_k prefac Re(q_j e^{ik.r_j} conj(sigma_I(k))), whose
+// sum over atoms is kspace_ewald.f90:287.  Every wavefront's slots hold one
+// charged type.
 __global__ void __launch_bounds__(EW_BLOCK) k_ew_forces(double* __restrict__ Fpart, const double2* __restrict__ T,
                                                         const double* __restrict__ q, const int4* __restrict__ kn,
                                                         const double4* __restrict__ kv,
@@ -191,7 +194,7 @@ __global__ void __launch_bounds__(EW_BLOCK) k_ew_forces(double* __restrict__ Fpa
   double qj = q[j];
   size_t zoff = (2 * (size_t)(nm + 1) + 1) * nslots;   // G3(1) = e^{2 pi i s_z}
   double2 z = T[zoff + j], v = make_double2(0.0, 0.0);
-  double fx = 0.0, fy = 0.0, fz = 0.0;
+  double fx = 0.0, fy = 0.0, fz = 0.0, er = 0.0;
   for (int k = k0; k < k1; k++) {
     int4 n = kn[k];
     v = (k == k0 || n.w) ? row_start(T, qj, n, j, nm, nslots) : cmul(v, z);
@@ -201,25 +204,32 @@ __global__ void __launch_bounds__(EW_BLOCK) k_ew_forces(double* __restrict__ Fpa
     fx += w.x * c;
     fy += w.y * c;
     fz += w.z * c;
+    er += w.w * (s.x * v.x + s.y * v.y);
   }
-  size_t o = (size_t)blockIdx.y * 3 * nslots + j;
+  size_t o = (size_t)blockIdx.y * 4 * nslots + j;
   Fpart[o] = fx;
   Fpart[o + nslots] = fy;
   Fpart[o + 2 * (size_t)nslots] = fz;
+  Fpart[o + 3 * (size_t)nslots] = er;
 }
 
-// F(slot of atom) += sum over k ranges (fixed order), kspace_ewald.f90:297.
-__global__ void k_ew_force_add(double* __restrict__ F, const double* __restrict__ Fpart, const int* __restrict__ item,
-                               const int* __restrict__ iperm, int nslots, int ks, int Npad) {
+// F(slot of atom) += sum over k ranges (fixed order), kspace_ewald.f90:297;
+// with eatom, also the atom's reciprocal and self energies
+// (self: -lambda_II alpha q^2 / sqrt(pi), modelClass_kspace.f90:223-229).
+__global__ void k_ew_force_add(double* __restrict__ F, double* __restrict__ eatom, const double* __restrict__ Fpart,
+                               const int* __restrict__ item, const int* __restrict__ iperm,
+                               const double* __restrict__ q, const int* __restrict__ groupType,
+                               const double* __restrict__ selfc, int nslots, int ks, int Npad) {
   int j = blockIdx.x * blockDim.x + threadIdx.x;
   if (j >= nslots) return;
   int i = item[j];
   if (i < 0) return;
-  double f[3] = { 0.0, 0.0, 0.0 };
+  double f[4] = { 0.0, 0.0, 0.0, 0.0 };
   for (int y = 0; y < ks; y++)
-    for (int x = 0; x < 3; x++) f[x] += Fpart[((size_t)y * 3 + x) * nslots + j];
+    for (int x = 0; x < 4; x++) f[x] += Fpart[((size_t)y * 4 + x) * nslots + j];
   int s = iperm[i];
   for (int x = 0; x < 3; x++) F[x * (size_t)Npad + s] += f[x];
+  if (eatom) eatom[s] += f[3] + selfc[groupType[j / EWALD_GROUP]] * q[j] * q[j];
 }
 
 void dev_ewald_setup(Device* d, const EwaldSetup& s) {
@@ -261,7 +271,12 @@ void dev_ewald_setup(Device* d, const EwaldSetup& s) {
   e->Spart.alloc((size_t)e->ngroups * s.nvecs);
   e->S.alloc((size_t)s.nt * s.nvecs);
   e->sigma.alloc((size_t)s.nt * s.nvecs);
-  e->Fpart.alloc((size_t)e->ksF * 3 * s.nslots);
+  e->Fpart.alloc((size_t)e->ksF * 4 * s.nslots);
+  std::vector<double> selfc((size_t)s.nl * s.nt);
+  for (int l = 0; l < s.nl; l++)
+    for (int t = 0; t < s.nt; t++)
+      selfc[(size_t)l * s.nt + t] = -s.alpha * s.lambda[(size_t)l * s.nt * s.nt + t * s.nt + t] / std::sqrt(3.14159265358979324);
+  put(e->selfc, selfc);
   e->part.alloc(div_up(s.nvecs, BLOCK));
   e->result.alloc(1);
 }
@@ -290,8 +305,9 @@ double dev_ewald_compute(Device* d, int layer, double L) {
                      e->T.p, e->q.p, e->kn.p, e->kv.p, e->sigma.p, e->groupType.p, e->nm, e->nslots, e->nvecs,
                      e->kchF);
   launch_check("ewald forces");
-  hipLaunchKernelGGL(k_ew_force_add, dim3(div_up(e->nslots, BLOCK)), dim3(BLOCK), 0, d->stream, F, e->Fpart.p,
-                     e->item.p, d->iperm.p, e->nslots, e->ksF, Np);
+  hipLaunchKernelGGL(k_ew_force_add, dim3(div_up(e->nslots, BLOCK)), dim3(BLOCK), 0, d->stream, F,
+                     d->wantAtomE ? d->eatom.p : nullptr, e->Fpart.p, e->item.p, d->iperm.p, e->q.p, e->groupType.p,
+                     e->selfc.p + (size_t)layer * e->nt, e->nslots, e->ksF, Np);
   launch_check("ewald force add");
   double E = 0;
   HIP_CHECK(hipMemcpyAsync(d->hresult, e->result.p, sizeof(double), hipMemcpyDeviceToHost, d->stream));

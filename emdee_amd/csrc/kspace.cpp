@@ -73,6 +73,7 @@ EwaldHost ewald_initialize(double accuracy, double Rc, double L, int nlayers, co
   h.alpha = s / Rc;
   h.kmax = 2.0 * h.alpha * s;
   h.beta = 2.0 * h.alpha / std::sqrt(PI_);
+  h.dev.alpha = h.alpha;
 
   // slots: every atom of the charged types, as cfirst/clast/citem of
   // modelClass_kspace.f90:141-150 (all atoms of a charged type are listed)

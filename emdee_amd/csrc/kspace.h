@@ -20,6 +20,7 @@ constexpr int EWALD_GROUP = 256;
 
 struct EwaldSetup {
   int nm = 0, nvecs = 0, nt = 0, nl = 0, nslots = 0;
+  double alpha = 0;            // Ewald splitting parameter
   std::vector<int> kn;         // [nvecs][4]: n1, n2, n3, row-start flag
   std::vector<double> kv;      // [nvecs][4]: 2 prefac k (x, y, z), prefac
   std::vector<int> slotItem;   // [nslots]: original atom index (0-based) or -1

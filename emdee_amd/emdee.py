@@ -121,6 +121,16 @@ GPU_SIGNATURES = {
     "EmDeeHMC_destroy": (None, [c_void_p]),
     "EmDeeHMC_momenta": (None, [ctypes.c_ulonglong, c_int, c_int, c_int, c_double, _dp]),
     "EmDeeHMC_uniform": (c_double, [ctypes.c_ulonglong, c_int, c_int]),
+    "EmDeeGPU_atom_energies": (None, [POINTER(tEmDee), _dp]),
+    "EmDeeSurrogate_create": (c_void_p, [c_int, c_double, ctypes.c_ulonglong]),
+    "EmDeeGPU_surrogate_sample": (None, [tEmDee, c_void_p, _dp]),
+    "EmDeeSurrogate_rows": (c_int, [c_void_p]),
+    "EmDeeSurrogate_train": (c_double, [c_void_p, c_int, c_int, c_double]),
+    "EmDeeSurrogate_param_count": (c_int, [c_void_p]),
+    "EmDeeSurrogate_get_params": (None, [c_void_p, POINTER(ctypes.c_float), POINTER(ctypes.c_float)]),
+    "EmDeeSurrogate_set_params": (None, [c_void_p, POINTER(ctypes.c_float), POINTER(ctypes.c_float)]),
+    "EmDeeSurrogate_destroy": (None, [c_void_p]),
+    "EmDeeGPU_attach_surrogate": (None, [tEmDee, c_void_p, c_int]),
 }
 
 
@@ -204,6 +214,11 @@ class EmDeeLibrary:
         out = np.zeros((max(n, 1), 2), dtype=np.int32)
         self.lib.EmDeeGPU_neighbor_pairs(md, n, out.ctypes.data_as(_ip))
         return out[:n]
+
+    def atom_energies(self, md, N):
+        e = np.zeros(N)
+        self.lib.EmDeeGPU_atom_energies(ctypes.byref(md), e.ctypes.data_as(_dp))
+        return e
 
     def kernel_stats(self, md):
         out = np.zeros(8)

@@ -103,3 +103,24 @@ def integrated_autocorrelation(x):
             break
         tau += 2.0 * g
     return max(tau, 1.0)
+
+
+if __name__ == "__main__":
+    # Profiling workload: python -m emdee_amd.hmc [replicas] [cycles]
+    import sys
+    import time
+
+    from . import load
+    nrep = int(sys.argv[1]) if len(sys.argv) > 1 else 512
+    cycles = int(sys.argv[2]) if len(sys.argv) > 2 else 4
+    lib = load()
+    N, L, T, Rs = replica_set(nrep, seed=1)
+    h = BatchedHMC(lib, N, L, T, Rs, seed=7, precision=1)
+    print("launch shape", h.tune(), flush=True)
+    t0 = time.perf_counter()
+    h.run(launches=cycles, cycles=1)
+    h.synchronize()
+    t = time.perf_counter() - t0
+    U, acc = h.energies()
+    print(f"{nrep} replicas, {int(N.sum())} atoms, {cycles} cycles in {t:.3f} s; accepted {int(acc.sum())}", flush=True)
+    h.close()

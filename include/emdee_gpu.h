@@ -64,6 +64,34 @@ void EmDeeHMC_destroy( EmDeeHMC* h );
 void   EmDeeHMC_momenta( unsigned long long seed, int replica, int cycle, int n, double kT, double* P );
 double EmDeeHMC_uniform( unsigned long long seed, int replica, int cycle );
 
+/* Atom energies of the current layer in original atom order (evaluates the
+   forces): half of each pair term of the atom, its k-space (reciprocal)
+   energy and self energy, and its share of an attached learned model.  Bonded
+   terms and the constant intrabody k-space correction are not included. */
+void EmDeeGPU_atom_energies( tEmDee* md, double* e );
+
+/* Learned per-atom energy model (radial-Gaussian descriptors, 32 per
+   neighbour species within rcut, a 2 x 128 SiLU MLP per species on BF16 MFMA
+   with FP32 accumulation), trained on per-atom energy labels.  Attached to a
+   model layer it adds its energy, forces and virial to that layer, standing
+   in for a costly term (e.g. the k-space electrostatics).  No counterpart in
+   the reference. */
+typedef struct EmDeeSurrogate EmDeeSurrogate;
+EmDeeSurrogate* EmDeeSurrogate_create( int ntypes, double rcut, unsigned long long seed );
+/* Stores the descriptors of md's current configuration (its neighbor list
+   must be current, e.g. after a force evaluation) with one label per atom. */
+void   EmDeeGPU_surrogate_sample( tEmDee md, EmDeeSurrogate* s, const double* labels );
+int    EmDeeSurrogate_rows( EmDeeSurrogate* s );
+/* Adam on the per-atom mean squared error of the normalised labels; returns
+   the last epoch's loss.  The first call fixes the normalisation. */
+double EmDeeSurrogate_train( EmDeeSurrogate* s, int epochs, int batchRows, double lr );
+int    EmDeeSurrogate_param_count( EmDeeSurrogate* s );
+void   EmDeeSurrogate_get_params( EmDeeSurrogate* s, float* params, float* norm );
+void   EmDeeSurrogate_set_params( EmDeeSurrogate* s, const float* params, const float* norm );
+void   EmDeeSurrogate_destroy( EmDeeSurrogate* s );
+/* layer 1..layers, or s = NULL to detach */
+void   EmDeeGPU_attach_surrogate( tEmDee md, EmDeeSurrogate* s, int layer );
+
 /* Waits for all device work of the system. */
 void EmDeeGPU_synchronize( tEmDee md );
 
