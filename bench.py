@@ -34,6 +34,20 @@ sys.path.insert(0, ROOT)
 
 METRIC = "atom-steps/s vs plain FP64 at stated force error; HMC effective samples/s"
 HBM_PEAK_GBS = 8000.0
+TRAFFIC_PROFILE = os.path.join(ROOT, "profiles", "k_pairs_lj1m_traffic.txt")
+
+
+def measured_traffic(form):
+    """HBM bytes per launch of the pair kernel from the committed rocprofv3
+    FETCH_SIZE/WRITE_SIZE summary (calibrated counters), or None."""
+    try:
+        with open(TRAFFIC_PROFILE) as f:
+            for line in f:
+                if line.startswith(f"traffic_bytes_per_launch[{form}]"):
+                    return float(line.split("=")[1])
+    except OSError:
+        pass
+    return None
 
 
 def fcc_lattice(natoms, density):
@@ -220,10 +234,16 @@ def main():
         "config": {"workload": "lj_fluid_fcc", "natoms": N, "density": 0.8442, "T": 1.0, "rc": 2.5,
                    "skin": 0.3, "dt": dt, "integrator": "boost/displace/boost via C ABI, energies every step",
                    "fp64_value": v64, "mixed_value": vmx, "mixed_over_fp64": vmx / v64,
+                   "fp64_pair_kernel_ms": r64["pair_ms"], "mixed_pair_kernel_ms": rmx["pair_ms"],
+                   "mixed_kernel_over_fp64_kernel": r64["pair_ms"] / rmx["pair_ms"],
+                   "list_build_ms": best["list_ms"],
                    "force_rms_error": err, "list_builds": best["builds"], "half_list_pairs_per_atom": n_half},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
-                     "kernel": "k_pairs", "kernel_ms": best["pair_ms"], "bytes_per_launch": bytes_per_launch},
+                     "frac": achieved / HBM_PEAK_GBS,
+                     "traffic": measured_traffic("mixed" if use_mixed else "fp64"),
+                     "kernel": "k_pairs_mx" if use_mixed else "k_pairs", "kernel_ms": best["pair_ms"],
+                     "bytes_per_launch": bytes_per_launch,
+                     "traffic_source": "profiles/k_pairs_lj1m_traffic.txt"},
         "cpu_baseline": cpu,
         "hmc": hmc,
     }

@@ -83,6 +83,8 @@ struct EmDeeSurrogate {
   long workRows = 0;
   DBuf<float> X, Z1, H1, Z2, H2, G2, G1, dX, e, dXsum, part;
   DBuf<double> Fw, red;          // results; per-block partials
+  DBuf<float4> lE;               // compact neighbour list (k_sg_list)
+  DBuf<int> lJ, lN, lOver;
   hipStream_t own = nullptr;     // training stream
   hipStream_t stream = nullptr;  // stream of the current operation
 };
@@ -103,48 +105,78 @@ struct ParamLayout {
 
 // ---- descriptors ------------------------------------------------------------
 
-__device__ inline void pair_sep(const double* __restrict__ R, int Npad, int i, int j, double L, float& dx, float& dy,
-                                float& dz) {
+__device__ inline void pair_sep(const double* __restrict__ R, int Npad, int i, int j, double L, double invL,
+                                float& dx, float& dy, float& dz) {
   double ex = R[i] - R[j], ey = R[Npad + i] - R[Npad + j], ez = R[2 * Npad + i] - R[2 * Npad + j];
-  ex -= L * rint(ex / L);
-  ey -= L * rint(ey / L);
-  ez -= L * rint(ez / L);
+  ex -= L * rint(ex * invL);
+  ey -= L * rint(ey * invL);
+  ez -= L * rint(ez * invL);
   dx = (float)ex; dy = (float)ey; dz = (float)ez;
 }
 
-// Raw descriptors of every slot: D[slot][t*NB + k].
-template <int NT>
-__global__ void __launch_bounds__(BLOCK) k_sg_desc(float* __restrict__ D, const double* __restrict__ R,
-                                                   const int* __restrict__ type, const int* __restrict__ nbr,
-                                                   const int* __restrict__ nnb, SurrogateParams sp, double L, int N,
+// Compact neighbour list within rc, built from the engine's Verlet list at
+// every use: SG_SUBS consecutive lanes per atom scan interleaved parts of its
+// list and each appends its hits to its own segment; an entry keeps the
+// separation vector and distance (FP32) and the neighbour's slot with its
+// species in the low two bits.  Segments are interleaved: entry c of segment
+// s of atom i sits at (i SG_SEG + c) SG_SUBS + s, so the lanes of one atom
+// write and read neighbouring addresses.
+constexpr int SG_SUBS = 16, SG_SEG = 40;
+__device__ inline size_t sg_at(int i, int c, int s) { return ((size_t)i * SG_SEG + c) * SG_SUBS + s; }
+__global__ void __launch_bounds__(BLOCK) k_sg_list(float4* __restrict__ E, int* __restrict__ J, int* __restrict__ cnt,
+                                                   int* __restrict__ overflow, const double* __restrict__ R,
+                                                   const int* __restrict__ nbr, const int* __restrict__ nnb,
+                                                   const int* __restrict__ type, float rc, double L, int N,
                                                    int Npad) {
-  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  int gt = blockIdx.x * blockDim.x + threadIdx.x;
+  int i = gt / SG_SUBS, sub = gt % SG_SUBS;
   if (i >= N) return;
-  float acc[NT * SG_NB];
-#pragma unroll
-  for (int f = 0; f < NT * SG_NB; f++) acc[f] = 0.0f;
-  const float rc2 = sp.rc * sp.rc;
-  int n = nnb[i];
-  for (int k = 0; k < n; k++) {
+  const float rc2 = rc * rc;
+  const double invL = 1.0 / L;
+  int n = nnb[i], c = 0;
+  for (int k = sub; k < n; k += SG_SUBS) {
     int j = nbr[(size_t)k * Npad + i];
     float dx, dy, dz;
-    pair_sep(R, Npad, i, j, L, dx, dy, dz);
+    pair_sep(R, Npad, i, j, L, invL, dx, dy, dz);
     float r2 = dx * dx + dy * dy + dz * dz;
-    if (r2 >= rc2) continue;
-    float r = sqrtf(r2), fc = 0.5f * (__cosf(PI_F * r / sp.rc) + 1.0f);
-    int t = type[j];
+    if (r2 < rc2) {
+      if (c < SG_SEG) { E[sg_at(i, c, sub)] = make_float4(dx, dy, dz, sqrtf(r2)); J[sg_at(i, c, sub)] = (j << 2) | type[j]; }
+      c++;
+    }
+  }
+  cnt[i * SG_SUBS + sub] = min(c, SG_SEG);
+  if (c > SG_SEG) atomicMax(overflow, c);
+}
+
+// Descriptors from the compact list: 32 lanes per atom, lane b owns Gaussian b
+// of every neighbour species; entries are read once per atom (broadcast).
+template <int NT>
+__global__ void __launch_bounds__(BLOCK) k_sg_desc2(float* __restrict__ D, const float4* __restrict__ E,
+                                                    const int* __restrict__ J, const int* __restrict__ cnt,
+                                                    const int* __restrict__ type, const float* __restrict__ mean,
+                                                    const float* __restrict__ sd, SurrogateParams sp, int N) {
+  int gt = blockIdx.x * blockDim.x + threadIdx.x;
+  int i = gt / SG_NB, b = gt % SG_NB;
+  if (i >= N) return;
+  float acc[NT];
 #pragma unroll
-    for (int q = 0; q < NT; q++) {
-      if (q != t) continue;
+  for (int t = 0; t < NT; t++) acc[t] = 0.0f;
+  const float mu = sp.mu0 + b * sp.dmu, kpi = PI_F / sp.rc;
+  for (int s = 0; s < SG_SUBS; s++) {
+    int n = cnt[i * SG_SUBS + s];
+    for (int c = 0; c < n; c++) {
+      float r = E[sg_at(i, c, s)].w;
+      int t = J[sg_at(i, c, s)] & 3;
+      float u = r - mu, g = __expf(-sp.eta * u * u) * 0.5f * (__cosf(kpi * r) + 1.0f);
 #pragma unroll
-      for (int b = 0; b < SG_NB; b++) {
-        float u = r - (sp.mu0 + b * sp.dmu);
-        acc[q * SG_NB + b] += __expf(-sp.eta * u * u) * fc;
-      }
+      for (int q = 0; q < NT; q++) acc[q] += (q == t) ? g : 0.0f;
     }
   }
 #pragma unroll
-  for (int f = 0; f < NT * SG_NB; f++) D[(size_t)i * (NT * SG_NB) + f] = acc[f];
+  for (int t = 0; t < NT; t++) {
+    int f = t * SG_NB + b;
+    D[(size_t)i * (NT * SG_NB) + f] = mean ? (acc[t] - mean[f]) / sd[f] : acc[t];   // normalised when mean given
+  }
 }
 
 __global__ void k_sg_normalise(float* __restrict__ X, const float* __restrict__ D, const float* __restrict__ mean,
@@ -161,36 +193,55 @@ __global__ void k_sg_normalise(float* __restrict__ X, const float* __restrict__ 
 // transposed, [NC][K] in bf16, so a lane loads its 4 K-values contiguously.
 // Lane layout of v_mfma_f32_16x16x16_bf16: A[i][k] with i = lane%16,
 // k = 4 (lane/16) + r; B[k][j] with j = lane%16; D[i][j] with i = 4 (lane/16) + r.
-template <int K, int NC, bool ACT>
+enum { EPI_Z = 0, EPI_SILU = 1, EPI_DSILU = 2, EPI_ADD = 3 };
+// One block of 4 wavefronts computes 16 rows x NC columns; wave w takes the
+// column tiles w, w+4, ...  Epilogues: EPI_Z stores Z = A.B (+ bias);
+// EPI_SILU also stores H = SiLU(Z); EPI_DSILU stores Z * SiLU'(aux);
+// EPI_ADD adds Z into the output.
+template <int K, int NC, int EPI>
 __global__ void __launch_bounds__(256) k_sg_gemm(const float* __restrict__ A, const unsigned short* __restrict__ Bt,
                                                  const float* __restrict__ bias, float* __restrict__ Z,
-                                                 float* __restrict__ Hout) {
+                                                 float* __restrict__ Hout, const float* __restrict__ aux) {
+  constexpr int TILES = NC / 16, PER = (TILES + 3) / 4;
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-  const long r0 = (long)blockIdx.x * SG_ROWS + 16 * w;
+  const long r0 = (long)blockIdx.x * 16;
   const int ai = lane & 15, ak = 4 * (lane >> 4);
-  f32x4 acc[NC / 16];
+  f32x4 acc[PER];
 #pragma unroll
-  for (int t = 0; t < NC / 16; t++) acc[t] = f32x4{ 0.0f, 0.0f, 0.0f, 0.0f };
+  for (int m = 0; m < PER; m++) acc[m] = f32x4{ 0.0f, 0.0f, 0.0f, 0.0f };
+  // all fragments are loaded before the MFMA chain so that the loads overlap
+  float4 av[K / 16];
+  bf16x4 bv[PER][K / 16];
 #pragma unroll
-  for (int k0 = 0; k0 < K; k0 += 16) {
-    float4 av = *reinterpret_cast<const float4*>(A + (r0 + ai) * K + k0 + ak);
-    bf16x4 a = { (short)f2bf(av.x), (short)f2bf(av.y), (short)f2bf(av.z), (short)f2bf(av.w) };
+  for (int q = 0; q < K / 16; q++) av[q] = *reinterpret_cast<const float4*>(A + (r0 + ai) * K + 16 * q + ak);
 #pragma unroll
-    for (int t = 0; t < NC / 16; t++) {
-      bf16x4 b = *reinterpret_cast<const bf16x4*>(Bt + (size_t)(16 * t + ai) * K + k0 + ak);
-      acc[t] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, acc[t], 0, 0, 0);
-    }
+  for (int m = 0; m < PER; m++) {
+    int t = w + 4 * m < TILES ? w + 4 * m : TILES - 1;
+#pragma unroll
+    for (int q = 0; q < K / 16; q++)
+      bv[m][q] = *reinterpret_cast<const bf16x4*>(Bt + (size_t)(16 * t + ai) * K + 16 * q + ak);
   }
 #pragma unroll
-  for (int t = 0; t < NC / 16; t++) {
+  for (int q = 0; q < K / 16; q++) {
+    bf16x4 a = { (short)f2bf(av[q].x), (short)f2bf(av[q].y), (short)f2bf(av[q].z), (short)f2bf(av[q].w) };
+#pragma unroll
+    for (int m = 0; m < PER; m++)
+      if (w + 4 * m < TILES) acc[m] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, bv[m][q], acc[m], 0, 0, 0);
+  }
+#pragma unroll
+  for (int m = 0; m < PER; m++) {
+    int t = w + 4 * m;
+    if (t >= TILES) continue;
     int j = 16 * t + ai;
     float bj = bias ? bias[j] : 0.0f;
 #pragma unroll
     for (int r = 0; r < 4; r++) {
-      long i = r0 + ak + r;
-      float z = acc[t][r] + bj;
-      Z[i * NC + j] = z;
-      if (ACT) Hout[i * NC + j] = silu(z);
+      long o = (r0 + ak + r) * NC + j;
+      float z = acc[m][r] + bj;
+      if (EPI == EPI_Z) Z[o] = z;
+      else if (EPI == EPI_SILU) { Z[o] = z; Hout[o] = silu(z); }
+      else if (EPI == EPI_DSILU) Z[o] = z * dsilu(aux[o]);
+      else Z[o] += z;
     }
   }
 }
@@ -252,48 +303,47 @@ __global__ void k_sg_reduce(float* __restrict__ dst, const float* __restrict__ p
   dst[idx] = s;
 }
 
-// Output layer of species s and the seed of the backward pass.
+// Output layer of species s and the seed of the backward pass, one
+// wavefront per row (two columns per lane).
 //   mode 0 (inference): eatom[row] = ystd out + ymean for rows of species s,
 //       seed = ystd (dE/d out).
 //   mode 1 (training): loss partials of (out - y)^2, seed = 2 (out - y) / count.
 // G2 = seed * w3 * SiLU'(Z2) (zero for rows of other species).
-__global__ void k_sg_output(float* __restrict__ G2, float* __restrict__ eatom, float* __restrict__ seedOut,
-                            double* __restrict__ red, const float* __restrict__ H2, const float* __restrict__ Z2,
-                            const float* __restrict__ p, const int* __restrict__ species, const float* __restrict__ y,
-                            int s, float ymean, float ystd, float invCount, long rows, int mode, ParamLayout pl) {
-  long row = (long)blockIdx.x * blockDim.x + threadIdx.x;
+__global__ void __launch_bounds__(256) k_sg_output(float* __restrict__ G2, float* __restrict__ eatom,
+                                                   float* __restrict__ seedOut, double* __restrict__ red,
+                                                   const float* __restrict__ H2, const float* __restrict__ Z2,
+                                                   const float* __restrict__ p, const int* __restrict__ species,
+                                                   const float* __restrict__ y, int s, float ymean, float ystd,
+                                                   float invCount, long rows, int mode, ParamLayout pl) {
+  const int lane = threadIdx.x & 63;
+  long row = (long)blockIdx.x * 4 + (threadIdx.x >> 6);
   double v[1] = { 0.0 };
   if (row < rows) {
     const float* w3 = p + pl.w3();
-    float out = p[pl.b3()];
-    for (int n = 0; n < SG_H; n++) out += H2[row * SG_H + n] * w3[n];
+    const float* h = H2 + row * SG_H;
+    float part = h[lane] * w3[lane] + h[lane + 64] * w3[lane + 64];
+    float out = wave_sum(part) + p[pl.b3()];
+    out = __shfl(out, 0, 64);
     bool mine = species[row] == s;
     float seed = 0.0f;
     if (mine) {
       if (mode == 0) {
-        eatom[row] = ystd * out + ymean;
         seed = ystd;
-        v[0] = (double)eatom[row];
+        if (lane == 0) { eatom[row] = ystd * out + ymean; v[0] = (double)(ystd * out + ymean); }
       } else {
-        float d = out - y[row];
-        seed = 2.0f * d * invCount;
-        v[0] = (double)d * (double)d;
+        float dd = out - y[row];
+        seed = 2.0f * dd * invCount;
+        if (lane == 0) v[0] = (double)dd * (double)dd;
       }
     }
-    seedOut[row] = seed;
-    for (int n = 0; n < SG_H; n++) G2[row * SG_H + n] = seed * w3[n] * dsilu(Z2[row * SG_H + n]);
+    if (lane == 0) seedOut[row] = seed;
+    const float* z = Z2 + row * SG_H;
+    float* g = G2 + row * SG_H;
+    g[lane] = seed * w3[lane] * dsilu(z[lane]);
+    g[lane + 64] = seed * w3[lane + 64] * dsilu(z[lane + 64]);
   }
   block_sum<1>(v);
   if (threadIdx.x == 0) red[blockIdx.x] = v[0];
-}
-
-__global__ void k_sg_dsilu(float* __restrict__ G, const float* __restrict__ Z, long n) {
-  long idx = (long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (idx < n) G[idx] *= dsilu(Z[idx]);
-}
-__global__ void k_sg_accumulate(float* __restrict__ dst, const float* __restrict__ src, long n, int first) {
-  long idx = (long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (idx < n) dst[idx] = first ? src[idx] : dst[idx] + src[idx];
 }
 
 // Adam (Kingma & Ba 2015) on the fp32 master parameters of one species.
@@ -328,48 +378,54 @@ __global__ void k_sg_pack(unsigned short* __restrict__ w1t, unsigned short* __re
   }
 }
 
-// Forces from dE/dD (normalised-input gradient dXn / xstd) and the model's
-// pair virial; F (slot order, SoA) is incremented.
+// Forces from dE/dD over the compact list: 16 lanes per atom share its
+// entries; each entry sums its 32 Gaussian derivatives; the lanes' forces are
+// combined by a fixed butterfly.
 template <int NT>
-__global__ void __launch_bounds__(BLOCK) k_sg_force(double* __restrict__ F, double* __restrict__ red,
-                                                    const float* __restrict__ dXn, const float* __restrict__ xstd,
-                                                    const double* __restrict__ R, const int* __restrict__ type,
-                                                    const int* __restrict__ nbr, const int* __restrict__ nnb,
-                                                    SurrogateParams sp, double L, int N, int Npad) {
-  const int din = NT * SG_NB;
-  int i = blockIdx.x * blockDim.x + threadIdx.x;
-  double w[1] = { 0.0 };
+__global__ void __launch_bounds__(BLOCK) k_sg_force2(double* __restrict__ F, double* __restrict__ red,
+                                                     const float* __restrict__ dXn, const float* __restrict__ xstd,
+                                                     const float4* __restrict__ E, const int* __restrict__ J,
+                                                     const int* __restrict__ cnt, const int* __restrict__ type,
+                                                     SurrogateParams sp, int N, int Npad) {
+  constexpr int TPA = 16, din = NT * SG_NB;
+  int gt = blockIdx.x * blockDim.x + threadIdx.x;
+  int i = gt / TPA, sub = gt % TPA;
+  double fx = 0, fy = 0, fz = 0, w[1] = { 0.0 };
+  static_assert(TPA == SG_SUBS, "one lane per list segment");
   if (i < N) {
-    const float rc2 = sp.rc * sp.rc, kpi = PI_F / sp.rc;
+    const float kpi = PI_F / sp.rc;
     int ti = type[i];
-    double fx = 0, fy = 0, fz = 0;
-    int n = nnb[i];
-    for (int k = 0; k < n; k++) {
-      int j = nbr[(size_t)k * Npad + i];
-      float dx, dy, dz;
-      pair_sep(R, Npad, i, j, L, dx, dy, dz);
-      float r2 = dx * dx + dy * dy + dz * dz;
-      if (r2 >= rc2) continue;
-      float r = sqrtf(r2);
-      float fc = 0.5f * (__cosf(kpi * r) + 1.0f), dfc = -0.5f * kpi * __sinf(kpi * r);
-      int tj = type[j];
-      const float* gi = dXn + (size_t)i * din + tj * SG_NB;
-      const float* gj = dXn + (size_t)j * din + ti * SG_NB;
-      const float* si = xstd + tj * SG_NB;
-      const float* sj = xstd + ti * SG_NB;
-      float dEdr = 0.0f;
+    {
+      int n = cnt[i * SG_SUBS + sub];
+      for (int c = 0; c < n; c++) {
+        float4 e = E[sg_at(i, c, sub)];
+        int jt = J[sg_at(i, c, sub)], j = jt >> 2, tj = jt & 3;
+        float r = e.w, fc = 0.5f * (__cosf(kpi * r) + 1.0f), dfc = -0.5f * kpi * __sinf(kpi * r);
+        const float* gi = dXn + (size_t)i * din + tj * SG_NB;
+        const float* gj = dXn + (size_t)j * din + ti * SG_NB;
+        const float* si = xstd + tj * SG_NB;
+        const float* sj = xstd + ti * SG_NB;
+        float dEdr = 0.0f;
 #pragma unroll 8
-      for (int b = 0; b < SG_NB; b++) {
-        float u = r - (sp.mu0 + b * sp.dmu), g = __expf(-sp.eta * u * u);
-        float dG = g * (dfc - 2.0f * sp.eta * u * fc);
-        dEdr += (gi[b] / si[b] + gj[b] / sj[b]) * dG;
+        for (int b = 0; b < SG_NB; b++) {
+          float u = r - (sp.mu0 + b * sp.dmu), g = __expf(-sp.eta * u * u);
+          dEdr += (gi[b] / si[b] + gj[b] / sj[b]) * (g * (dfc - 2.0f * sp.eta * u * fc));
+        }
+        float k = -dEdr / r;
+        fx += (double)(k * e.x);
+        fy += (double)(k * e.y);
+        fz += (double)(k * e.z);
+        w[0] += -0.5 * (double)(dEdr * r);
       }
-      float c = -dEdr / r;
-      fx += (double)(c * dx);
-      fy += (double)(c * dy);
-      fz += (double)(c * dz);
-      w[0] += -0.5 * (double)(dEdr * r);
     }
+  }
+#pragma unroll
+  for (int off = TPA / 2; off > 0; off >>= 1) {
+    fx += __shfl_xor(fx, off, 64);
+    fy += __shfl_xor(fy, off, 64);
+    fz += __shfl_xor(fz, off, 64);
+  }
+  if (i < N && sub == 0) {
     F[i] += fx;
     F[Npad + i] += fy;
     F[2 * Npad + i] += fz;
@@ -387,35 +443,59 @@ static void ensure_work(EmDeeSurrogate* s, long rows) {
   s->X.alloc(pad * din); s->dX.alloc(pad * din); s->dXsum.alloc(pad * din);
   for (DBuf<float>* b : { &s->Z1, &s->H1, &s->Z2, &s->H2, &s->G2, &s->G1 }) b->alloc(pad * H);
   s->e.alloc(pad * 2);
-  s->red.alloc(pad / 64 + 64);   // >= one partial per 256-row block
-  s->part.alloc((size_t)(pad / 1024 + 2) * H * H);
+  s->red.alloc(pad / 4 + 64);    // >= one partial per block (four rows per block in k_sg_output)
+  s->part.alloc((size_t)(pad / 256 + 2) * H * H);
   HIP_CHECK(hipMemsetAsync(s->X.p, 0, sizeof(float) * pad * din, s->stream));
   for (DBuf<float>* b : { &s->Z1, &s->H1, &s->Z2, &s->H2, &s->G2, &s->G1 })
     HIP_CHECK(hipMemsetAsync(b->p, 0, sizeof(float) * pad * H, s->stream));
   s->workRows = pad;
 }
 
-static void launch_desc(EmDeeSurrogate* s, Device* d, float* D, double L, hipStream_t st) {
-  int g = div_up(d->N, BLOCK);
-  switch (s->sp.nt) {
-    case 1: hipLaunchKernelGGL(k_sg_desc<1>, dim3(g), dim3(BLOCK), 0, st, D, d->R.p, d->type.p, d->nbr.p, d->nnb.p, s->sp, L, d->N, d->Npad); break;
-    case 2: hipLaunchKernelGGL(k_sg_desc<2>, dim3(g), dim3(BLOCK), 0, st, D, d->R.p, d->type.p, d->nbr.p, d->nnb.p, s->sp, L, d->N, d->Npad); break;
-    case 3: hipLaunchKernelGGL(k_sg_desc<3>, dim3(g), dim3(BLOCK), 0, st, D, d->R.p, d->type.p, d->nbr.p, d->nnb.p, s->sp, L, d->N, d->Npad); break;
-    default: hipLaunchKernelGGL(k_sg_desc<4>, dim3(g), dim3(BLOCK), 0, st, D, d->R.p, d->type.p, d->nbr.p, d->nnb.p, s->sp, L, d->N, d->Npad); break;
-  }
+static void build_list(EmDeeSurrogate* s, Device* d, double L, hipStream_t st) {
+  int N = d->N;
+  s->lE.alloc((size_t)N * SG_SUBS * SG_SEG);
+  s->lJ.alloc((size_t)N * SG_SUBS * SG_SEG);
+  s->lN.alloc((size_t)N * SG_SUBS);
+  s->lOver.alloc(1);
+  HIP_CHECK(hipMemsetAsync(s->lOver.p, 0, sizeof(int), st));
+  hipLaunchKernelGGL(k_sg_list, dim3(div_up((long)d->Npad * SG_SUBS, BLOCK)), dim3(BLOCK), 0, st, s->lE.p, s->lJ.p, s->lN.p,
+                     s->lOver.p, d->R.p, d->nbr.p, d->nnb.p, d->type.p, s->sp.rc, L, N, d->Npad);
+  launch_check("surrogate list");
+  int over = 0;
+  HIP_CHECK(hipMemcpyAsync(&over, s->lOver.p, sizeof(int), hipMemcpyDeviceToHost, st));
+  HIP_CHECK(hipStreamSynchronize(st));
+  if (over) fatal("surrogate", "more neighbours within the descriptor cutoff than the compact list holds");
+}
+
+static void launch_desc(EmDeeSurrogate* s, Device* d, float* D, double L, hipStream_t st, bool normalised) {
+  build_list(s, d, L, st);
+  int g = div_up((long)d->N * SG_NB, BLOCK);
+  const float* m = normalised ? s->xmean.p : nullptr;
+  const float* sd = normalised ? s->xstd.p : nullptr;
+#define SG_DESC(NT) hipLaunchKernelGGL(k_sg_desc2<NT>, dim3(g), dim3(BLOCK), 0, st, D, s->lE.p, s->lJ.p, s->lN.p, d->type.p, m, sd, s->sp, d->N)
+  switch (s->sp.nt) { case 1: SG_DESC(1); break; case 2: SG_DESC(2); break; case 3: SG_DESC(3); break; default: SG_DESC(4); }
+#undef SG_DESC
   launch_check("surrogate descriptors");
 }
 
-static void gemm(EmDeeSurrogate* s, int K, int NC, bool act, const float* A, const unsigned short* Bt, const float* bias,
-                 float* Z, float* H, long padRows) {
-  dim3 grid((unsigned)(padRows / SG_ROWS));
-#define SG_GEMM(KK, NN, AA) hipLaunchKernelGGL((k_sg_gemm<KK, NN, AA>), grid, dim3(256), 0, s->stream, A, Bt, bias, Z, H)
-  if (K == 64 && NC == 128) { if (act) SG_GEMM(64, 128, true); else SG_GEMM(64, 128, false); }
-  else if (K == 128 && NC == 128) { if (act) SG_GEMM(128, 128, true); else SG_GEMM(128, 128, false); }
-  else if (K == 128 && NC == 64) SG_GEMM(128, 64, false);
-  else if (K == 32 && NC == 128) { if (act) SG_GEMM(32, 128, true); else SG_GEMM(32, 128, false); }
-  else if (K == 128 && NC == 32) SG_GEMM(128, 32, false);
+static void gemm(EmDeeSurrogate* s, int K, int NC, int epi, const float* A, const unsigned short* Bt, const float* bias,
+                 float* Z, float* H, const float* aux, long padRows) {
+  dim3 grid((unsigned)(padRows / 16));
+#define SG_GEMM(KK, NN, EE) hipLaunchKernelGGL((k_sg_gemm<KK, NN, EE>), grid, dim3(256), 0, s->stream, A, Bt, bias, Z, H, aux)
+#define SG_SHAPE(KK, NN)                                                                            \
+  do {                                                                                              \
+    if (epi == EPI_Z) SG_GEMM(KK, NN, EPI_Z); else if (epi == EPI_SILU) SG_GEMM(KK, NN, EPI_SILU); \
+    else if (epi == EPI_DSILU) SG_GEMM(KK, NN, EPI_DSILU); else SG_GEMM(KK, NN, EPI_ADD);          \
+  } while (0)
+  if (K == 64 && NC == 128) SG_SHAPE(64, 128);
+  else if (K == 128 && NC == 128) SG_SHAPE(128, 128);
+  else if (K == 128 && NC == 64) SG_SHAPE(128, 64);
+  else if (K == 32 && NC == 128) SG_SHAPE(32, 128);
+  else if (K == 128 && NC == 32) SG_SHAPE(128, 32);
+  else if (K == 96 && NC == 128) SG_SHAPE(96, 128);
+  else if (K == 128 && NC == 96) SG_SHAPE(128, 96);
   else fatal("surrogate", "unsupported layer shape");
+#undef SG_SHAPE
 #undef SG_GEMM
   launch_check("surrogate gemm");
 }
@@ -432,22 +512,22 @@ static double mlp_pass(EmDeeSurrogate* s, int spc, long rows, const int* species
   const unsigned short* w1 = s->w1.p + (size_t)spc * din * H;
   const unsigned short* w2t = s->w2t.p + (size_t)spc * H * H;
   const unsigned short* w2 = s->w2.p + (size_t)spc * H * H;
-  gemm(s, din, H, true, s->X.p, w1t, p + pl.b1(), s->Z1.p, s->H1.p, pad);
-  gemm(s, H, H, true, s->H1.p, w2t, p + pl.b2(), s->Z2.p, s->H2.p, pad);
-  int gr = div_up(rows, BLOCK);
+  gemm(s, din, H, EPI_SILU, s->X.p, w1t, p + pl.b1(), s->Z1.p, s->H1.p, nullptr, pad);
+  gemm(s, H, H, EPI_SILU, s->H1.p, w2t, p + pl.b2(), s->Z2.p, s->H2.p, nullptr, pad);
+  int gr = div_up(rows, 4);
   float* seed = s->e.p + pad;   // second half of e holds the per-row seeds
   hipLaunchKernelGGL(k_sg_output, dim3(gr), dim3(BLOCK), 0, s->stream, s->G2.p, eatom, seed, s->red.p, s->H2.p, s->Z2.p, p,
                      species, y, spc, s->ymean[spc], s->ystd[spc], invCount, rows, mode, pl);
   launch_check("surrogate output");
   // G1 = (G2 W2^T) * SiLU'(Z1)
-  gemm(s, H, H, false, s->G2.p, w2, nullptr, s->G1.p, nullptr, pad);
-  hipLaunchKernelGGL(k_sg_dsilu, dim3(div_up(pad * H, BLOCK)), dim3(BLOCK), 0, s->stream, s->G1.p, s->Z1.p, pad * H);
+  gemm(s, H, H, EPI_DSILU, s->G2.p, w2, nullptr, s->G1.p, nullptr, s->Z1.p, pad);
   if (mode == 0) {
-    gemm(s, H, din, false, s->G1.p, w1, nullptr, s->dX.p, nullptr, pad);
+    // dE/dX of this species' rows, added to the other species' (first pass stores)
+    gemm(s, H, din, spc == 0 ? EPI_Z : EPI_ADD, s->G1.p, w1, nullptr, s->dXsum.p, nullptr, nullptr, pad);
   } else {
     // weight gradients of this species into grads
     float* g = s->grads.p + (size_t)spc * s->P;
-    int rpb = 1024, nb = (int)((rows + rpb - 1) / rpb);
+    int rpb = 256, nb = (int)((rows + rpb - 1) / rpb);
 #define SG_WGRAD(KK, NN, AP, GP, DST)                                                                          \
   do {                                                                                                        \
     hipLaunchKernelGGL((k_sg_wgrad<KK, NN>), dim3(nb), dim3(256), 0, s->stream, s->part.p, AP, GP, rows, rpb); \
@@ -509,28 +589,22 @@ double dev_surrogate_apply(Device* d, EmDeeSurrogate* s, int layer, double L, do
   ensure_work(s, N);
   long pad = s->workRows;
   int din = s->sp.din;
-  // descriptors into dX (scratch), then normalised into X
-  launch_desc(s, d, s->dX.p, L, s->stream);
-  hipLaunchKernelGGL(k_sg_normalise, dim3(div_up(pad * din, BLOCK)), dim3(BLOCK), 0, s->stream, s->X.p, s->dX.p,
-                     s->xmean.p, s->xstd.p, (long)N, din, pad);
+  // normalised descriptors straight into X (rows past N stay zero)
+  launch_desc(s, d, s->X.p, L, s->stream, true);
   double E = 0;
   float* eatom = s->e.p;
-  for (int t = 0; t < s->sp.nt; t++) {
-    E += mlp_pass(s, t, N, d->type.p, nullptr, 0.0f, 0, eatom);
-    hipLaunchKernelGGL(k_sg_accumulate, dim3(div_up(pad * din, BLOCK)), dim3(BLOCK), 0, s->stream, s->dXsum.p, s->dX.p,
-                       pad * din, t == 0 ? 1 : 0);
-  }
-  int g = div_up(N, BLOCK);
-#define SG_FORCE(NT)                                                                                              \
-  hipLaunchKernelGGL(k_sg_force<NT>, dim3(g), dim3(BLOCK), 0, s->stream, F, s->red.p, s->dXsum.p, s->xstd.p, d->R.p, \
-                     d->type.p, d->nbr.p, d->nnb.p, s->sp, L, N, d->Npad)
+  for (int t = 0; t < s->sp.nt; t++) E += mlp_pass(s, t, N, d->type.p, nullptr, 0.0f, 0, eatom);
+  int g = div_up((long)N * 16, BLOCK);
+#define SG_FORCE(NT)                                                                                                \
+  hipLaunchKernelGGL(k_sg_force2<NT>, dim3(g), dim3(BLOCK), 0, s->stream, F, s->red.p, s->dXsum.p, s->xstd.p, s->lE.p, \
+                     s->lJ.p, s->lN.p, d->type.p, s->sp, N, d->Npad)
   switch (s->sp.nt) { case 1: SG_FORCE(1); break; case 2: SG_FORCE(2); break; case 3: SG_FORCE(3); break; default: SG_FORCE(4); }
 #undef SG_FORCE
   launch_check("surrogate forces");
   hipLaunchKernelGGL(k_reduce_partials<1>, dim3(1), dim3(BLOCK), 0, s->stream, s->Fw.p + 1, s->red.p, g);
   HIP_CHECK(hipMemcpyAsync(W, s->Fw.p + 1, sizeof(double), hipMemcpyDeviceToHost, s->stream));
   if (d->wantAtomE)
-    hipLaunchKernelGGL(k_sg_add_atom, dim3(g), dim3(BLOCK), 0, s->stream, d->eatom.p, eatom, N);
+    hipLaunchKernelGGL(k_sg_add_atom, dim3(div_up(N, BLOCK)), dim3(BLOCK), 0, s->stream, d->eatom.p, eatom, N);
   HIP_CHECK(hipStreamSynchronize(s->stream));
   return E;
 }
@@ -559,7 +633,7 @@ void dev_surrogate_add_sample(Device* d, EmDeeSurrogate* s, double L, const doub
     s->Xs = nx; s->Ys = ny; s->Ss = ns;
     s->caprows = cap;
   }
-  launch_desc(s, d, s->Xs.p + s->nrows * din, L, d->stream);
+  launch_desc(s, d, s->Xs.p + s->nrows * din, L, d->stream, false);
   std::vector<float> y(N);
   for (int i = 0; i < N; i++) y[i] = (float)labelsSlot[i];
   HIP_CHECK(hipMemcpyAsync(s->Ys.p + s->nrows, y.data(), sizeof(float) * N, hipMemcpyHostToDevice, d->stream));
@@ -732,6 +806,7 @@ void EmDeeSurrogate_destroy(EmDeeSurrogate* s) {
                           &s->H1, &s->Z2, &s->H2, &s->G2, &s->G1, &s->dX, &s->e, &s->dXsum, &s->part })
     b->release();
   s->w1t.release(); s->w1.release(); s->w2t.release(); s->w2.release(); s->Ss.release();
+  s->lE.release(); s->lJ.release(); s->lN.release(); s->lOver.release();
   s->Fw.release(); s->red.release();
   (void)hipStreamDestroy(s->own);
   delete s;
