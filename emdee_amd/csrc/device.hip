@@ -272,22 +272,111 @@ __global__ void k_permute_n(double* __restrict__ dst, const double* __restrict__
   for (int c = 0; c < ncomp; c++) dst[(size_t)c * Npad + s] = src[(size_t)c * Npad + o];
 }
 
+// ---- mixed form: FP32 pair arithmetic, FP64 accumulation -----------------
+// Positions are also kept as 32-bit fixed point of the scaled coordinate,
+// u = round(2^32 (s - floor(s))) mod 2^32.  The wrapped integer difference
+// (int)(u_i - u_j) is the minimum-image separation in units of L/2^32, so a
+// neighbor costs one 16-byte gather {ux, uy, uz, type} and three integer
+// subtractions, with no rounding-to-image step.  Converted to FP32 it carries
+// a relative error below 1e-7.  Pairs whose FP32 r^2 falls within 2e-6 of the
+// cutoff are decided by the FP64 test of the plain form on the FP64 scaled
+// positions, so both forms keep exactly the same set of interacting pairs.
+// The band is 2e-6 (FP32 rounding) plus the quantisation of the coordinates
+// (half a unit each, at most 2^-32 of L per component of a separation):
+// |r^2 - r_q^2| <= 2 sqrt(3) 2^-32 r + 3 2^-64, relative to Rc^2 below
+// 4 2^-32 / (Rc/L) + 3 2^-64 / (Rc/L)^2 (band_rel).  The Verlet list build
+// uses the same fixed-point test with the same band.
+// Pair forces are summed in FP32 over groups of four neighbors, and each
+// group is added to FP64 accumulators.
+struct MixedCut {
+  float K;          // (2^32 / L)^2: invR2 (length^-2) = K / r2 (fixed-point units)
+  float lo, hi;     // FP32 band around the cutoff in fixed-point units^2
+  double Rc2s;      // FP64 cutoff on scaled coordinates (the plain form's)
+  double unit;      // L / 2^32
+};
+
+__global__ void k_scale4q(double4* __restrict__ Rs4, int4* __restrict__ Q4, float* __restrict__ qf,
+                          const double* __restrict__ R, const int* __restrict__ type, const double* __restrict__ charge,
+                          const int* __restrict__ charged, double L, int N, int Npad) {
+  int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= N) return;
+  double v[3] = { R[s] / L, R[Npad + s] / L, R[2 * Npad + s] / L };
+  unsigned int u[3];
+  for (int x = 0; x < 3; x++) {
+    double f = v[x] - floor(v[x]);
+    u[x] = (unsigned int)(unsigned long long)llrint(f * 4294967296.0);
+  }
+  Rs4[s] = make_double4(v[0], v[1], v[2], (double)type[s]);
+  Q4[s] = make_int4((int)u[0], (int)u[1], (int)u[2], type[s]);
+  qf[s] = charged[s] ? (float)charge[s] : 0.0f;
+}
+
+// relative half-width of the FP32 band around a cutoff Rc2s (scaled units^2)
+inline double band_rel(double Rc2s) {
+  const double q = 1.0 / 4294967296.0;
+  return 2e-6 + 4.0 * q / std::sqrt(Rc2s) + 3.0 * q * q / Rc2s;
+}
+
+__device__ __noinline__ bool exact_inside(const double4* __restrict__ Rs4, int i, int j, double Rc2s) {
+  double4 a = Rs4[i], b = Rs4[j];
+  double dx = a.x - b.x, dy = a.y - b.y, dz = a.z - b.z;
+  dx = dx - rint(dx); dy = dy - rint(dy); dz = dz - rint(dz);
+  double r2;
+  {
+#pragma clang fp contract(off)
+    r2 = dx * dx + dy * dy + dz * dz;
+  }
+  return r2 < Rc2s;
+}
+
 // ---- Verlet list ------------------------------------------------------------
-// Full list: for slot i, every j != i in the 125 cells within +-2 cells with
-// r^2 < (rc+skin)^2/L^2 computed as in neighbor_lists.f90:244 (FP64,
-// minimum image by anint, no FMA contraction), that is not excluded, not in
-// the same rigid body and whose types interact (neighbor_lists.f90:250-259).
-// Slots are sorted by cell with x fastest, so for each of the 25 (dy, dz)
-// offsets the five cells x-2..x+2 form one contiguous slot range (two when
-// the row wraps); lanes of a wave (neighbouring slots) scan nearly the same
-// ranges, which keeps the gathers in L1.
+constexpr int CELL_REACH = 3;
+// Full list: for slot i, every j != i in the cells within +-CELL_REACH cells
+// (cell edge >= (rc+skin)/CELL_REACH) with r^2 < (rc+skin)^2/L^2 computed as in
+// neighbor_lists.f90:244 (FP64, minimum image by anint, no FMA contraction),
+// that is not excluded, not in the same rigid body and whose types interact
+// (neighbor_lists.f90:250-259).  The reference uses cells of edge rc+skin and
+// the 27 around; thinner cells cut the candidates per atom (about 140 for 78
+// neighbours at LJ liquid density, against 190 with edge (rc+skin)/2) while
+// the exact test keeps the same pairs.  Slots are sorted by cell with x
+// fastest, so for each (dy, dz) offset the cells x-R..x+R form one contiguous
+// slot range (two when the row wraps); lanes of a wave (neighbouring slots)
+// scan nearly the same ranges, which keeps the gathers in L1.  Candidates are loaded four at a
+// time so that the loads of a lane overlap; they are still tested and
+// appended in ascending slot order.
+// SIMPLE: no rigid bodies, no exclusions and every type pair interacts, so
+// the distance test alone decides (free atoms have distinct body ids); it is
+// made on the 16-byte fixed-point positions, and only candidates within the
+// FP32 band around the cutoff take the FP64 test on the scaled positions.
+template <bool SIMPLE>
 __device__ __forceinline__ void scan_range(int a, int b, int i, double xi, double yi, double zi, int ti, int bi, int oi,
                                            const double4* __restrict__ Rs4, const int* __restrict__ body,
                                            const int* __restrict__ perm, const uint8_t* __restrict__ interact,
                                            const int* __restrict__ exItem, int e0, int e1, int ntypes, double xRc2,
-                                           int cap, int Npad, int* __restrict__ nbr, int& cnt) {
-  for (int j = a; j < b; j++) {
-    double4 rj = Rs4[j];
+                                           int cap, int Npad, int* __restrict__ nbr, int& cnt,
+                                           const int4* __restrict__ Q4, int4 qi, float lo, float hi) {
+  if (SIMPLE) {
+    auto testq = [&](int j, int4 qj) {
+      float dx = (float)(int)((unsigned int)qi.x - (unsigned int)qj.x);
+      float dy = (float)(int)((unsigned int)qi.y - (unsigned int)qj.y);
+      float dz = (float)(int)((unsigned int)qi.z - (unsigned int)qj.z);
+      float r2 = dx * dx + dy * dy + dz * dz;
+      bool in = r2 < lo;
+      if (!in && r2 < hi) in = exact_inside(Rs4, i, j, xRc2);
+      if (in && j != i) {
+        if (cnt < cap) nbr[(size_t)cnt * Npad + i] = j;
+        cnt++;
+      }
+    };
+    int j = a;
+    for (; j + 3 < b; j += 4) {
+      int4 q0 = Q4[j], q1 = Q4[j + 1], q2 = Q4[j + 2], q3 = Q4[j + 3];
+      testq(j, q0); testq(j + 1, q1); testq(j + 2, q2); testq(j + 3, q3);
+    }
+    for (; j < b; j++) testq(j, Q4[j]);
+    return;
+  }
+  auto test = [&](int j, double4 rj) {
     double r2;
     {
 #pragma clang fp contract(off)
@@ -296,25 +385,33 @@ __device__ __forceinline__ void scan_range(int a, int b, int i, double xi, doubl
       r2 = d0 * d0 + d1 * d1 + d2 * d2;
     }
     if (r2 < xRc2 && j != i) {
-      if (body[j] == bi) continue;
-      if (!interact[(int)rj.w * ntypes + ti]) continue;
-      if (e1 > e0) {
-        int oj = perm[j];
-        bool ex = false;
-        for (int e = e0; e < e1; e++) if (exItem[e] == oj) { ex = true; break; }
-        if (ex) continue;
+      if (!SIMPLE) {
+        if (body[j] == bi) return;
+        if (!interact[(int)rj.w * ntypes + ti]) return;
+        if (e1 > e0) {
+          int oj = perm[j];
+          for (int e = e0; e < e1; e++) if (exItem[e] == oj) return;
+        }
       }
       if (cnt < cap) nbr[(size_t)cnt * Npad + i] = j;
       cnt++;
     }
+  };
+  int j = a;
+  for (; j + 3 < b; j += 4) {
+    double4 r0 = Rs4[j], r1 = Rs4[j + 1], r2 = Rs4[j + 2], r3 = Rs4[j + 3];
+    test(j, r0); test(j + 1, r1); test(j + 2, r2); test(j + 3, r3);
   }
+  for (; j < b; j++) test(j, Rs4[j]);
 }
+template <bool SIMPLE>
 __global__ void __launch_bounds__(BLOCK)
 k_build_list(int* __restrict__ nbr, int* __restrict__ nnb, int* __restrict__ flags, double* __restrict__ part,
              const double4* __restrict__ Rs4, const int* __restrict__ key, const int* __restrict__ start,
              const int* __restrict__ body, const int* __restrict__ perm,
              const uint8_t* __restrict__ interact, const int* __restrict__ exOff, const int* __restrict__ exItem,
-             int nexcl, int ntypes, int M, double xRc2, int cap, int N, int Npad) {
+             int nexcl, int ntypes, int M, double xRc2, int cap, int N, int Npad, const int4* __restrict__ Q4,
+             float lo, float hi) {
   int blk = xcd_block();
   int i = blk * BLOCK + threadIdx.x;
   int cnt = 0;
@@ -323,6 +420,7 @@ k_build_list(int* __restrict__ nbr, int* __restrict__ nnb, int* __restrict__ fla
     int c = key[i];
     int cx = c % M, cy = (c / M) % M, cz = c / (M * M);
     int ti = (int)ri.w, bi = body[i], oi = perm[i];
+    int4 qi = SIMPLE ? Q4[i] : int4{ 0, 0, 0, 0 };
     int e0 = 0, e1 = 0;
     if (nexcl) { e0 = exOff[oi]; e1 = exOff[oi + 1]; }
     // Conservative culling: with f the fractional position of atom i inside
@@ -338,29 +436,43 @@ k_build_list(int* __restrict__ nbr, int* __restrict__ nnb, int* __restrict__ fla
     };
     double fx = frac(ri.x, cx), fy = frac(ri.y, cy), fz = frac(ri.z, cz);
     double rcell2 = xRc2 * (double)M * (double)M * (1.0 + 1e-6) + 1e-9;
-    auto scan_cells = [&](int base, int c0, int c1) {   // cells c0..c1 of one row, 0 <= c0 <= c1 < M
-      scan_range(start[base + c0], start[base + c1 + 1], i, ri.x, ri.y, ri.z, ti, bi, oi, Rs4, body, perm, interact,
-                 exItem, e0, e1, ntypes, xRc2, cap, Npad, nbr, cnt);
-    };
-    for (int dz = -2; dz <= 2; dz++) {
-      double gz = dz > 0 ? dz - fz : (dz < 0 ? fz - dz - 1 : 0.0);
-      gz = fmax(gz, 0.0);
+    // slot ranges of row r of the (dy, dz) offsets: cells xa..xb of the row,
+    // split in two where the row wraps; empty when the row is culled
+    struct Ranges { int a0, b0, a1, b1; };
+    constexpr int W = 2 * CELL_REACH + 1;
+    auto row = [&](int r) {
+      Ranges q = { 0, 0, 0, 0 };
+      int dz = r / W - CELL_REACH, dy = r % W - CELL_REACH;
+      double gz = fmax(dz > 0 ? dz - fz : (dz < 0 ? fz - dz - 1 : 0.0), 0.0);
+      double gy = fmax(dy > 0 ? dy - fy : (dy < 0 ? fy - dy - 1 : 0.0), 0.0);
+      double rem = rcell2 - gy * gy - gz * gz;
+      if (rem <= 0.0) return q;
+      double e = sqrt(rem) + 1e-6;
+      int xa = cx + max((int)floor(fx - e), -CELL_REACH), xb = cx + min((int)floor(fx + e), CELL_REACH);
+      int y = cy + dy; y = y < 0 ? y + M : (y >= M ? y - M : y);
       int z = cz + dz; z = z < 0 ? z + M : (z >= M ? z - M : z);
-      for (int dy = -2; dy <= 2; dy++) {
-        double gy = dy > 0 ? dy - fy : (dy < 0 ? fy - dy - 1 : 0.0);
-        gy = fmax(gy, 0.0);
-        double rem = rcell2 - gy * gy - gz * gz;
-        if (rem <= 0.0) continue;
-        double e = sqrt(rem) + 1e-6;
-        int xa = cx + max((int)floor(fx - e), -2), xb = cx + min((int)floor(fx + e), 2);
-        int y = cy + dy; y = y < 0 ? y + M : (y >= M ? y - M : y);
-        int base = M * (y + M * z);
-        if (xa >= 0 && xb < M) scan_cells(base, xa, xb);
-        else if (xa < 0 && xb >= 0) { scan_cells(base, xa + M, M - 1); scan_cells(base, 0, xb); }
-        else if (xa < 0) scan_cells(base, xa + M, xb + M);
-        else if (xa < M) { scan_cells(base, xa, M - 1); scan_cells(base, 0, xb - M); }
-        else scan_cells(base, xa - M, xb - M);
-      }
+      int base = M * (y + M * z);
+      int c0 = xa, c1 = xb, d0 = 0, d1 = -1;
+      if (xa < 0 && xb >= 0) { c0 = xa + M; c1 = M - 1; d1 = xb; }
+      else if (xa < 0) { c0 = xa + M; c1 = xb + M; }
+      else if (xb >= M && xa < M) { c1 = M - 1; d1 = xb - M; }
+      else if (xa >= M) { c0 = xa - M; c1 = xb - M; }
+      q.a0 = start[base + c0]; q.b0 = start[base + c1 + 1];
+      if (d1 >= d0) { q.a1 = start[base + d0]; q.b1 = start[base + d1 + 1]; }
+      return q;
+    };
+    auto scan = [&](int a, int b) {
+      scan_range<SIMPLE>(a, b, i, ri.x, ri.y, ri.z, ti, bi, oi, Rs4, body, perm, interact, exItem, e0, e1, ntypes, xRc2,
+                         cap, Npad, nbr, cnt, Q4, qi, lo, hi);
+    };
+    // the cell starts of the next row are loaded before the current row is
+    // scanned, so their latency overlaps the scan
+    Ranges cur = row(0);
+    for (int r = 0; r < W * W; r++) {
+      Ranges nxt = r + 1 < W * W ? row(r + 1) : Ranges{ 0, 0, 0, 0 };
+      scan(cur.a0, cur.b0);
+      scan(cur.a1, cur.b1);
+      cur = nxt;
     }
     nnb[i] = cnt;
     // pad the list to a multiple of 4 with the atom itself (zero separation):
@@ -546,51 +658,7 @@ k_pairs(double* __restrict__ F, double* __restrict__ partials,
   }
 }
 
-// ---- mixed form: FP32 pair arithmetic, FP64 accumulation -----------------
-// Positions are also kept as 32-bit fixed point of the scaled coordinate,
-// u = round(2^32 (s - floor(s))) mod 2^32.  The wrapped integer difference
-// (int)(u_i - u_j) is the minimum-image separation in units of L/2^32, so a
-// neighbor costs one 16-byte gather {ux, uy, uz, type} and three integer
-// subtractions, with no rounding-to-image step.  Converted to FP32 it carries
-// a relative error below 1e-7.  Pairs whose FP32 r^2 falls within 2e-6 of the
-// cutoff are decided by the FP64 test of the plain form on the FP64 scaled
-// positions, so both forms keep exactly the same set of interacting pairs.
-// Pair forces are summed in FP32 over groups of four neighbors, and each
-// group is added to FP64 accumulators.
-struct MixedCut {
-  float K;          // (2^32 / L)^2: invR2 (length^-2) = K / r2 (fixed-point units)
-  float lo, hi;     // FP32 band around the cutoff in fixed-point units^2
-  double Rc2s;      // FP64 cutoff on scaled coordinates (the plain form's)
-  double unit;      // L / 2^32
-};
-
-__global__ void k_scale4q(double4* __restrict__ Rs4, int4* __restrict__ Q4, float* __restrict__ qf,
-                          const double* __restrict__ R, const int* __restrict__ type, const double* __restrict__ charge,
-                          const int* __restrict__ charged, double L, int N, int Npad) {
-  int s = blockIdx.x * blockDim.x + threadIdx.x;
-  if (s >= N) return;
-  double v[3] = { R[s] / L, R[Npad + s] / L, R[2 * Npad + s] / L };
-  unsigned int u[3];
-  for (int x = 0; x < 3; x++) {
-    double f = v[x] - floor(v[x]);
-    u[x] = (unsigned int)(unsigned long long)llrint(f * 4294967296.0);
-  }
-  Rs4[s] = make_double4(v[0], v[1], v[2], (double)type[s]);
-  Q4[s] = make_int4((int)u[0], (int)u[1], (int)u[2], type[s]);
-  qf[s] = charged[s] ? (float)charge[s] : 0.0f;
-}
-
-__device__ __noinline__ bool exact_inside(const double4* __restrict__ Rs4, int i, int j, double Rc2s) {
-  double4 a = Rs4[i], b = Rs4[j];
-  double dx = a.x - b.x, dy = a.y - b.y, dz = a.z - b.z;
-  dx = dx - rint(dx); dy = dy - rint(dy); dz = dz - rint(dz);
-  double r2;
-  {
-#pragma clang fp contract(off)
-    r2 = dx * dx + dy * dy + dz * dz;
-  }
-  return r2 < Rc2s;
-}
+// ---- mixed form pair kernels (fixed-point helpers are above the list build) --
 
 struct MixedAcc { float fx, fy, fz, Ep, Ec, Wp, Wc; };
 
@@ -995,6 +1063,7 @@ void dev_set_exclusions(Device* d, const std::vector<int>& offset, const std::ve
 }
 
 void dev_set_interact(Device* d, const std::vector<uint8_t>& table) {
+  d->allInteract = std::all_of(table.begin(), table.end(), [](uint8_t v) { return v != 0; });
   HIP_CHECK(hipMemcpyAsync(d->interact.p, table.data(), table.size(), hipMemcpyHostToDevice, d->stream));
   HIP_CHECK(hipStreamSynchronize(d->stream));
 }
@@ -1084,8 +1153,8 @@ void dev_rebuild(Device* d, double L, double xRc, double xInRc) {
   int N = d->N, Np = d->Npad;
   if (N == 0) return;
   int g = div_up(N, BLOCK);
-  int M = (int)std::floor(2 * L / xRc);
-  if (M < 5) M = 5;
+  int M = (int)std::floor(CELL_REACH * L / xRc);
+  if (M < 2 * CELL_REACH + 1) M = 2 * CELL_REACH + 1;   // every cell of a row at most once
   if ((long)M * M * M > 64l * 1024 * 1024) M = 400;   // bound the cell table
   ensure_cells(d, M);
   int nc = d->ncells, nc1 = nc + 1, gc = div_up(nc1, BLOCK);
@@ -1119,13 +1188,19 @@ void dev_rebuild(Device* d, double L, double xRc, double xInRc) {
   permI(d->type); permI(d->body); permI(d->perm); permI(d->freeFlag); permI(d->chargedFlag);
   hipLaunchKernelGGL(k_invert_perm, dim3(g), dim3(BLOCK), 0, d->stream, d->iperm.p, d->perm.p, N);
   launch_check("permute");
-  // scaled positions and keys in the new order (Rs4 is a function of R, type)
-  hipLaunchKernelGGL(k_scale4, dim3(g), dim3(BLOCK), 0, d->stream, d->Rs4.p, d->R.p, d->type.p, L, N, Np);
+  // scaled and fixed-point positions and keys in the new order (functions of
+  // R, type and charge)
+  d->Q4.alloc(Np);
+  d->qf.alloc(Np);
+  hipLaunchKernelGGL(k_scale4q, dim3(g), dim3(BLOCK), 0, d->stream, d->Rs4.p, d->Q4.p, d->qf.p, d->R.p, d->type.p,
+                     d->charge.p, d->chargedFlag.p, L, N, Np);
   hipLaunchKernelGGL(k_cell_key, dim3(g), dim3(BLOCK), 0, d->stream, d->key.p, d->cellCount.p, d->Rs4.p, M, N);
   // R0 = R
   HIP_CHECK(hipMemcpyAsync(d->R0.p, d->R.p, 3 * sizeof(double) * Np, hipMemcpyDeviceToDevice, d->stream));
   // Verlet list, growing the ELL capacity on overflow
   double xRc2 = (xRc * xRc) / (L * L);
+  const double two32 = 4294967296.0, rc2u = xRc2 * two32 * two32, band = band_rel(xRc2);
+  float lo = (float)(rc2u * (1.0 - band)), hi = (float)(rc2u * (1.0 + band));
   if (d->cap == 0) {
     double vol = 4.18879020478639 * xRc * xRc * xRc * ((double)N / (L * L * L));
     d->cap = std::max(32, (int)(1.35 * vol + 16));
@@ -1135,9 +1210,10 @@ void dev_rebuild(Device* d, double L, double xRc, double xInRc) {
     d->nbr.alloc((size_t)d->cap * Np);
     HIP_CHECK(hipMemsetAsync(d->flags.p, 0, sizeof(int) * 4, d->stream));
     if (d->profiling) HIP_CHECK(hipEventRecord(d->ev0, d->stream));
-    hipLaunchKernelGGL(k_build_list, dim3(g), dim3(BLOCK), 0, d->stream, d->nbr.p, d->nnb.p, d->flags.p, d->partials.p,
+    bool simple = d->bodies.nb == 0 && d->nexcl == 0 && d->allInteract;
+    hipLaunchKernelGGL(simple ? k_build_list<true> : k_build_list<false>, dim3(g), dim3(BLOCK), 0, d->stream, d->nbr.p, d->nnb.p, d->flags.p, d->partials.p,
                        d->Rs4.p, d->key.p, d->cellStart.p, d->body.p, d->perm.p, d->interact.p, d->exOff.p, d->exItem.p,
-                       d->nexcl, d->ntypes, M, xRc2, d->cap, N, Np);
+                       d->nexcl, d->ntypes, M, xRc2, d->cap, N, Np, d->Q4.p, lo, hi);
     launch_check("build_list");
     if (d->profiling) HIP_CHECK(hipEventRecord(d->ev1, d->stream));
     hipLaunchKernelGGL(k_reduce_partials<1>, dim3(1), dim3(BLOCK), 0, d->stream, d->result.p + 8, d->partials.p, g);
@@ -1209,8 +1285,8 @@ void dev_compute_pairs(Device* d, const PairCall& c, ForceSums* out) {
     MixedCut mc;
     double two32 = 4294967296.0, rc2u = c.Rc2 * two32 * two32;
     mc.K = (float)(two32 * two32 / (c.L * c.L));
-    mc.lo = (float)(rc2u * (1.0 - 2e-6));
-    mc.hi = (float)(rc2u * (1.0 + 2e-6));
+    mc.lo = (float)(rc2u * (1.0 - band_rel(c.Rc2)));
+    mc.hi = (float)(rc2u * (1.0 + band_rel(c.Rc2)));
     mc.Rc2s = c.Rc2;
     mc.unit = c.L / two32;
 #define LAUNCH_MX(E, U, J, T)                                                                                      \

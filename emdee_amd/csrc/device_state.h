@@ -135,6 +135,7 @@ struct Device {
   DBuf<uint8_t> interact;
   DBuf<int> exOff, exItem;
   int nexcl = 0;
+  bool allInteract = false;   // every type pair has a model (interaction table all ones)
   std::vector<int> hostType;
   // reductions
   DBuf<double> partials, result;
