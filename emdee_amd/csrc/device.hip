@@ -20,6 +20,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <atomic>
 #include <climits>
 #include <cmath>
 #include <cstdio>
@@ -40,6 +41,20 @@ namespace emdee {
 void warning(const std::string& msg) { fprintf(stderr, "WARNING: %s.\n", msg.c_str()); }
 
 // --------------------------------------------------------------- kernels
+
+// Waits for the post with sequence number seq (k_reduce_post): polls the
+// coherent host word, then falls back to a stream wait, which also reports a
+// failed launch.
+static void post_wait(Device* d, unsigned long long seq) {
+  volatile unsigned long long* f = d->hseq;
+  for (int k = 0; k < (1 << 20); k++) {
+    if (*f == seq) { std::atomic_thread_fence(std::memory_order_acquire); return; }
+    __builtin_ia32_pause();
+  }
+  HIP_CHECK(hipStreamSynchronize(d->stream));
+  if (*f != seq) fatal("device", "a posted result did not arrive");
+  std::atomic_thread_fence(std::memory_order_acquire);
+}
 
 __global__ void k_iota(int* p, int n) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -167,7 +182,9 @@ __global__ void k_disp_c(double* __restrict__ part, const double* __restrict__ d
     part[blockIdx.x] = v;
   }
 }
-__global__ void k_disp_c_reduce(double* __restrict__ trig, const double* __restrict__ part, int nb) {
+__global__ void k_disp_c_reduce(double* __restrict__ trig, const double* __restrict__ part, int nb,
+                                double* __restrict__ host, unsigned long long* __restrict__ hostSeq,
+                                unsigned long long seq) {
   __shared__ double smax[BLOCK];
   double v = 0.0;
   for (int b = threadIdx.x; b < nb; b += BLOCK) v = fmax(v, part[b]);
@@ -177,7 +194,12 @@ __global__ void k_disp_c_reduce(double* __restrict__ trig, const double* __restr
     if (threadIdx.x < h) smax[threadIdx.x] = fmax(smax[threadIdx.x], smax[threadIdx.x + h]);
     __syncthreads();
   }
-  if (threadIdx.x == 0) trig[2] = smax[0];
+  if (threadIdx.x == 0) {   // posts {max, k, next} to the host (as k_reduce_post)
+    trig[2] = smax[0];
+    host[0] = trig[0]; host[1] = trig[1]; host[2] = smax[0];
+    __threadfence_system();
+    *(volatile unsigned long long*)hostSeq = seq;
+  }
 }
 
 // ---- cell sort ------------------------------------------------------------
@@ -978,6 +1000,11 @@ Device* dev_create(const DeviceSetup& s) {
   d->stage.alloc(3 * (size_t)std::max(s.N, 1));
   HIP_CHECK(hipHostMalloc(&d->hstage, 3 * sizeof(double) * std::max(s.N, 1), hipHostMallocDefault));
   HIP_CHECK(hipHostMalloc(&d->hresult, 64 * sizeof(double), hipHostMallocDefault));
+  HIP_CHECK(hipHostMalloc((void**)&d->hpost, 64 * sizeof(double), hipHostMallocCoherent));
+  HIP_CHECK(hipHostMalloc((void**)&d->hseq, 64, hipHostMallocCoherent));
+  HIP_CHECK(hipHostGetDevicePointer((void**)&d->hpostDev, d->hpost, 0));
+  HIP_CHECK(hipHostGetDevicePointer((void**)&d->hseqDev, d->hseq, 0));
+  *d->hseq = 0;
   d->result.alloc(64);
   d->partials.alloc(64 * (size_t)(Np / BLOCK + 1));   // up to 16 threads per atom x 4 sums
   d->d2.alloc(Np);
@@ -1029,6 +1056,8 @@ void dev_destroy(Device* d) {
   dev_structs_release(d);
   if (d->hstage) (void)hipHostFree(d->hstage);
   if (d->hresult) (void)hipHostFree(d->hresult);
+  if (d->hpost) (void)hipHostFree(d->hpost);
+  if (d->hseq) (void)hipHostFree(d->hseq);
   (void)hipEventDestroy(d->ev0); (void)hipEventDestroy(d->ev1);
   (void)hipStreamDestroy(d->stream);
   delete d;
@@ -1130,11 +1159,12 @@ bool dev_needs_rebuild(Device* d, double skinSq) {
   hipLaunchKernelGGL(k_disp_a, dim3(g), dim3(BLOCK), 0, d->stream, d->d2.p, d->partials.p, d->R.p, d->R0.p, d->perm.p, N, d->Npad);
   hipLaunchKernelGGL(k_disp_a_reduce, dim3(1), dim3(BLOCK), 0, d->stream, d->trig.p, d->partials.p, g);
   hipLaunchKernelGGL(k_disp_c, dim3(g), dim3(BLOCK), 0, d->stream, d->partials.p, d->d2.p, d->perm.p, d->trig.p, N);
-  hipLaunchKernelGGL(k_disp_c_reduce, dim3(1), dim3(BLOCK), 0, d->stream, d->trig.p, d->partials.p, g);
+  unsigned long long seq = ++d->seq;
+  hipLaunchKernelGGL(k_disp_c_reduce, dim3(1), dim3(BLOCK), 0, d->stream, d->trig.p, d->partials.p, g, d->hpostDev,
+                     d->hseqDev, seq);
   launch_check("trigger");
-  HIP_CHECK(hipMemcpyAsync(d->hresult, d->trig.p, 3 * sizeof(double), hipMemcpyDeviceToHost, d->stream));
-  HIP_CHECK(hipStreamSynchronize(d->stream));
-  double maximum = d->hresult[0], next = (d->hresult[1] == 0.0) ? maximum : d->hresult[2];
+  post_wait(d, seq);
+  double maximum = d->hpost[0], next = (d->hpost[1] == 0.0) ? maximum : d->hpost[2];
   double approach = maximum + 2 * std::sqrt(maximum * next) + next;
   return approach > skinSq;
 }
@@ -1329,18 +1359,20 @@ void dev_compute_pairs(Device* d, const PairCall& c, ForceSums* out) {
   }
   launch_check("pairs");
   if (d->profiling) HIP_CHECK(hipEventRecord(d->ev1, d->stream));
-  hipLaunchKernelGGL(k_reduce_partials<4>, dim3(1), dim3(BLOCK), 0, d->stream, d->result.p, d->partials.p, gp);
+  unsigned long long seq = ++d->seq;
+  hipLaunchKernelGGL(k_reduce_post<4>, dim3(1), dim3(BLOCK), 0, d->stream, d->hpostDev, d->hseqDev, seq, d->partials.p, gp);
   launch_check("reduce");
-  HIP_CHECK(hipMemcpyAsync(d->hresult, d->result.p, 4 * sizeof(double), hipMemcpyDeviceToHost, d->stream));
-  HIP_CHECK(hipStreamSynchronize(d->stream));
+  post_wait(d, seq);
   if (d->profiling) {
-    float ms = 0; HIP_CHECK(hipEventElapsedTime(&ms, d->ev0, d->ev1));
+    float ms = 0;
+    HIP_CHECK(hipEventSynchronize(d->ev1));
+    HIP_CHECK(hipEventElapsedTime(&ms, d->ev0, d->ev1));
     d->pairMs += ms; d->pairLaunches++;
   }
-  out->Epair = d->hresult[0];
-  out->Ecoul = d->hresult[1];
-  out->Wpair = d->hresult[2];
-  out->Wcoul = d->hresult[3];
+  out->Epair = d->hpost[0];
+  out->Ecoul = d->hpost[1];
+  out->Wpair = d->hpost[2];
+  out->Wcoul = d->hpost[3];
 }
 
 void dev_boost(Device* d, int layer, double CP, double CF, bool translate, double* twoKE) {
@@ -1350,10 +1382,11 @@ void dev_boost(Device* d, int layer, double CP, double CF, bool translate, doubl
   const double* F = d->F.p + 3 * (size_t)Np * layer;
   if (twoKE) {
     hipLaunchKernelGGL(k_boost<true>, dim3(g), dim3(BLOCK), 0, d->stream, d->P.p, d->partials.p, F, d->invMass.p, d->freeFlag.p, CP, CF, (int)translate, N, Np);
-    hipLaunchKernelGGL(k_reduce_partials<3>, dim3(1), dim3(BLOCK), 0, d->stream, d->result.p, d->partials.p, g);
-    HIP_CHECK(hipMemcpyAsync(d->hresult, d->result.p, 3 * sizeof(double), hipMemcpyDeviceToHost, d->stream));
-    HIP_CHECK(hipStreamSynchronize(d->stream));
-    for (int x = 0; x < 3; x++) twoKE[x] = d->hresult[x];
+    unsigned long long seq = ++d->seq;
+    hipLaunchKernelGGL(k_reduce_post<3>, dim3(1), dim3(BLOCK), 0, d->stream, d->hpostDev, d->hseqDev, seq, d->partials.p, g);
+    launch_check("boost");
+    post_wait(d, seq);
+    for (int x = 0; x < 3; x++) twoKE[x] = d->hpost[x];
   } else if (translate) {
     hipLaunchKernelGGL(k_boost<false>, dim3(g), dim3(BLOCK), 0, d->stream, d->P.p, d->partials.p, F, d->invMass.p, d->freeFlag.p, CP, CF, 1, N, Np);
   }

@@ -77,6 +77,33 @@ __global__ void k_reduce_partials(double* __restrict__ out, const double* __rest
   }
 }
 
+// The same reduction, with the NV totals written straight into pinned,
+// coherent host memory followed by a sequence number: the host polls the
+// number (post_wait) instead of copying the totals back and waiting on the
+// stream, which saves a copy launch and the wake-up latency of a stream wait
+// on every energy, kinetic-energy and list-trigger read.  (Finishing the
+// reduction in the last block of the producing kernel instead would need a
+// device-scope release fence per block, which writes back the XCD's L2 and
+// costs more than the launch it saves.)
+template <int NV>
+__global__ void k_reduce_post(double* __restrict__ host, unsigned long long* __restrict__ hostSeq,
+                              unsigned long long seq, const double* __restrict__ partials, int nblocks) {
+  double v[NV];
+#pragma unroll
+  for (int k = 0; k < NV; k++) v[k] = 0;
+  for (int b = threadIdx.x; b < nblocks; b += BLOCK) {
+#pragma unroll
+    for (int k = 0; k < NV; k++) v[k] += partials[NV * b + k];
+  }
+  block_sum<NV>(v);
+  if (threadIdx.x == 0) {
+#pragma unroll
+    for (int k = 0; k < NV; k++) host[k] = v[k];
+    __threadfence_system();
+    *(volatile unsigned long long*)hostSeq = seq;
+  }
+}
+
 // ------------------------------------------------------------ device state
 
 template <typename T>
@@ -140,6 +167,13 @@ struct Device {
   // reductions
   DBuf<double> partials, result;
   double* hresult = nullptr;
+  // posted results (k_reduce_post): coherent pinned host memory, its device
+  // address, and the sequence number of the last post requested
+  double* hpost = nullptr;
+  double* hpostDev = nullptr;
+  unsigned long long* hseq = nullptr;
+  unsigned long long* hseqDev = nullptr;
+  unsigned long long seq = 0;
   DBuf<double> d2;
   DBuf<double> trig;      // [0] max, [1] first original index holding it, [2] max before it
   // rigid bodies; iperm: original atom -> slot
