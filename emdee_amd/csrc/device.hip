@@ -779,13 +779,7 @@ k_pairs_mx(double* __restrict__ F, double* __restrict__ partials, const int4* __
       // lists are padded to a multiple of 4 with the atom itself; its zero
       // separation marks the padding
       const int n4 = (n + 3) & ~3;
-      for (int k = 0; k < n4; k += 4) {
-        int j[4];
-        int4 q[4];
-#pragma unroll
-        for (int u = 0; u < 4; u++) j[u] = __builtin_nontemporal_load(col + (size_t)(k + u) * Npad);
-#pragma unroll
-        for (int u = 0; u < 4; u++) q[u] = Q4[j[u]];
+      auto group4 = [&](const int* j, const int4* q) {
         MixedAcc a = { 0, 0, 0, 0, 0, 0, 0 };
 #pragma unroll
         for (int u = 0; u < 4; u++) {
@@ -807,6 +801,29 @@ k_pairs_mx(double* __restrict__ F, double* __restrict__ partials, const int4* __
           a.fz += g * dz;
         }
         flush(a);
+      };
+      // two groups of four in flight per iteration; each group is still
+      // summed in FP32 and flushed to FP64 on its own
+      int k = 0;
+      for (; k + 12 <= n4; k += 12) {
+        int j[12];
+        int4 q[12];
+#pragma unroll
+        for (int u = 0; u < 12; u++) j[u] = __builtin_nontemporal_load(col + (size_t)(k + u) * Npad);
+#pragma unroll
+        for (int u = 0; u < 12; u++) q[u] = Q4[j[u]];
+        group4(j, q);
+        group4(j + 4, q + 4);
+        group4(j + 8, q + 8);
+      }
+      for (; k < n4; k += 4) {
+        int j[4];
+        int4 q[4];
+#pragma unroll
+        for (int u = 0; u < 4; u++) j[u] = __builtin_nontemporal_load(col + (size_t)(k + u) * Npad);
+#pragma unroll
+        for (int u = 0; u < 4; u++) q[u] = Q4[j[u]];
+        group4(j, q);
       }
     } else if (TPA == 1) {
       int k = 0;
