@@ -2,44 +2,19 @@
 363-412, 486-520, 929-946): two layers with different Coulomb models, an
 inner cutoff on one of them (layer_based_parameters), layer switching with
 per-layer forces, energies and virials, and dynamics on either layer."""
-import ctypes
-
 import numpy as np
 import pytest
 
+from emdee_amd import mts
 from emdee_amd import surrogate as sg
 import water
 
 pytestmark = pytest.mark.gpu
 
 
-def layered(lib, inner):
-    cfg = water.hosts.read_configuration(water.hosts.GOLDEN + "/NIST_spce_sample.mol")
-    md = lib.system(1, 2, 9.0, 1.0, cfg["N"], cfg["types"], cfg["mass"], cfg["molecule"])
-    if inner:
-        apply = (ctypes.c_int * 2)(0, 1)
-        bonded = (ctypes.c_int * 2)(1, 1)
-        lib.EmDee_layer_based_parameters(md, 6.0, apply, bonded)
-    eps = cfg["epsilon"] / water.MVV2E
-    kc = (ctypes.c_double * 2)(water.KCOUL, water.KCOUL)
-    for i in range(cfg["ntypes"]):
-        if eps[i] == 0.0:
-            models = (ctypes.c_void_p * 2)(lib.EmDee_pair_none(), lib.EmDee_pair_none())
-        else:
-            models = (ctypes.c_void_p * 2)(lib.EmDee_pair_lj_cut(eps[i], cfg["sigma"][i]),
-                                           lib.EmDee_shifted_force(lib.EmDee_pair_lj_cut(eps[i], cfg["sigma"][i])))
-        lib.EmDee_set_pair_multimodel(md, i + 1, i + 1, models, kc)
-    lib.EmDee_set_coul_multimodel(md, (ctypes.c_void_p * 2)(lib.EmDee_coul_long(), lib.EmDee_coul_damped(0.25)))
-    lib.EmDee_set_kspace_model(md, lib.EmDee_kspace_ewald(1e-5))
-    lib.upload(md, "charges", cfg["Q"])
-    lib.upload(md, "coordinates", cfg["R"])
-    lib.upload(md, "box", np.array([cfg["L"]]))
-    return md, cfg
-
-
 @pytest.mark.parametrize("inner", [False, True])
 def test_two_layers(gpu, oracle, inner):
-    (mg, cfg), (mo, _) = layered(gpu, inner), layered(oracle, inner)
+    (mg, cfg), (mo, _) = water.two_layer_fixture(gpu, inner), water.two_layer_fixture(oracle, inner)
     N = cfg["N"]
     for layer in (1, 2, 1):
         for lib, md in ((gpu, mg), (oracle, mo)):
@@ -81,3 +56,18 @@ def test_atom_energies_sum_to_the_potential(gpu):
     e = gpu.atom_energies(md, N)
     gpu.compute_forces(md)
     assert md.Energy.Potential - e.sum() == pytest.approx(c1, rel=1e-9)
+
+
+def test_respa_matches_oracle(gpu, oracle):
+    """r-RESPA over the two layers (emdee_amd.mts, 1:4) on the GPU against
+    the same reference-API calls on the oracle."""
+    runs = []
+    for lib in (gpu, oracle):
+        md, cfg = water.two_layer_fixture(lib, False)
+        lib.random_momenta(md, water.KB * 298.0, True, 7)
+        H = mts.respa(lib, md, 4, 2.0, 4)
+        runs.append((H, lib.download(md, "coordinates", (cfg["N"], 3)), md.Builds))
+    (Hg, Rg, bg), (Ho, Ro, bo) = runs
+    assert Hg == pytest.approx(Ho, rel=1e-10)
+    assert np.max(np.abs(Rg - Ro)) < 1e-8
+    assert bg == bo
