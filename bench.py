@@ -164,6 +164,53 @@ def hmc_cpu_baseline(replicas, sample, threads, seed=1):
     return float(np.sum(N[idx])) * 10 / t, t
 
 
+def water_bench(lib, nmol, steps, warm, mode, dt=2.0, seed=1):
+    """Rigid SPC/E water (32,768 molecules, 98,304 atoms, L = 99.3 A):
+    coul_damped_smoothed(0.2, 1 A), rc 10 A, skin 2 A, exact rotations,
+    velocity Verlet through the C ABI with energies every step."""
+    from emdee_amd.water_box import KB, water_system
+    md, cfg = water_system(lib, nmol, seed, coul="damped_smoothed", alpha=0.2)
+    lib.EmDeeGPU_set_precision(md, mode)
+    lib.random_momenta(md, KB * 298.0, True, seed)
+    run_steps(lib, md, warm, dt)
+    lib.EmDeeGPU_synchronize(md)
+    t0 = time.perf_counter()
+    run_steps(lib, md, steps, dt)
+    lib.EmDeeGPU_synchronize(md)
+    t = time.perf_counter() - t0
+    lib.EmDeeGPU_destroy(ctypes.byref(md))
+    return cfg["N"] * steps / t, 1e3 * t / steps
+
+
+def layers_bench(lib, nmol, outer, seed=1):
+    """Rigid SPC/E water, 4096 molecules, two layers (1: coul_long + Ewald,
+    2: its real-space part): milliseconds per femtosecond of trajectory for
+    velocity Verlet on layer 1 (dt 1 fs), r-RESPA 1:4 (outer 4 fs, inner 1 fs
+    on layer 2) and velocity Verlet on layer 2 alone."""
+    from emdee_amd import mts
+    from emdee_amd.surrogate import two_layer_water
+    from emdee_amd.water_box import KB
+    out = {}
+    for name in ("exact_vv", "respa_1_4", "short_vv"):
+        md, cfg = two_layer_water(lib, nmol, seed)
+        lib.random_momenta(md, KB * 298.0, True, seed)
+        def go(k):
+            if name == "respa_1_4":
+                mts.respa(lib, md, k, 4.0, 4)
+            else:
+                lib.switch_model_layer(md, 1 if name == "exact_vv" else 2)
+                run_steps(lib, md, 4 * k, 1.0)
+        go(2)
+        lib.EmDeeGPU_synchronize(md)
+        t0 = time.perf_counter()
+        go(outer)
+        lib.EmDeeGPU_synchronize(md)
+        out[name + "_ms_per_fs"] = 1e3 * (time.perf_counter() - t0) / (4 * outer)
+        lib.EmDeeGPU_destroy(ctypes.byref(md))
+    out["respa_speedup_over_exact"] = out["exact_vv_ms_per_fs"] / out["respa_1_4_ms_per_fs"]
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -175,10 +222,11 @@ def main():
     ap.add_argument("--hmc-replicas", type=int, default=512)
     ap.add_argument("--hmc-cycles", type=int, default=60)
     ap.add_argument("--no-hmc", action="store_true")
+    ap.add_argument("--no-water", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build measures one GPU; --gpus {args.gpus} is not supported "
-              "(the engine runs one system per GPU; see DESIGN.md for the multi-GPU design)", file=sys.stderr)
+              "(the engine runs one system per GPU)", file=sys.stderr)
         sys.exit(2)
     os.environ.setdefault("EMDEE_QUIET", "1")
     import emdee_amd
@@ -225,6 +273,17 @@ def main():
                                    "sample": f"reference-API replay of all {len(reps[0])} replicas, one 10-step "
                                              f"cycle each ({ht:.1f} s)"}
 
+    water = None
+    if not args.no_water:
+        print("# rigid water: 32768 molecules; two-layer water: 4096 molecules", file=sys.stderr, flush=True)
+        w64, m64 = water_bench(lib, 32768, 100, 10, 0)
+        wmx, mmx = water_bench(lib, 32768, 100, 10, 1)
+        water = {"rigid_spce_98304": {"fp64_atom_steps_per_s": w64, "fp64_ms_per_step": m64,
+                                      "mixed_atom_steps_per_s": wmx, "mixed_ms_per_step": mmx,
+                                      "coul": "damped_smoothed(0.2, 1.0)", "rc": 10.0, "skin": 2.0, "dt_fs": 2.0},
+                 "two_layer_spce_12288": layers_bench(lib, 4096, 10)}
+        print(f"# water: {water}", file=sys.stderr, flush=True)
+
     line = {
         "metric": METRIC, "value": value, "unit": "atom-steps/s", "n_gpus": 1,
         "steps": args.steps, "warmup": args.warmup,
@@ -246,6 +305,7 @@ def main():
                      "traffic_source": "profiles/k_pairs_lj1m_traffic.txt"},
         "cpu_baseline": cpu,
         "hmc": hmc,
+        "water": water,
     }
     print(json.dumps(line), flush=True)
 
