@@ -6,6 +6,7 @@
 // struct.  Per-atom state lives on the GPU (device.hip); this file keeps the
 // system description and drives the kernels.
 #include <algorithm>
+#include <memory>
 #include <chrono>
 #include <cmath>
 #include <cstdio>
@@ -35,6 +36,22 @@ double wtime() {
 
 struct EnergyRec { double Potential, Dispersion, Coulomb, Bond, Angle, Dihedral, ShadowPotential; bool UpToDate; };
 struct VirialRec { double Total, Body; };
+
+struct System;
+
+// Phase space shared between systems (EmDee_share_phase_space,
+// EmDeeCode.f90:239-269).  The reference makes the systems alias one set of
+// coordinates, momenta, rigid bodies and box.  Each system here keeps its
+// atoms on the GPU in its own cell order, so the sharing is kept coherent by
+// version: the last member to change the phase space is its writer, and any
+// other member copies the writer's state into its own arrays before it next
+// reads or changes it.  Forces, lists and energies stay with each system, as
+// in the reference.
+struct SharedPhase {
+  System* writer = nullptr;
+  unsigned long long version = 0;
+  std::vector<System*> members;
+};
 
 // tData (EmDeeData.f90:66-151), host part.
 struct System {
@@ -71,6 +88,8 @@ struct System {
   int precision = 0;
   Device* dev = nullptr;
   std::vector<double> mirror;                // host buffer for EmDee_memory_address
+  std::shared_ptr<SharedPhase> phase;        // set once the system shares its phase space
+  unsigned long long phaseSeen = 0;          // version of the shared state this system holds
 
   PairEntry& P(int i, int j, int l) { return pair[(i - 1) + ntypes * ((j - 1) + ntypes * (l - 1))]; }
   char& T2(std::vector<char>& a, int i, int j) { return a[(i - 1) + ntypes * (j - 1)]; }
@@ -90,6 +109,48 @@ double phi_fn(double x) {   // math.f90:230-237
   if (std::fabs(x) > 1e-4) return (1.0 - std::exp(-x)) / x;
   return 1.0 + 0.5 * x * ((1.0 / 3.0) * x * (1.0 - 0.25 * x) - 1.0);
 }
+
+// Brings the system's copy of a shared phase space up to date.
+void phase_acquire(System* me) {
+  if (!me->phase) return;
+  SharedPhase& h = *me->phase;
+  if (h.writer && h.writer != me && me->phaseSeen != h.version) {
+    dev_copy_phase(me->dev, h.writer->dev);
+    me->L = h.writer->L;
+    me->phaseSeen = h.version;
+  }
+}
+
+// Records that the system has changed the shared phase space.
+void phase_publish(System* me) {
+  if (!me->phase) return;
+  me->phase->writer = me;
+  me->phaseSeen = ++me->phase->version;
+}
+
+// Leaves a shared phase space; the others first take the latest state.
+void phase_leave(System* me) {
+  if (!me->phase) return;
+  std::shared_ptr<SharedPhase> h = me->phase;
+  auto& m = h->members;
+  m.erase(std::remove(m.begin(), m.end(), me), m.end());
+  if (h->writer == me) {
+    for (System* o : m) phase_acquire(o);
+    h->writer = m.empty() ? nullptr : m.front();
+  }
+  me->phase.reset();
+  me->phaseSeen = 0;
+}
+
+// Keeps a shared phase space coherent around an entry point: the state is
+// brought up to date on entry and, for a call that changes it, published on
+// exit.
+struct PhaseScope {
+  System* me;
+  bool writes;
+  PhaseScope(System* s, bool w) : me(s), writes(w) { phase_acquire(me); }
+  ~PhaseScope() { if (writes) phase_publish(me); }
+};
 
 void mark_stale(System* me, tEmDee* md) {
   for (int l = 1; l <= me->nlayers; l++) { me->forcesUpToDate[l] = 0; me->layerEnergy[l - 1].UpToDate = false; }
@@ -208,6 +269,7 @@ void initialize_system(System* me, tEmDee* md) {
 void compute_forces(tEmDee* md) {
   System* me = S(*md);
   if (!me->initialized) fatal("force computation", "box and coordinates have not been defined");
+  phase_acquire(me);
   double E[5] = { 0, 0, 0, 0, 0 }, W[5] = { 0, 0, 0, 0, 0 };
   enum { PAIR_ = 0, COUL_ = 1, LONG_ = 2, BOND_ = 3, ANGLE_ = 4 };
   // handle_neighbor_lists: neighbor_lists.f90:175-195
@@ -418,6 +480,7 @@ tEmDee EmDee_system(int threads, int layers, double rc, double skin, int N, int*
 // address returned is a host snapshot taken at the time of the call.
 void* EmDee_memory_address(tEmDee md, char* option) {
   System* me = S(md);
+  PhaseScope ps(me, false);
   std::string item(option);
   me->mirror.assign(3 * (size_t)me->natoms * (item == "layerForces" ? me->nlayers : 1), 0.0);
   if (item == "coordinates") dev_download_positions(me->dev, me->mirror.data());
@@ -431,8 +494,35 @@ void* EmDee_memory_address(tEmDee md, char* option) {
 
 // EmDeeCode.f90:239-269
 void EmDee_share_phase_space(tEmDee mdkeep, tEmDee* mdlose) {
-  (void)mdkeep; (void)mdlose;
-  fatal("phase space sharing", "not available in this build of the engine");
+  const char* task = "phase space sharing";
+  System *keep = S(mdkeep), *lose = S(*mdlose);
+  if (!(keep->initialized && lose->initialized)) fatal(task, "EmDee system 1 has not been initialized");
+  if (keep->natoms != lose->natoms) fatal(task, "different numbers of atoms");
+  if (keep->atomType != lose->atomType) fatal(task, "atom types do not match");
+  if (keep->mass != lose->mass) fatal(task, "atom masses do not match");
+  if (keep->atomBody != lose->atomBody) fatal(task, "rigid bodies do not match");
+  if (keep == lose) return;
+  phase_acquire(keep);
+  if (!keep->phase) {
+    keep->phase = std::make_shared<SharedPhase>();
+    keep->phase->members.push_back(keep);
+    phase_publish(keep);
+  }
+  if (lose->phase != keep->phase) {
+    phase_leave(lose);
+    lose->phase = keep->phase;
+    lose->phase->members.push_back(lose);
+  }
+  // lose now holds the shared state: copy it in place of the pointer swap
+  lose->phaseSeen = 0;
+  keep->phase->writer = keep;
+  phase_acquire(lose);
+  mdlose->Kinetic.Kinetic = mdkeep.Kinetic.Kinetic;
+  for (int k = 0; k < 3; k++) {
+    mdlose->Kinetic.TransPart[k] = mdkeep.Kinetic.TransPart[k];
+    mdlose->Kinetic.RotPart[k] = mdkeep.Kinetic.RotPart[k];
+  }
+  mdlose->Kinetic.Rotational = mdkeep.Kinetic.Rotational;
 }
 
 // EmDeeCode.f90:273-305
@@ -618,6 +708,7 @@ void EmDee_add_dihedral(tEmDee md, int i, int j, int k, int l, void* model) {
 // EmDeeCode.f90:659-801
 void EmDee_download(tEmDee md, char* option, double* address) {
   System* me = S(md);
+  PhaseScope ps(me, false);
   std::string item(option);
   if (!address) fatal("download", "provided address is invalid");
   int nb = me->nbodies;
@@ -659,6 +750,7 @@ void EmDee_download(tEmDee md, char* option, double* address) {
 // EmDeeCode.f90:805-925
 void EmDee_upload(tEmDee* md, char* option, double* address) {
   System* me = S(*md);
+  PhaseScope ps(me, true);
   std::string item(option);
   if (!address) fatal("upload", "provided address is invalid");
   if (item == "box") {
@@ -710,6 +802,7 @@ static void body_pi_from_omega(const double* q, const double* MoI, const double*
 // and the results are uploaded.
 void EmDee_random_momenta(tEmDee* md, double kT, bool adjust, int seed) {
   System* me = S(*md);
+  PhaseScope ps(me, true);
   if (me->rng.seeding_required) me->rng.setup(seed);
   double twoKEt[3] = { 0, 0, 0 }, twoKEr[3] = { 0, 0, 0 };
   int N = me->natoms, nb = me->nbodies;
@@ -781,6 +874,7 @@ void EmDee_random_momenta(tEmDee* md, double kT, bool adjust, int seed) {
 // EmDeeCode.f90:1024-1065
 void EmDee_boost(tEmDee* md, double lambda, double alpha, double dt) {
   System* me = S(*md);
+  PhaseScope ps(me, true);
   double CF = phi_fn(alpha * dt) * dt;
   double CP = 1.0 - alpha * CF;
   CF = lambda * CF;
@@ -803,6 +897,7 @@ void EmDee_boost(tEmDee* md, double lambda, double alpha, double dt) {
 // EmDeeCode.f90:1069-1103
 static void displace(tEmDee* md, double lambda, double alpha, double dt) {
   System* me = S(*md);
+  PhaseScope ps(me, true);
   md->Time.Motion -= wtime();
   double CR, CP;
   if (alpha != 0.0) {
@@ -825,6 +920,7 @@ void EmDee_move(tEmDee* md, double lambda, double alpha, double dt) { displace(m
 // EmDeeCode.f90:1107-1211
 void EmDee_verlet_step(tEmDee* md, double dt) {
   System* me = S(*md);
+  PhaseScope ps(me, true);
   double dt_2 = 0.5 * dt;
   bool compute = md->Options.Compute;
   int mode = md->Options.RotationMode, l = me->layer;
@@ -862,6 +958,7 @@ void EmDee_rdf(tEmDee md, int bins, double Rc, int pairs, int itype[], int jtype
   const char* task = "radial distribution calculation";
   const double Pi4_3 = 4.188790204786391;
   System* me = S(md);
+  PhaseScope ps(me, false);
   int nt = me->ntypes;
   for (int p = 0; p < pairs; p++)
     if (!ranged({ itype[p], jtype[p] }, nt)) fatal(task, "at least one provided type index is out of range");
@@ -919,11 +1016,13 @@ void EmDeeGPU_attach_surrogate(tEmDee md, EmDeeSurrogate* s, int layer) {
 void EmDeeGPU_surrogate_sample(tEmDee md, EmDeeSurrogate* s, const double* labels) {
   System* me = S(md);
   if (!me->initialized) fatal("surrogate sampling", "box and coordinates have not been defined");
+  PhaseScope ps(me, false);
   dev_surrogate_add_sample(me->dev, s, me->L, labels);
 }
 void EmDeeGPU_destroy(tEmDee* md) {
   if (!md || !md->Data) return;
   System* me = S(*md);
+  phase_leave(me);
   dev_destroy(me->dev);
   delete me;
   md->Data = nullptr;

@@ -1497,4 +1497,29 @@ void dev_kernel_stats(Device* d, double* out, int n) {
 }
 void dev_sync(Device* d) { HIP_CHECK(hipStreamSynchronize(d->stream)); }
 
+// dst slot s holds original atom dperm[s], which sits at slot siperm[o] of src
+__global__ void k_copy_phase(double* __restrict__ dR, double* __restrict__ dP, const double* __restrict__ sR,
+                             const double* __restrict__ sP, const int* __restrict__ dperm,
+                             const int* __restrict__ siperm, int N, int dNpad, int sNpad) {
+  int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= N) return;
+  int t = siperm[dperm[s]];
+#pragma unroll
+  for (int c = 0; c < 3; c++) {
+    dR[(size_t)c * dNpad + s] = sR[(size_t)c * sNpad + t];
+    dP[(size_t)c * dNpad + s] = sP[(size_t)c * sNpad + t];
+  }
+}
+
+void dev_copy_phase(Device* dst, Device* src) {
+  if (dst->N != src->N) fatal("phase space sharing", "different numbers of atoms");
+  HIP_CHECK(hipStreamSynchronize(src->stream));
+  if (dst->N > 0)
+    hipLaunchKernelGGL(k_copy_phase, dim3(div_up(dst->N, BLOCK)), dim3(BLOCK), 0, dst->stream, dst->R.p, dst->P.p,
+                       src->R.p, src->P.p, dst->perm.p, src->iperm.p, dst->N, dst->Npad, src->Npad);
+  launch_check("phase copy");
+  dev_bodies_copy(dst, src);
+  HIP_CHECK(hipStreamSynchronize(dst->stream));
+}
+
 }  // namespace emdee

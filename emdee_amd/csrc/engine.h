@@ -161,6 +161,10 @@ void dev_get_atom_energies(Device* d, double* host);
 void dev_set_profiling(Device* d, bool on);
 void dev_kernel_stats(Device* d, double* out, int n);
 void dev_sync(Device* d);
+// Phase-space sharing: copies coordinates and momenta of every atom (each
+// system in its own slot order) and the rigid-body state from src to dst.
+void dev_copy_phase(Device* dst, Device* src);
+void dev_bodies_copy(Device* dst, Device* src);
 
 // Runtime errors: prints "Error in <task>: <msg>." and exits (global.f90:51-56).
 [[noreturn]] void fatal(const std::string& task, const std::string& msg);

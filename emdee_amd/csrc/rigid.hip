@@ -787,6 +787,21 @@ void dev_bodies_place(Device* d) {
 }
 
 // host <-> device copies of one body field, host layout [nb][ncomp]
+// The whole body record is shared (ArBee.f90 tBody): static properties,
+// body-frame and space-frame member offsets and the dynamic state.  Forces
+// and torques stay with each system.
+void dev_bodies_copy(Device* dst, Device* src) {
+  BodyState &a = dst->bodies, &b = src->bodies;
+  if (a.nb != b.nb || a.nm != b.nm) fatal("phase space sharing", "rigid bodies do not match");
+  if (a.nb == 0) return;
+  auto cp = [&](DBuf<double>& x, const DBuf<double>& y) {
+    HIP_CHECK(hipMemcpyAsync(x.p, y.p, sizeof(double) * std::min(x.n, y.n), hipMemcpyDeviceToDevice, dst->stream));
+  };
+  cp(a.mass, b.mass); cp(a.invMass, b.invMass); cp(a.MoI, b.MoI); cp(a.invMoI, b.invMoI); cp(a.coef, b.coef);
+  cp(a.d, b.d); cp(a.delta, b.delta);
+  cp(a.rcm, b.rcm); cp(a.pcm, b.pcm); cp(a.q, b.q); cp(a.pi, b.pi); cp(a.omega, b.omega);
+}
+
 void dev_bodies_get(Device* d, int f, double* host) {
   BodyState& B = d->bodies;
   if (B.nb == 0) return;
