@@ -274,7 +274,7 @@ void compute_forces(tEmDee* md) {
   enum { PAIR_ = 0, COUL_ = 1, LONG_ = 2, BOND_ = 3, ANGLE_ = 4 };
   // handle_neighbor_lists: neighbor_lists.f90:175-195
   md->Time.Neighbor -= wtime();
-  if (dev_needs_rebuild(me->dev, me->skinSq)) {
+  if (dev_needs_rebuild(me->dev, me->skinSq, me->L, me->precision != 0)) {
     dev_rebuild(me->dev, me->L, me->xRc, std::sqrt(me->xInRcSq));
     md->Builds += 1;
   }
@@ -291,6 +291,7 @@ void compute_forces(tEmDee* md) {
   c.L = me->L;
   c.invL2 = 1.0 / (me->L * me->L);
   c.Rc2 = (me->useInRc[l] ? me->InRcSq : me->RcSq) * c.invL2;
+  c.scaled = 1;
   ForceSums fs{ 0, 0, 0, 0 };
   if (me->pairs_exist[l]) {
     dev_compute_pairs(me->dev, c, &fs);

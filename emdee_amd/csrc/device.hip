@@ -117,8 +117,37 @@ __global__ void k_scale4(double4* __restrict__ Rs4, const double* __restrict__ R
 // Pass A: per block (max, smallest original index attaining it); its
 // reduction gives the maximum and the first index k.  Pass C: per block max
 // over atoms with original index < k.  Both reductions are deterministic.
+// Scaled positions of a slot: Rs = R / L (EmDeeCode.f90:1228, IEEE division)
+// with the type, and, when Q4 is given, the mixed form's fixed-point copy
+// u = round(2^32 (s - floor s)) mod 2^32 and FP32 charge.
+struct Scaled {
+  double4* Rs4;
+  int4* Q4;
+  float* qf;
+  const int* type;
+  const double* charge;
+  const int* charged;
+  double L;
+};
+__device__ __forceinline__ void scale_slot(const Scaled& sc, const double* __restrict__ R, int s, int Npad) {
+  double v[3] = { R[s] / sc.L, R[Npad + s] / sc.L, R[2 * Npad + s] / sc.L };
+  int t = sc.type[s];
+  sc.Rs4[s] = make_double4(v[0], v[1], v[2], (double)t);
+  if (sc.Q4) {
+    unsigned int u[3];
+    for (int x = 0; x < 3; x++) {
+      double f = v[x] - floor(v[x]);
+      u[x] = (unsigned int)(unsigned long long)llrint(f * 4294967296.0);
+    }
+    sc.Q4[s] = make_int4((int)u[0], (int)u[1], (int)u[2], t);
+    sc.qf[s] = sc.charged[s] ? (float)sc.charge[s] : 0.0f;
+  }
+}
+
+// Pass A also writes the scaled positions of the current coordinates (sc),
+// which the pair kernels of the same force evaluation read.
 __global__ void k_disp_a(double* __restrict__ d2, double* __restrict__ part, const double* __restrict__ R,
-                         const double* __restrict__ R0, const int* __restrict__ perm, int N, int Npad) {
+                         const double* __restrict__ R0, const int* __restrict__ perm, int N, int Npad, Scaled sc) {
   __shared__ double smax[BLOCK / 64];
   __shared__ int sidx[BLOCK / 64];
   int s = blockIdx.x * BLOCK + threadIdx.x;
@@ -130,6 +159,7 @@ __global__ void k_disp_a(double* __restrict__ d2, double* __restrict__ part, con
     v = dx * dx + dy * dy + dz * dz;
     d2[s] = v;
     idx = perm[s];
+    scale_slot(sc, R, s, Npad);
   }
   for (int off = 32; off > 0; off >>= 1) {
     double ov = __shfl_down(v, off, 64);
@@ -322,15 +352,8 @@ __global__ void k_scale4q(double4* __restrict__ Rs4, int4* __restrict__ Q4, floa
                           const int* __restrict__ charged, double L, int N, int Npad) {
   int s = blockIdx.x * blockDim.x + threadIdx.x;
   if (s >= N) return;
-  double v[3] = { R[s] / L, R[Npad + s] / L, R[2 * Npad + s] / L };
-  unsigned int u[3];
-  for (int x = 0; x < 3; x++) {
-    double f = v[x] - floor(v[x]);
-    u[x] = (unsigned int)(unsigned long long)llrint(f * 4294967296.0);
-  }
-  Rs4[s] = make_double4(v[0], v[1], v[2], (double)type[s]);
-  Q4[s] = make_int4((int)u[0], (int)u[1], (int)u[2], type[s]);
-  qf[s] = charged[s] ? (float)charge[s] : 0.0f;
+  Scaled sc{ Rs4, Q4, qf, type, charge, charged, L };
+  scale_slot(sc, R, s, Npad);
 }
 
 // relative half-width of the FP32 band around a cutoff Rc2s (scaled units^2)
@@ -1169,11 +1192,14 @@ void dev_add_forces(Device* d, int layer, const double* F) {
   HIP_CHECK(hipStreamSynchronize(d->stream));
 }
 
-bool dev_needs_rebuild(Device* d, double skinSq) {
+bool dev_needs_rebuild(Device* d, double skinSq, double L, bool mixed) {
   int N = d->N;
   if (N == 0) return false;
   int g = div_up(N, BLOCK);
-  hipLaunchKernelGGL(k_disp_a, dim3(g), dim3(BLOCK), 0, d->stream, d->d2.p, d->partials.p, d->R.p, d->R0.p, d->perm.p, N, d->Npad);
+  if (mixed) { d->Q4.alloc(d->Npad); d->qf.alloc(d->Npad); }
+  Scaled sc{ d->Rs4.p, mixed ? d->Q4.p : nullptr, d->qf.p, d->type.p, d->charge.p, d->chargedFlag.p, L };
+  hipLaunchKernelGGL(k_disp_a, dim3(g), dim3(BLOCK), 0, d->stream, d->d2.p, d->partials.p, d->R.p, d->R0.p, d->perm.p,
+                     N, d->Npad, sc);
   hipLaunchKernelGGL(k_disp_a_reduce, dim3(1), dim3(BLOCK), 0, d->stream, d->trig.p, d->partials.p, g);
   hipLaunchKernelGGL(k_disp_c, dim3(g), dim3(BLOCK), 0, d->stream, d->partials.p, d->d2.p, d->perm.p, d->trig.p, N);
   unsigned long long seq = ++d->seq;
@@ -1304,7 +1330,9 @@ void dev_compute_pairs(Device* d, const PairCall& c, ForceSums* out) {
   double* F = d->F.p + 3 * (size_t)Np * c.layer;
   if (N == 0) { *out = ForceSums{ 0, 0, 0, 0 }; return; }
   int g = div_up(N, BLOCK);
-  if (c.mixed) {
+  if (c.scaled) {
+    // the trigger of this force evaluation (or the list build) wrote them
+  } else if (c.mixed) {
     d->Q4.alloc(Np);
     d->qf.alloc(Np);
     hipLaunchKernelGGL(k_scale4q, dim3(g), dim3(BLOCK), 0, d->stream, d->Rs4.p, d->Q4.p, d->qf.p, d->R.p, d->type.p,

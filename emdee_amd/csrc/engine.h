@@ -80,6 +80,7 @@ struct PairCall {
   int energy;         // Options.Compute
   int mixed;          // FP32 pair arithmetic with FP64 accumulation
   double L, invL2, Rc2;   // box and scaled squared cutoff
+  int scaled;         // scaled positions are current (written by the trigger or the list build)
 };
 
 // Sums returned by one force evaluation (indices of EmDeeCode.f90:1219).
@@ -113,7 +114,9 @@ void dev_upload_forces(Device* d, int layer, const double* F);
 void dev_download_forces(Device* d, int layer, double* F);
 
 // Neighbor lists.
-bool dev_needs_rebuild(Device* d, double skinSq);                 // maximum_approach_sq trigger
+// Also writes the scaled positions of the current coordinates (and, when
+// mixed, their fixed-point copy) for the pair kernels of the same evaluation.
+bool dev_needs_rebuild(Device* d, double skinSq, double L, bool mixed);                 // maximum_approach_sq trigger
 void dev_rebuild(Device* d, double L, double xRc, double xInRc);  // sort + full Verlet list
 int dev_neighbor_pairs(Device* d, int max, int* out);             // half set, 1-based i<j
 
