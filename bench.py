@@ -52,11 +52,13 @@ def measured_traffic(form):
 
 def fcc_lattice(natoms, density):
     nc = int(np.ceil((natoms / 4) ** (1.0 / 3.0) - 1e-9))
-    a = (4.0 / density) ** (1.0 / 3.0)
+    # the nc^3 lattice spans the box, so a truncated lattice (natoms < 4 nc^3)
+    # leaves vacancies instead of overlapping its periodic images
+    L = (natoms / density) ** (1.0 / 3.0)
+    a = L / nc
     basis = np.array([[0, 0, 0], [0.5, 0.5, 0], [0.5, 0, 0.5], [0, 0.5, 0.5]])
     g = np.stack(np.meshgrid(np.arange(nc), np.arange(nc), np.arange(nc), indexing="ij"), -1).reshape(-1, 3)
     R = (g[:, None, :] + basis[None, :, :]).reshape(-1, 3)[:natoms] * a
-    L = (natoms / density) ** (1.0 / 3.0)
     return np.ascontiguousarray(R), L
 
 

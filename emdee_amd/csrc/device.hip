@@ -120,6 +120,11 @@ __global__ void k_scale4(double4* __restrict__ Rs4, const double* __restrict__ R
 // Scaled positions of a slot: Rs = R / L (EmDeeCode.f90:1228, IEEE division)
 // with the type, and, when Q4 is given, the mixed form's fixed-point copy
 // u = round(2^32 (s - floor s)) mod 2^32 and FP32 charge.
+// Q4.w holds the type; a slot whose scaled position is not finite also
+// carries Q4_NONFINITE, so that the fixed-point prefilter of the list build
+// leaves its candidates to the FP64 test (which rejects them, as the
+// reference's r^2 comparison does with NaN).
+constexpr int Q4_NONFINITE = 1 << 30, Q4_TYPE = Q4_NONFINITE - 1;
 struct Scaled {
   double4* Rs4;
   int4* Q4;
@@ -139,7 +144,8 @@ __device__ __forceinline__ void scale_slot(const Scaled& sc, const double* __res
       double f = v[x] - floor(v[x]);
       u[x] = (unsigned int)(unsigned long long)llrint(f * 4294967296.0);
     }
-    sc.Q4[s] = make_int4((int)u[0], (int)u[1], (int)u[2], t);
+    bool fin = isfinite(v[0]) && isfinite(v[1]) && isfinite(v[2]);
+    sc.Q4[s] = make_int4((int)u[0], (int)u[1], (int)u[2], fin ? t : (t | Q4_NONFINITE));
     sc.qf[s] = sc.charged[s] ? (float)sc.charge[s] : 0.0f;
   }
 }
@@ -406,8 +412,9 @@ __device__ __forceinline__ void scan_range(int a, int b, int i, double xi, doubl
       float dy = (float)(int)((unsigned int)qi.y - (unsigned int)qj.y);
       float dz = (float)(int)((unsigned int)qi.z - (unsigned int)qj.z);
       float r2 = dx * dx + dy * dy + dz * dz;
-      bool in = r2 < lo;
-      if (!in && r2 < hi) in = exact_inside(Rs4, i, j, xRc2);
+      bool odd = ((qi.w | qj.w) & Q4_NONFINITE) != 0;
+      bool in = r2 < lo && !odd;
+      if (!in && (r2 < hi || odd)) in = exact_inside(Rs4, i, j, xRc2);
       if (in && j != i) {
         if (cnt < cap) nbr[(size_t)cnt * Npad + i] = j;
         cnt++;
@@ -762,7 +769,7 @@ k_pairs_mx(double* __restrict__ F, double* __restrict__ partials, const int4* __
     const DevCoul cm = *coulp;
     const int4 qi = Q4[i];
     const float qqi = qf[i];
-    const DevPair* prow = pairs + qi.w * ntypes;
+    const DevPair* prow = pairs + (qi.w & Q4_TYPE) * ntypes;
     DevPair pu;
     if (UNIFORM) pu = pairs[0];   // uniform address: scalar loads, no VGPRs
     const int n = nnb[i];
@@ -783,7 +790,7 @@ k_pairs_mx(double* __restrict__ F, double* __restrict__ partials, const int4* __
           float qiqj = jc ? float(pu.kCoul) * qqi * qf[j] : 0.0f;
           pair_body_mx<ENERGY, false>(pu, cm, jc, qiqj, dx, dy, dz, invR2, a);
         } else {
-          const DevPair p = prow[qj.w];
+          const DevPair p = prow[qj.w & Q4_TYPE];
           bool jc = qqi != 0.0f && p.coulomb && qf[j] != 0.0f;
           float qiqj = jc ? float(p.kCoul) * qqi * qf[j] : 0.0f;
           pair_body_mx<ENERGY, false>(p, cm, jc, qiqj, dx, dy, dz, invR2, a);
