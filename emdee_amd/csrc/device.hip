@@ -399,13 +399,57 @@ constexpr int CELL_REACH = 3;
 // the distance test alone decides (free atoms have distinct body ids); it is
 // made on the 16-byte fixed-point positions, and only candidates within the
 // FP32 band around the cutoff take the FP64 test on the scaled positions.
-template <bool SIMPLE>
+// S > 1: the S consecutive lanes of an atom take S consecutive candidates at
+// a time; a ballot over the group places the accepted ones, so the list
+// keeps the ascending order of the one-lane scan (for long lists: water).
+template <bool SIMPLE, int S>
 __device__ __forceinline__ void scan_range(int a, int b, int i, double xi, double yi, double zi, int ti, int bi, int oi,
                                            const double4* __restrict__ Rs4, const int* __restrict__ body,
                                            const int* __restrict__ perm, const uint8_t* __restrict__ interact,
                                            const int* __restrict__ exItem, int e0, int e1, int ntypes, double xRc2,
                                            int cap, int Npad, int* __restrict__ nbr, int& cnt,
                                            const int4* __restrict__ Q4, int4 qi, float lo, float hi) {
+  if (S > 1) {
+    auto accept = [&](int j) -> bool {
+      if (SIMPLE) {
+        int4 qj = Q4[j];
+        float dx = (float)(int)((unsigned int)qi.x - (unsigned int)qj.x);
+        float dy = (float)(int)((unsigned int)qi.y - (unsigned int)qj.y);
+        float dz = (float)(int)((unsigned int)qi.z - (unsigned int)qj.z);
+        float r2 = dx * dx + dy * dy + dz * dz;
+        bool odd = ((qi.w | qj.w) & Q4_NONFINITE) != 0;
+        bool in = r2 < lo && !odd;
+        if (!in && (r2 < hi || odd)) in = exact_inside(Rs4, i, j, xRc2);
+        return in && j != i;
+      }
+      double4 rj = Rs4[j];
+      double r2;
+      {
+#pragma clang fp contract(off)
+        double d0 = xi - rj.x, d1 = yi - rj.y, d2 = zi - rj.z;
+        d0 = d0 - rint(d0); d1 = d1 - rint(d1); d2 = d2 - rint(d2);
+        r2 = d0 * d0 + d1 * d1 + d2 * d2;
+      }
+      if (!(r2 < xRc2 && j != i)) return false;
+      if (body[j] == bi) return false;
+      if (!interact[(int)rj.w * ntypes + ti]) return false;
+      if (e1 > e0) {
+        int oj = perm[j];
+        for (int e = e0; e < e1; e++) if (exItem[e] == oj) return false;
+      }
+      return true;
+    };
+    const int lane = threadIdx.x & 63, sub = lane & (S - 1), gbase = lane & ~(S - 1);
+    for (int j0 = a; j0 < b; j0 += S) {
+      int j = j0 + sub;
+      bool ok = j < b && accept(j);
+      unsigned int g = (unsigned int)(__ballot(ok) >> gbase) & ((1u << S) - 1u);
+      int pos = cnt + __popc(g & ((1u << sub) - 1u));
+      if (ok && pos < cap) nbr[(size_t)pos * Npad + i] = j;
+      cnt += __popc(g);
+    }
+    return;
+  }
   if (SIMPLE) {
     auto testq = [&](int j, int4 qj) {
       float dx = (float)(int)((unsigned int)qi.x - (unsigned int)qj.x);
@@ -456,7 +500,7 @@ __device__ __forceinline__ void scan_range(int a, int b, int i, double xi, doubl
   }
   for (; j < b; j++) test(j, Rs4[j]);
 }
-template <bool SIMPLE>
+template <bool SIMPLE, int S>
 __global__ void __launch_bounds__(BLOCK)
 k_build_list(int* __restrict__ nbr, int* __restrict__ nnb, int* __restrict__ flags, double* __restrict__ part,
              const double4* __restrict__ Rs4, const int* __restrict__ key, const int* __restrict__ start,
@@ -465,7 +509,8 @@ k_build_list(int* __restrict__ nbr, int* __restrict__ nnb, int* __restrict__ fla
              int nexcl, int ntypes, int M, double xRc2, int cap, int N, int Npad, const int4* __restrict__ Q4,
              float lo, float hi) {
   int blk = xcd_block();
-  int i = blk * BLOCK + threadIdx.x;
+  int gt = blk * BLOCK + threadIdx.x;
+  int i = S == 1 ? gt : gt / S, sub = S == 1 ? 0 : gt % S;
   int cnt = 0;
   if (i < N) {
     double4 ri = Rs4[i];
@@ -514,7 +559,7 @@ k_build_list(int* __restrict__ nbr, int* __restrict__ nnb, int* __restrict__ fla
       return q;
     };
     auto scan = [&](int a, int b) {
-      scan_range<SIMPLE>(a, b, i, ri.x, ri.y, ri.z, ti, bi, oi, Rs4, body, perm, interact, exItem, e0, e1, ntypes, xRc2,
+      scan_range<SIMPLE, S>(a, b, i, ri.x, ri.y, ri.z, ti, bi, oi, Rs4, body, perm, interact, exItem, e0, e1, ntypes, xRc2,
                          cap, Npad, nbr, cnt, Q4, qi, lo, hi);
     };
     // the cell starts of the next row are loaded before the current row is
@@ -526,14 +571,16 @@ k_build_list(int* __restrict__ nbr, int* __restrict__ nnb, int* __restrict__ fla
       scan(cur.a1, cur.b1);
       cur = nxt;
     }
-    nnb[i] = cnt;
-    // pad the list to a multiple of 4 with the atom itself (zero separation):
-    // the mixed-form kernel reads whole groups of four without a bound test
-    if (cnt <= cap)
-      for (int k = cnt; k < ((cnt + 3) & ~3); k++) nbr[(size_t)k * Npad + i] = i;
-    if (cnt > cap) atomicMax(&flags[0], cnt);
+    if (sub == 0) {
+      nnb[i] = cnt;
+      // pad the list to a multiple of 4 with the atom itself (zero separation):
+      // the mixed-form kernel reads whole groups of four without a bound test
+      if (cnt <= cap)
+        for (int k = cnt; k < ((cnt + 3) & ~3); k++) nbr[(size_t)k * Npad + i] = i;
+      if (cnt > cap) atomicMax(&flags[0], cnt);
+    }
   }
-  double v[1] = { (double)cnt };
+  double v[1] = { sub == 0 ? (double)cnt : 0.0 };
   block_sum<1>(v);
   if (threadIdx.x == 0) part[blk] = v[0];
 }
@@ -1281,8 +1328,8 @@ void dev_rebuild(Device* d, double L, double xRc, double xInRc) {
   double xRc2 = (xRc * xRc) / (L * L);
   const double two32 = 4294967296.0, rc2u = xRc2 * two32 * two32, band = band_rel(xRc2);
   float lo = (float)(rc2u * (1.0 - band)), hi = (float)(rc2u * (1.0 + band));
+  double vol = 4.18879020478639 * xRc * xRc * xRc * ((double)N / (L * L * L));   // expected list length
   if (d->cap == 0) {
-    double vol = 4.18879020478639 * xRc * xRc * xRc * ((double)N / (L * L * L));
     d->cap = std::max(32, (int)(1.35 * vol + 16));
     d->cap = (d->cap + 7) & ~7;
   }
@@ -1291,12 +1338,20 @@ void dev_rebuild(Device* d, double L, double xRc, double xInRc) {
     HIP_CHECK(hipMemsetAsync(d->flags.p, 0, sizeof(int) * 4, d->stream));
     if (d->profiling) HIP_CHECK(hipEventRecord(d->ev0, d->stream));
     bool simple = d->bodies.nb == 0 && d->nexcl == 0 && d->allInteract;
-    hipLaunchKernelGGL(simple ? k_build_list<true> : k_build_list<false>, dim3(g), dim3(BLOCK), 0, d->stream, d->nbr.p, d->nnb.p, d->flags.p, d->partials.p,
+    // lanes per atom: at least ~24 candidates per lane (the scanned region is
+    // ~2.6 times the list volume) and at most 2^21 lanes in all
+    double cand = 2.6 * vol;
+    long n = N;
+    int S = (cand >= 16 * 24 && n * 16 <= (1L << 21)) ? 16 : ((cand >= 4 * 24 && n * 4 <= (1L << 21)) ? 4 : 1);
+    auto kern = simple ? (S == 16 ? k_build_list<true, 16> : S == 4 ? k_build_list<true, 4> : k_build_list<true, 1>)
+                       : (S == 16 ? k_build_list<false, 16> : S == 4 ? k_build_list<false, 4> : k_build_list<false, 1>);
+    int gl = div_up((long)N * S, BLOCK);
+    hipLaunchKernelGGL(kern, dim3(gl), dim3(BLOCK), 0, d->stream, d->nbr.p, d->nnb.p, d->flags.p, d->partials.p,
                        d->Rs4.p, d->key.p, d->cellStart.p, d->body.p, d->perm.p, d->interact.p, d->exOff.p, d->exItem.p,
                        d->nexcl, d->ntypes, M, xRc2, d->cap, N, Np, d->Q4.p, lo, hi);
     launch_check("build_list");
     if (d->profiling) HIP_CHECK(hipEventRecord(d->ev1, d->stream));
-    hipLaunchKernelGGL(k_reduce_partials<1>, dim3(1), dim3(BLOCK), 0, d->stream, d->result.p + 8, d->partials.p, g);
+    hipLaunchKernelGGL(k_reduce_partials<1>, dim3(1), dim3(BLOCK), 0, d->stream, d->result.p + 8, d->partials.p, gl);
     HIP_CHECK(hipMemcpyAsync(d->hresult + 8, d->result.p + 8, sizeof(double), hipMemcpyDeviceToHost, d->stream));
     int* hf = reinterpret_cast<int*>(d->hresult + 16);
     HIP_CHECK(hipMemcpyAsync(hf, d->flags.p, sizeof(int), hipMemcpyDeviceToHost, d->stream));
@@ -1353,10 +1408,13 @@ void dev_compute_pairs(Device* d, const PairCall& c, ForceSums* out) {
   bool U = c.ntypes == 1;
   const DevPair& hp = d->hostPairs[(size_t)c.layer * c.ntypes * c.ntypes];
   bool LJ = U && !d->anyCharged && hp.kind == PAIR_LJ_CUT && (hp.modifier == MOD_NONE || hp.modifier == MOD_SHIFTED);
-  // threads per atom: enough wavefronts to fill the chip (about 2^18 lanes)
+  // threads per atom: enough wavefronts to fill the chip (at least 2^18
+  // lanes) and about 24 list entries per lane, so that long lists (water at
+  // rc 10 A holds ~700 entries per atom) are spread over many waves
   int tpa = 1;
   if (!LJ) {
-    long want = (1L << 18) / std::max(N, 1);
+    long lanes = std::max(1L << 18, (long)(d->listEntries / 24.0));
+    long want = lanes / std::max(N, 1);
     tpa = want >= 16 ? 16 : (want >= 4 ? 4 : 1);
   }
   int gp = div_up((long)N * tpa, BLOCK);

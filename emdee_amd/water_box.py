@@ -75,3 +75,30 @@ def water_system(lib, nmol, seed, rc=10.0, skin=2.0, coul="long", accuracy=np.ex
     lib.upload(md, "coordinates", R)
     lib.upload(md, "box", np.array([L]))
     return md, dict(R=R, types=types, mol=mol, L=L, N=len(types))
+
+
+if __name__ == "__main__":
+    # Profiling workload: python -m emdee_amd.water_box [molecules] [steps] [mixed]
+    # rigid SPC/E, coul_damped_smoothed(0.2, 1 A), rc 10 A, skin 2 A, dt 2 fs
+    import os
+    import sys
+    import time
+
+    from . import load
+    nmol = int(sys.argv[1]) if len(sys.argv) > 1 else 32768
+    steps = int(sys.argv[2]) if len(sys.argv) > 2 else 20
+    mixed = int(sys.argv[3]) if len(sys.argv) > 3 else 0
+    os.environ.setdefault("EMDEE_QUIET", "1")
+    lib = load()
+    md, cfg = water_system(lib, nmol, 1, coul="damped_smoothed", alpha=0.2)
+    lib.EmDeeGPU_set_precision(md, mixed)
+    lib.random_momenta(md, KB * 298.0, True, 1)
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        md.Options.Compute = True
+        lib.boost(md, 1.0, 0.0, 1.0)
+        lib.displace(md, 1.0, 0.0, 2.0)
+        lib.boost(md, 1.0, 0.0, 1.0)
+    lib.EmDeeGPU_synchronize(md)
+    t = time.perf_counter() - t0
+    print(f"{cfg['N']} atoms, {steps} steps: {1e3 * t / steps:.3f} ms/step, builds {md.Builds}", flush=True)
