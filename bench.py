@@ -1,7 +1,7 @@
 """Throughput of the EmDee force engine on one MI355X.
 
-Workload (BASELINE config "Large LJ, FP64 plain vs FP32-pair/FP64-accumulate"):
-Lennard-Jones fluid of 1,048,576 atoms (fcc 64^3 cells) at density 0.8442,
+Main workload (large LJ, plain FP64 against FP32 pair arithmetic with FP64
+accumulation): Lennard-Jones fluid of 1,048,576 atoms (fcc 64^3 cells) at density 0.8442,
 T = 1.0, lj_cut with rc = 2.5 and skin 0.3, velocity Verlet dt = 0.005 driven
 exactly as the reference hosts drive it (boost dt/2, displace dt, boost dt/2
 through the C ABI; energies every step).  Both arithmetic forms run: plain
@@ -18,7 +18,11 @@ cpu_baseline times the CPU oracle (the plain-C restatement of the reference
 algorithm, OpenMP over the host cores) on the same system for a bounded
 number of steps.
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W] [--natoms N]
+Also reported: batched HMC of 512 LJ replicas (effective samples/s), rigid
+SPC/E water of 98,304 atoms, and two-layer water (exact, r-RESPA 1:4 and
+short-layer step costs).
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--natoms N] [--no-cpu] [--no-hmc] [--no-water]
 """
 import argparse
 import ctypes
