@@ -879,7 +879,7 @@ k_pairs_mx(double* __restrict__ F, double* __restrict__ partials, const int4* __
         }
         flush(a);
       };
-      // two groups of four in flight per iteration; each group is still
+      // three groups of four in flight per iteration; each group is still
       // summed in FP32 and flushed to FP64 on its own
       int k = 0;
       for (; k + 12 <= n4; k += 12) {
