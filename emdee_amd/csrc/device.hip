@@ -137,7 +137,7 @@ struct Scaled {
 __device__ __forceinline__ void scale_slot(const Scaled& sc, const double* __restrict__ R, int s, int Npad) {
   double v[3] = { R[s] / sc.L, R[Npad + s] / sc.L, R[2 * Npad + s] / sc.L };
   int t = sc.type[s];
-  sc.Rs4[s] = make_double4(v[0], v[1], v[2], (double)t);
+  if (sc.Rs4) sc.Rs4[s] = make_double4(v[0], v[1], v[2], (double)t);
   if (sc.Q4) {
     unsigned int u[3];
     for (int x = 0; x < 3; x++) {
@@ -351,6 +351,7 @@ struct MixedCut {
   float lo, hi;     // FP32 band around the cutoff in fixed-point units^2
   double Rc2s;      // FP64 cutoff on scaled coordinates (the plain form's)
   double unit;      // L / 2^32
+  double L;
 };
 
 __global__ void k_scale4q(double4* __restrict__ Rs4, int4* __restrict__ Q4, float* __restrict__ qf,
@@ -366,6 +367,19 @@ __global__ void k_scale4q(double4* __restrict__ Rs4, int4* __restrict__ Q4, floa
 inline double band_rel(double Rc2s) {
   const double q = 1.0 / 4294967296.0;
   return 2e-6 + 4.0 * q / std::sqrt(Rc2s) + 3.0 * q * q / Rc2s;
+}
+
+// The same test from the unscaled positions: Rs = R / L as k_scale4 forms it.
+__device__ __noinline__ bool exact_inside_R(const double* __restrict__ R, int Npad, double L, int i, int j,
+                                           double Rc2s) {
+  double dx = R[i] / L - R[j] / L, dy = R[Npad + i] / L - R[Npad + j] / L, dz = R[2 * Npad + i] / L - R[2 * Npad + j] / L;
+  dx = dx - rint(dx); dy = dy - rint(dy); dz = dz - rint(dz);
+  double r2;
+  {
+#pragma clang fp contract(off)
+    r2 = dx * dx + dy * dy + dz * dz;
+  }
+  return r2 < Rc2s;
 }
 
 __device__ __noinline__ bool exact_inside(const double4* __restrict__ Rs4, int i, int j, double Rc2s) {
@@ -805,7 +819,7 @@ __device__ __forceinline__ void pair_body_mx(const DevPair& p, const DevCoul& cm
 template <bool ENERGY, bool UNIFORM, bool LJ, int TPA>
 __global__ void __launch_bounds__(BLOCK)
 k_pairs_mx(double* __restrict__ F, double* __restrict__ partials, const int4* __restrict__ Q4,
-           const double4* __restrict__ Rs4, const int* __restrict__ nbr, const int* __restrict__ nnb,
+           const double* __restrict__ R, const int* __restrict__ nbr, const int* __restrict__ nnb,
            const float* __restrict__ qf, const DevPair* __restrict__ pairs, const DevCoul* __restrict__ coulp,
            int ntypes, MixedCut mc, int N, int Npad, double* __restrict__ eatom) {
   int blk = xcd_block();
@@ -827,7 +841,7 @@ k_pairs_mx(double* __restrict__ F, double* __restrict__ partials, const int4* __
       float dz = (float)(int)((unsigned int)qi.z - (unsigned int)qj.z);
       float r2 = dx * dx + dy * dy + dz * dz;
       bool in = r2 < mc.lo;
-      if (!in && r2 < mc.hi) in = exact_inside(Rs4, i, j, mc.Rc2s);
+      if (!in && r2 < mc.hi) in = exact_inside_R(R, Npad, mc.L, i, j, mc.Rc2s);
       if (in) {
         float invR2 = mc.K * __builtin_amdgcn_rcpf(r2);
         if (LJ) {
@@ -866,7 +880,7 @@ k_pairs_mx(double* __restrict__ F, double* __restrict__ partials, const int4* __
           float r2 = dx * dx + dy * dy + dz * dz;
           if (r2 == 0.0f) r2 = mc.hi;
           bool in = r2 < mc.lo;
-          if (r2 >= mc.lo && r2 < mc.hi) in = exact_inside(Rs4, i, j[u], mc.Rc2s);
+          if (r2 >= mc.lo && r2 < mc.hi) in = exact_inside_R(R, Npad, mc.L, i, j[u], mc.Rc2s);
           float invR2 = mc.K * __builtin_amdgcn_rcpf(r2);
           float sr2 = c2 * invR2, sr6 = sr2 * sr2 * sr2;
           float W = c1 * sr6 * (sr6 + sr6 - 1.0f);
@@ -966,15 +980,15 @@ k_pairs_mx(double* __restrict__ F, double* __restrict__ partials, const int4* __
 // ---- integrator -------------------------------------------------------------
 // boost, EmDeeData.f90:864-896 (free atoms): P = CP*P + CF*F;
 // kinetic_energies, EmDeeData.f90:900-922: 2KE_x = sum invMass*P_x^2.
-template <bool KE>
+template <bool KE, bool UNIF>
 __global__ void k_boost(double* __restrict__ P, double* __restrict__ partials, const double* __restrict__ F,
                         const double* __restrict__ invMass, const int* __restrict__ freeFlag,
-                        double CP, double CF, int translate, int N, int Npad) {
+                        double CP, double CF, int translate, int N, int Npad, double invMass0) {
   int blk = blockIdx.x;
   int s = blk * BLOCK + threadIdx.x;
   double v[3] = { 0, 0, 0 };
-  if (s < N && freeFlag[s]) {
-    double im = invMass[s];
+  if (s < N && (UNIF || freeFlag[s])) {
+    double im = UNIF ? invMass0 : invMass[s];
 #pragma unroll
     for (int x = 0; x < 3; x++) {
       double p = P[x * Npad + s];
@@ -988,11 +1002,12 @@ __global__ void k_boost(double* __restrict__ P, double* __restrict__ partials, c
   }
 }
 // move, EmDeeData.f90:823-860 (free atoms): R = CR*R + CP*P*invMass.
+template <bool UNIF>
 __global__ void k_displace(double* __restrict__ R, const double* __restrict__ P, const double* __restrict__ invMass,
-                           const int* __restrict__ freeFlag, double CR, double CP, int N, int Npad) {
+                           const int* __restrict__ freeFlag, double CR, double CP, int N, int Npad, double invMass0) {
   int s = blockIdx.x * BLOCK + threadIdx.x;
-  if (s >= N || !freeFlag[s]) return;
-  double im = invMass[s];
+  if (s >= N || (!UNIF && !freeFlag[s])) return;
+  double im = UNIF ? invMass0 : invMass[s];
 #pragma unroll
   for (int x = 0; x < 3; x++) R[x * Npad + s] = CR * R[x * Npad + s] + CP * P[x * Npad + s] * im;
 }
@@ -1122,6 +1137,9 @@ Device* dev_create(const DeviceSetup& s) {
   std::vector<double> im(s.N), q(s.N);
   std::vector<int> free_(s.N), chg(s.N);
   for (int i = 0; i < s.N; i++) { im[i] = s.invMass[i]; q[i] = s.charge[i]; free_[i] = s.freeFlag[i]; chg[i] = std::fabs(s.charge[i]) > 2.220446049250313e-16; }
+  d->uniformFree = s.N > 0;
+  for (int i = 0; i < s.N; i++) d->uniformFree = d->uniformFree && free_[i] == 1 && im[i] == im[0];
+  d->invMass0 = s.N > 0 ? im[0] : 1.0;
   HIP_CHECK(hipMemcpyAsync(d->type.p, s.type.data(), sizeof(int) * s.N, hipMemcpyHostToDevice, d->stream));
   HIP_CHECK(hipMemcpyAsync(d->body.p, s.body.data(), sizeof(int) * s.N, hipMemcpyHostToDevice, d->stream));
   HIP_CHECK(hipMemcpyAsync(d->invMass.p, im.data(), sizeof(double) * s.N, hipMemcpyHostToDevice, d->stream));
@@ -1251,7 +1269,10 @@ bool dev_needs_rebuild(Device* d, double skinSq, double L, bool mixed) {
   if (N == 0) return false;
   int g = div_up(N, BLOCK);
   if (mixed) { d->Q4.alloc(d->Npad); d->qf.alloc(d->Npad); }
-  Scaled sc{ d->Rs4.p, mixed ? d->Q4.p : nullptr, d->qf.p, d->type.p, d->charge.p, d->chargedFlag.p, L };
+  // the mixed form reads fixed-point positions only (its FP64 band test
+  // divides R by L itself)
+  Scaled sc{ mixed ? nullptr : d->Rs4.p, mixed ? d->Q4.p : nullptr, d->qf.p, d->type.p, d->charge.p,
+             d->chargedFlag.p, L };
   hipLaunchKernelGGL(k_disp_a, dim3(g), dim3(BLOCK), 0, d->stream, d->d2.p, d->partials.p, d->R.p, d->R0.p, d->perm.p,
                      N, d->Npad, sc);
   hipLaunchKernelGGL(k_disp_a_reduce, dim3(1), dim3(BLOCK), 0, d->stream, d->trig.p, d->partials.p, g);
@@ -1429,9 +1450,10 @@ void dev_compute_pairs(Device* d, const PairCall& c, ForceSums* out) {
     mc.hi = (float)(rc2u * (1.0 + band_rel(c.Rc2)));
     mc.Rc2s = c.Rc2;
     mc.unit = c.L / two32;
+    mc.L = c.L;
 #define LAUNCH_MX(E, U, J, T)                                                                                      \
   hipLaunchKernelGGL((k_pairs_mx<E, U, J, T>), dim3(gp), dim3(BLOCK), 0, d->stream, F, d->partials.p, d->Q4.p,     \
-                     d->Rs4.p, d->nbr.p, d->nnb.p, d->qf.p, pt, cp, c.ntypes, mc, N, Np, ea)
+                     d->R.p, d->nbr.p, d->nnb.p, d->qf.p, pt, cp, c.ntypes, mc, N, Np, ea)
 #define GENERIC_MX(E, U)                                                                                           \
   do {                                                                                                            \
     if (tpa == 16) LAUNCH_MX(E, U, false, 16); else if (tpa == 4) LAUNCH_MX(E, U, false, 4);                      \
@@ -1491,14 +1513,17 @@ void dev_boost(Device* d, int layer, double CP, double CF, bool translate, doubl
   int g = div_up(N, BLOCK);
   const double* F = d->F.p + 3 * (size_t)Np * layer;
   if (twoKE) {
-    hipLaunchKernelGGL(k_boost<true>, dim3(g), dim3(BLOCK), 0, d->stream, d->P.p, d->partials.p, F, d->invMass.p, d->freeFlag.p, CP, CF, (int)translate, N, Np);
+    hipLaunchKernelGGL((d->uniformFree ? k_boost<true, true> : k_boost<true, false>), dim3(g), dim3(BLOCK), 0, d->stream,
+                       d->P.p, d->partials.p, F, d->invMass.p, d->freeFlag.p, CP, CF, (int)translate, N, Np,
+                       d->invMass0);
     unsigned long long seq = ++d->seq;
     hipLaunchKernelGGL(k_reduce_post<3>, dim3(1), dim3(BLOCK), 0, d->stream, d->hpostDev, d->hseqDev, seq, d->partials.p, g);
     launch_check("boost");
     post_wait(d, seq);
     for (int x = 0; x < 3; x++) twoKE[x] = d->hpost[x];
   } else if (translate) {
-    hipLaunchKernelGGL(k_boost<false>, dim3(g), dim3(BLOCK), 0, d->stream, d->P.p, d->partials.p, F, d->invMass.p, d->freeFlag.p, CP, CF, 1, N, Np);
+    hipLaunchKernelGGL((d->uniformFree ? k_boost<false, true> : k_boost<false, false>), dim3(g), dim3(BLOCK), 0,
+                       d->stream, d->P.p, d->partials.p, F, d->invMass.p, d->freeFlag.p, CP, CF, 1, N, Np, d->invMass0);
   }
   launch_check("boost");
 }
@@ -1508,7 +1533,8 @@ void dev_kinetic(Device* d, double* twoKE) { dev_boost(d, 0, 1.0, 0.0, false, tw
 void dev_displace(Device* d, double CR, double CP) {
   int N = d->N;
   if (N == 0) return;
-  hipLaunchKernelGGL(k_displace, dim3(div_up(N, BLOCK)), dim3(BLOCK), 0, d->stream, d->R.p, d->P.p, d->invMass.p, d->freeFlag.p, CR, CP, N, d->Npad);
+  hipLaunchKernelGGL((d->uniformFree ? k_displace<true> : k_displace<false>), dim3(div_up(N, BLOCK)), dim3(BLOCK), 0,
+                     d->stream, d->R.p, d->P.p, d->invMass.p, d->freeFlag.p, CR, CP, N, d->Npad, d->invMass0);
   launch_check("displace");
 }
 

@@ -162,7 +162,9 @@ struct Device {
   DBuf<uint8_t> interact;
   DBuf<int> exOff, exItem;
   int nexcl = 0;
-  bool allInteract = false;   // every type pair has a model (interaction table all ones)
+  bool allInteract = false;
+  bool uniformFree = false;   // every atom free and of one mass: the integrator skips the per-atom reads
+  double invMass0 = 1.0;   // every type pair has a model (interaction table all ones)
   std::vector<int> hostType;
   // reductions
   DBuf<double> partials, result;
