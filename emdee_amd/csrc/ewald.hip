@@ -31,7 +31,7 @@ namespace {
 const double TWO_PI = 2.0 * 3.14159265358979324;   // 2 pi with global.f90:38
 constexpr int EW_BLOCK = 64;                        // one wavefront per block
 constexpr int EW_ATOMS = EWALD_GROUP / EW_BLOCK;    // charges per thread in k_ew_structure
-constexpr int EW_TARGET_WAVES = 4096;               // k-range split target
+constexpr int EW_TARGET_WAVES = 16384;              // k-range split target
 }  // namespace
 
 struct EwaldState {
@@ -193,11 +193,12 @@ __global__ void __launch_bounds__(EW_BLOCK) k_ew_forces(double* __restrict__ Fpa
   const double2* sg = sigma + (size_t)t * nvecs;
   double qj = q[j];
   size_t zoff = (2 * (size_t)(nm + 1) + 1) * nslots;   // G3(1) = e^{2 pi i s_z}
-  double2 z = T[zoff + j], v = make_double2(0.0, 0.0);
+  double2 z = T[zoff + j], va = make_double2(0.0, 0.0), vb = va;
   double fx = 0.0, fy = 0.0, fz = 0.0, er = 0.0;
-  for (int k = k0; k < k1; k++) {
-    int4 n = kn[k];
-    v = (k == k0 || n.w) ? row_start(T, qj, n, j, nm, nslots) : cmul(v, z);
+  // two halves of the k range advance together: two independent recursions
+  // in flight per lane
+  const int kh = k0 + (k1 - k0 + 1) / 2;
+  auto term = [&](int k, const double2& v) {
     double2 s = sg[k];
     double4 w = kv[k];
     double c = s.x * v.y - v.x * s.y;
@@ -205,6 +206,17 @@ __global__ void __launch_bounds__(EW_BLOCK) k_ew_forces(double* __restrict__ Fpa
     fy += w.y * c;
     fz += w.z * c;
     er += w.w * (s.x * v.x + s.y * v.y);
+  };
+  for (int t = 0; k0 + t < kh; t++) {
+    int ka = k0 + t, kb = kh + t;
+    int4 na = kn[ka];
+    va = (t == 0 || na.w) ? row_start(T, qj, na, j, nm, nslots) : cmul(va, z);
+    if (kb < k1) {
+      int4 nb = kn[kb];
+      vb = (t == 0 || nb.w) ? row_start(T, qj, nb, j, nm, nslots) : cmul(vb, z);
+    }
+    term(ka, va);
+    if (kb < k1) term(kb, vb);
   }
   size_t o = (size_t)blockIdx.y * 4 * nslots + j;
   Fpart[o] = fx;
@@ -225,7 +237,21 @@ __global__ void k_ew_force_add(double* __restrict__ F, double* __restrict__ eato
   int i = item[j];
   if (i < 0) return;
   double f[4] = { 0.0, 0.0, 0.0, 0.0 };
-  for (int y = 0; y < ks; y++)
+  // the partials of four k ranges are loaded together (sixteen loads in
+  // flight), then added in range order
+  int y = 0;
+  for (; y + 4 <= ks; y += 4) {
+    double p[4][4];
+#pragma unroll
+    for (int u = 0; u < 4; u++)
+#pragma unroll
+      for (int x = 0; x < 4; x++) p[u][x] = Fpart[((size_t)(y + u) * 4 + x) * nslots + j];
+#pragma unroll
+    for (int u = 0; u < 4; u++)
+#pragma unroll
+      for (int x = 0; x < 4; x++) f[x] += p[u][x];
+  }
+  for (; y < ks; y++)
     for (int x = 0; x < 4; x++) f[x] += Fpart[((size_t)y * 4 + x) * nslots + j];
   int s = iperm[i];
   for (int x = 0; x < 3; x++) F[x * (size_t)Npad + s] += f[x];
