@@ -106,9 +106,15 @@ __device__ __forceinline__ double2 row_start(const double2* __restrict__ T, doub
 
 // Partial structure factors: Spart[g*nvecs + k] = sum over the EWALD_GROUP
 // slots of group g of q e^{ik.r}.  Grid (ngroups, ksS), one wavefront each.
+// Each lane sums its EW_ATOMS charges for EW_KB consecutive k-vectors in
+// registers; the 64 lane sums of those k are then added through LDS, one
+// lane per (k, real/imaginary part) in lane order, instead of a shuffle
+// tree per k.
+constexpr int EW_KB = 16;
 __global__ void __launch_bounds__(EW_BLOCK) k_ew_structure(double2* __restrict__ Spart, const double2* __restrict__ T,
                                                            const double* __restrict__ q, const int4* __restrict__ kn,
                                                            int nm, int nslots, int nvecs, int kch) {
+  __shared__ double red[2 * EW_KB][EW_BLOCK + 1];
   int g = blockIdx.x, lane = threadIdx.x;
   int k0 = blockIdx.y * kch, k1 = min(nvecs, k0 + kch);
   int j[EW_ATOMS];
@@ -122,21 +128,35 @@ __global__ void __launch_bounds__(EW_BLOCK) k_ew_structure(double2* __restrict__
     z[a] = T[zoff + j[a]];
     v[a] = make_double2(0.0, 0.0);
   }
-  for (int k = k0; k < k1; k++) {
-    int4 n = kn[k];
-    if (k == k0 || n.w) {
+  double* out = reinterpret_cast<double*>(Spart + (size_t)g * nvecs);
+  for (int kb = k0; kb < k1; kb += EW_KB) {
 #pragma unroll
-      for (int a = 0; a < EW_ATOMS; a++) v[a] = row_start(T, qa[a], n, j[a], nm, nslots);
-    } else {
+    for (int t = 0; t < EW_KB; t++) {
+      int k = kb + t;
+      double sr = 0.0, si = 0.0;
+      if (k < k1) {
+        int4 n = kn[k];
+        if (k == k0 || n.w) {
 #pragma unroll
-      for (int a = 0; a < EW_ATOMS; a++) v[a] = cmul(v[a], z[a]);
+          for (int a = 0; a < EW_ATOMS; a++) v[a] = row_start(T, qa[a], n, j[a], nm, nslots);
+        } else {
+#pragma unroll
+          for (int a = 0; a < EW_ATOMS; a++) v[a] = cmul(v[a], z[a]);
+        }
+#pragma unroll
+        for (int a = 0; a < EW_ATOMS; a++) { sr += v[a].x; si += v[a].y; }
+      }
+      red[2 * t][lane] = sr;
+      red[2 * t + 1][lane] = si;
     }
-    double sr = 0.0, si = 0.0;
-#pragma unroll
-    for (int a = 0; a < EW_ATOMS; a++) { sr += v[a].x; si += v[a].y; }
-    sr = wave_sum(sr);
-    si = wave_sum(si);
-    if (lane == 0) Spart[(size_t)g * nvecs + k] = make_double2(sr, si);
+    __syncthreads();
+    if (lane < 2 * EW_KB && kb + (lane >> 1) < k1) {
+      double s0 = 0.0, s1 = 0.0;
+#pragma unroll 8
+      for (int l = 0; l < EW_BLOCK; l += 2) { s0 += red[lane][l]; s1 += red[lane][l + 1]; }
+      out[2 * (size_t)kb + lane] = s0 + s1;
+    }
+    __syncthreads();
   }
 }
 
@@ -207,6 +227,7 @@ __global__ void __launch_bounds__(EW_BLOCK) k_ew_forces(double* __restrict__ Fpa
     fz += w.z * c;
     er += w.w * (s.x * v.x + s.y * v.y);
   };
+#pragma unroll 4
   for (int t = 0; k0 + t < kh; t++) {
     int ka = k0 + t, kb = kh + t;
     int4 na = kn[ka];
