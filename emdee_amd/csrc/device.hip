@@ -465,25 +465,41 @@ __device__ __forceinline__ void scan_range(int a, int b, int i, double xi, doubl
     return;
   }
   if (SIMPLE) {
-    auto testq = [&](int j, int4 qj) {
-      float dx = (float)(int)((unsigned int)qi.x - (unsigned int)qj.x);
-      float dy = (float)(int)((unsigned int)qi.y - (unsigned int)qj.y);
-      float dz = (float)(int)((unsigned int)qi.z - (unsigned int)qj.z);
-      float r2 = dx * dx + dy * dy + dz * dz;
-      bool odd = ((qi.w | qj.w) & Q4_NONFINITE) != 0;
-      bool in = r2 < lo && !odd;
-      if (!in && (r2 < hi || odd)) in = exact_inside(Rs4, i, j, xRc2);
-      if (in && j != i) {
-        if (cnt < cap) nbr[(size_t)cnt * Npad + i] = j;
-        cnt++;
+    // rows are short (a few candidates), so every row is taken four
+    // candidates at a time with validity masks; the rare band test is
+    // hoisted out of the per-candidate path
+    int* __restrict__ col = nbr + i;
+    for (int j = a; j < b; j += 4) {
+      int4 q[4];
+#pragma unroll
+      for (int u = 0; u < 4; u++) q[u] = Q4[min(j + u, b - 1)];
+      bool in[4], band = false;
+#pragma unroll
+      for (int u = 0; u < 4; u++) {
+        float dx = (float)(int)((unsigned int)qi.x - (unsigned int)q[u].x);
+        float dy = (float)(int)((unsigned int)qi.y - (unsigned int)q[u].y);
+        float dz = (float)(int)((unsigned int)qi.z - (unsigned int)q[u].z);
+        float r2 = dx * dx + dy * dy + dz * dz;
+        bool odd = ((qi.w | q[u].w) & Q4_NONFINITE) != 0;
+        bool fast = r2 < lo && !odd;
+        bool valid = j + u < b;
+        in[u] = valid && fast && j + u != i;
+        band = band || (valid && !fast && (r2 < hi || odd));
       }
-    };
-    int j = a;
-    for (; j + 3 < b; j += 4) {
-      int4 q0 = Q4[j], q1 = Q4[j + 1], q2 = Q4[j + 2], q3 = Q4[j + 3];
-      testq(j, q0); testq(j + 1, q1); testq(j + 2, q2); testq(j + 3, q3);
+      if (band) {
+#pragma unroll
+        for (int u = 0; u < 4; u++) {
+          if (j + u < b && !in[u] && j + u != i) in[u] = exact_inside(Rs4, i, j + u, xRc2);
+        }
+      }
+#pragma unroll
+      for (int u = 0; u < 4; u++) {
+        if (in[u]) {
+          if (cnt < cap) col[(size_t)cnt * Npad] = j + u;
+          cnt++;
+        }
+      }
     }
-    for (; j < b; j++) testq(j, Q4[j]);
     return;
   }
   auto test = [&](int j, double4 rj) {
