@@ -561,8 +561,10 @@ k_build_list(int* __restrict__ nbr, int* __restrict__ nnb, int* __restrict__ fla
       if (f > 1.5) f -= M;                  // the coordinate wrapped to cell 0
       return fmin(fmax(f, 0.0), 1.0);
     };
-    double fx = frac(ri.x, cx), fy = frac(ri.y, cy), fz = frac(ri.z, cz);
-    double rcell2 = xRc2 * (double)M * (double)M * (1.0 + 1e-6) + 1e-9;
+    // the culling arithmetic is FP32 with margins (1e-5 relative, 1e-4
+    // cells) far above its rounding, so it can only keep extra cells
+    const float fx = (float)frac(ri.x, cx), fy = (float)frac(ri.y, cy), fz = (float)frac(ri.z, cz);
+    const float rcell2 = (float)(xRc2 * (double)M * (double)M * (1.0 + 1e-5) + 1e-5);
     // slot ranges of row r of the (dy, dz) offsets: cells xa..xb of the row,
     // split in two where the row wraps; empty when the row is culled
     struct Ranges { int a0, b0, a1, b1; };
@@ -570,12 +572,12 @@ k_build_list(int* __restrict__ nbr, int* __restrict__ nnb, int* __restrict__ fla
     auto row = [&](int r) {
       Ranges q = { 0, 0, 0, 0 };
       int dz = r / W - CELL_REACH, dy = r % W - CELL_REACH;
-      double gz = fmax(dz > 0 ? dz - fz : (dz < 0 ? fz - dz - 1 : 0.0), 0.0);
-      double gy = fmax(dy > 0 ? dy - fy : (dy < 0 ? fy - dy - 1 : 0.0), 0.0);
-      double rem = rcell2 - gy * gy - gz * gz;
-      if (rem <= 0.0) return q;
-      double e = sqrt(rem) + 1e-6;
-      int xa = cx + max((int)floor(fx - e), -CELL_REACH), xb = cx + min((int)floor(fx + e), CELL_REACH);
+      float gz = fmaxf(dz > 0 ? dz - fz : (dz < 0 ? fz - dz - 1 : 0.0f), 0.0f);
+      float gy = fmaxf(dy > 0 ? dy - fy : (dy < 0 ? fy - dy - 1 : 0.0f), 0.0f);
+      float rem = rcell2 - gy * gy - gz * gz;
+      if (rem <= 0.0f) return q;
+      float e = sqrtf(rem) + 1e-4f;
+      int xa = cx + max((int)floorf(fx - e), -CELL_REACH), xb = cx + min((int)floorf(fx + e), CELL_REACH);
       int y = cy + dy; y = y < 0 ? y + M : (y >= M ? y - M : y);
       int z = cz + dz; z = z < 0 ? z + M : (z >= M ? z - M : z);
       int base = M * (y + M * z);
