@@ -274,7 +274,7 @@ void compute_forces(tEmDee* md) {
   enum { PAIR_ = 0, COUL_ = 1, LONG_ = 2, BOND_ = 3, ANGLE_ = 4 };
   // handle_neighbor_lists: neighbor_lists.f90:175-195
   md->Time.Neighbor -= wtime();
-  if (dev_needs_rebuild(me->dev, me->skinSq, me->L, me->precision != 0)) {
+  if (dev_needs_rebuild(me->dev, me->skinSq, me->L, me->precision != 0, me->layer - 1)) {
     dev_rebuild(me->dev, me->L, me->xRc, std::sqrt(me->xInRcSq));
     md->Builds += 1;
   }

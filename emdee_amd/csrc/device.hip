@@ -837,8 +837,9 @@ __device__ __forceinline__ void pair_body_mx(const DevPair& p, const DevCoul& cm
 template <bool ENERGY, bool UNIFORM, bool LJ, int TPA>
 __global__ void __launch_bounds__(BLOCK)
 k_pairs_mx(double* __restrict__ F, double* __restrict__ partials, const int4* __restrict__ Q4,
-           const double* __restrict__ R, const int* __restrict__ nbr, const int* __restrict__ nnb,
-           const float* __restrict__ qf, const DevPair* __restrict__ pairs, const DevCoul* __restrict__ coulp,
+           const double* __restrict__ R, const double4* __restrict__ Rs4, const int* __restrict__ nbr,
+           const int* __restrict__ nnb, const float* __restrict__ qf, const DevPair* __restrict__ pairs,
+           const DevCoul* __restrict__ coulp,
            int ntypes, MixedCut mc, int N, int Npad, double* __restrict__ eatom) {
   int blk = xcd_block();
   int gt = blk * BLOCK + threadIdx.x;
@@ -859,7 +860,7 @@ k_pairs_mx(double* __restrict__ F, double* __restrict__ partials, const int4* __
       float dz = (float)(int)((unsigned int)qi.z - (unsigned int)qj.z);
       float r2 = dx * dx + dy * dy + dz * dz;
       bool in = r2 < mc.lo;
-      if (!in && r2 < mc.hi) in = exact_inside_R(R, Npad, mc.L, i, j, mc.Rc2s);
+      if (!in && r2 < mc.hi) in = exact_inside(Rs4, i, j, mc.Rc2s);
       if (in) {
         float invR2 = mc.K * __builtin_amdgcn_rcpf(r2);
         if (LJ) {
@@ -1282,14 +1283,23 @@ void dev_add_forces(Device* d, int layer, const double* F) {
   HIP_CHECK(hipStreamSynchronize(d->stream));
 }
 
-bool dev_needs_rebuild(Device* d, double skinSq, double L, bool mixed) {
+// The LJ-specialized pair kernels serve the layer: one type, no charges,
+// pair_lj_cut with no modifier or an energy shift.
+static bool lj_path(const Device* d, int layer) {
+  if (d->ntypes != 1 || d->anyCharged || d->hostPairs.size() <= (size_t)layer) return false;
+  const DevPair& hp = d->hostPairs[(size_t)layer];
+  return hp.kind == PAIR_LJ_CUT && (hp.modifier == MOD_NONE || hp.modifier == MOD_SHIFTED);
+}
+
+bool dev_needs_rebuild(Device* d, double skinSq, double L, bool mixed, int layer) {
   int N = d->N;
   if (N == 0) return false;
   int g = div_up(N, BLOCK);
   if (mixed) { d->Q4.alloc(d->Npad); d->qf.alloc(d->Npad); }
-  // the mixed form reads fixed-point positions only (its FP64 band test
-  // divides R by L itself)
-  Scaled sc{ mixed ? nullptr : d->Rs4.p, mixed ? d->Q4.p : nullptr, d->qf.p, d->type.p, d->charge.p,
+  // the mixed LJ kernel reads fixed-point positions only (its FP64 band
+  // test divides R by L itself)
+  bool needRs4 = !mixed || !lj_path(d, layer);
+  Scaled sc{ needRs4 ? d->Rs4.p : nullptr, mixed ? d->Q4.p : nullptr, d->qf.p, d->type.p, d->charge.p,
              d->chargedFlag.p, L };
   hipLaunchKernelGGL(k_disp_a, dim3(g), dim3(BLOCK), 0, d->stream, d->d2.p, d->partials.p, d->R.p, d->R0.p, d->perm.p,
                      N, d->Npad, sc);
@@ -1445,8 +1455,7 @@ void dev_compute_pairs(Device* d, const PairCall& c, ForceSums* out) {
   const DevCoul* cp = d->couls.p + c.layer;
   if (d->profiling) HIP_CHECK(hipEventRecord(d->ev0, d->stream));
   bool U = c.ntypes == 1;
-  const DevPair& hp = d->hostPairs[(size_t)c.layer * c.ntypes * c.ntypes];
-  bool LJ = U && !d->anyCharged && hp.kind == PAIR_LJ_CUT && (hp.modifier == MOD_NONE || hp.modifier == MOD_SHIFTED);
+  bool LJ = lj_path(d, c.layer);
   // threads per atom: enough wavefronts to fill the chip (at least 2^18
   // lanes) and about 24 list entries per lane, so that long lists (water at
   // rc 10 A holds ~700 entries per atom) are spread over many waves
@@ -1471,7 +1480,7 @@ void dev_compute_pairs(Device* d, const PairCall& c, ForceSums* out) {
     mc.L = c.L;
 #define LAUNCH_MX(E, U, J, T)                                                                                      \
   hipLaunchKernelGGL((k_pairs_mx<E, U, J, T>), dim3(gp), dim3(BLOCK), 0, d->stream, F, d->partials.p, d->Q4.p,     \
-                     d->R.p, d->nbr.p, d->nnb.p, d->qf.p, pt, cp, c.ntypes, mc, N, Np, ea)
+                     d->R.p, d->Rs4.p, d->nbr.p, d->nnb.p, d->qf.p, pt, cp, c.ntypes, mc, N, Np, ea)
 #define GENERIC_MX(E, U)                                                                                           \
   do {                                                                                                            \
     if (tpa == 16) LAUNCH_MX(E, U, false, 16); else if (tpa == 4) LAUNCH_MX(E, U, false, 4);                      \
