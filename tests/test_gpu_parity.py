@@ -207,3 +207,33 @@ def test_mixed_precision_error(gpu, oracle):
     print("mixed-form force RMS error", err)
     assert err < 1e-5
     assert abs(mg.Energy.Potential - mo.Energy.Potential) < 1e-6 * abs(mo.Energy.Potential)
+
+
+@pytest.mark.parametrize("natoms", [1, 2, 7])
+@pytest.mark.parametrize("mixed", [0, 1])
+def test_tiny_and_dilute_systems(gpu, oracle, natoms, mixed):
+    """Edge cases of the list and pair kernels: one atom (an empty list),
+    two atoms, and a dilute box where most atoms have no neighbour; then a
+    few MD steps with identical rebuild instants."""
+    L = 12.0
+    R = pu.random_box(natoms, L, seed=20 + natoms, min_dist=1.0)
+    mg = pu.build(gpu, R, L, 2.5, 0.3, [lj()])
+    mo = pu.build(oracle, R, L, 2.5, 0.3, [lj()])
+    gpu.EmDeeGPU_set_precision(mg, mixed)
+    assert pu.gpu_pairs(gpu, mg) == pu.oracle_pairs(oracle, mo) == pu.brute_pairs(R, L, 2.8)
+    compare_state(gpu, oracle, mg, mo, natoms, ftol=1e-6 if mixed else FTOL)
+    for lib, md in ((gpu, mg), (oracle, mo)):
+        lib.random_momenta(md, 1.0, False, 5)    # no momentum adjustment: one atom has no free dof
+        bench_like_steps(lib, md, 20, 0.005)
+    assert mg.Builds == mo.Builds
+    tol = 1e-6 if mixed else 1e-10
+    assert np.max(np.abs(gpu.download(mg, "coordinates", (natoms, 3)) -
+                         oracle.download(mo, "coordinates", (natoms, 3)))) < tol
+
+
+def bench_like_steps(lib, md, n, dt):
+    for _ in range(n):
+        md.Options.Compute = True
+        lib.boost(md, 1.0, 0.0, 0.5 * dt)
+        lib.displace(md, 1.0, 0.0, dt)
+        lib.boost(md, 1.0, 0.0, 0.5 * dt)
