@@ -1284,8 +1284,11 @@ void dev_add_forces(Device* d, int layer, const double* F) {
 }
 
 // The LJ-specialized pair kernels serve the layer: one type, no charges,
-// pair_lj_cut with no modifier or an energy shift.
+// pair_lj_cut with no modifier or an energy shift, and enough atoms to fill
+// the chip with one thread per atom (smaller systems take the generic kernels,
+// which split each list over 4 or 16 lanes).
 static bool lj_path(const Device* d, int layer) {
+  if (d->N < (1 << 17)) return false;
   if (d->ntypes != 1 || d->anyCharged || d->hostPairs.size() <= (size_t)layer) return false;
   const DevPair& hp = d->hostPairs[(size_t)layer];
   return hp.kind == PAIR_LJ_CUT && (hp.modifier == MOD_NONE || hp.modifier == MOD_SHIFTED);
@@ -1387,11 +1390,12 @@ void dev_rebuild(Device* d, double L, double xRc, double xInRc) {
     HIP_CHECK(hipMemsetAsync(d->flags.p, 0, sizeof(int) * 4, d->stream));
     if (d->profiling) HIP_CHECK(hipEventRecord(d->ev0, d->stream));
     bool simple = d->bodies.nb == 0 && d->nexcl == 0 && d->allInteract;
-    // lanes per atom: at least ~24 candidates per lane (the scanned region is
-    // ~2.6 times the list volume) and at most 2^21 lanes in all
+    // lanes per atom: as many as keep at most 2^21 lanes in all and at least
+    // four candidates per lane (the scanned region is ~2.6 times the list
+    // volume)
     double cand = 2.6 * vol;
     long n = N;
-    int S = (cand >= 16 * 24 && n * 16 <= (1L << 21)) ? 16 : ((cand >= 4 * 24 && n * 4 <= (1L << 21)) ? 4 : 1);
+    int S = (cand >= 16 * 4 && n * 16 <= (1L << 21)) ? 16 : ((cand >= 4 * 4 && n * 4 <= (1L << 21)) ? 4 : 1);
     auto kern = simple ? (S == 16 ? k_build_list<true, 16> : S == 4 ? k_build_list<true, 4> : k_build_list<true, 1>)
                        : (S == 16 ? k_build_list<false, 16> : S == 4 ? k_build_list<false, 4> : k_build_list<false, 1>);
     int gl = div_up((long)N * S, BLOCK);
