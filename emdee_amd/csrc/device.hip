@@ -18,6 +18,7 @@
 // All launches go to one stream per system.  Host syncs happen only where
 // the C ABI must return host-visible values (energies, virial, builds).
 #include <hip/hip_runtime.h>
+#include <hip/hip_ext.h>
 
 #include <algorithm>
 #include <atomic>
@@ -1388,7 +1389,6 @@ void dev_rebuild(Device* d, double L, double xRc, double xInRc) {
   for (;;) {
     d->nbr.alloc((size_t)d->cap * Np);
     HIP_CHECK(hipMemsetAsync(d->flags.p, 0, sizeof(int) * 4, d->stream));
-    if (d->profiling) HIP_CHECK(hipEventRecord(d->ev0, d->stream));
     bool simple = d->bodies.nb == 0 && d->nexcl == 0 && d->allInteract;
     // lanes per atom: as many as keep at most 2^21 lanes in all and at least
     // four candidates per lane (the scanned region is ~2.6 times the list
@@ -1399,11 +1399,13 @@ void dev_rebuild(Device* d, double L, double xRc, double xInRc) {
     auto kern = simple ? (S == 16 ? k_build_list<true, 16> : S == 4 ? k_build_list<true, 4> : k_build_list<true, 1>)
                        : (S == 16 ? k_build_list<false, 16> : S == 4 ? k_build_list<false, 4> : k_build_list<false, 1>);
     int gl = div_up((long)N * S, BLOCK);
-    hipLaunchKernelGGL(kern, dim3(gl), dim3(BLOCK), 0, d->stream, d->nbr.p, d->nnb.p, d->flags.p, d->partials.p,
-                       d->Rs4.p, d->key.p, d->cellStart.p, d->body.p, d->perm.p, d->interact.p, d->exOff.p, d->exItem.p,
-                       d->nexcl, d->ntypes, M, xRc2, d->cap, N, Np, d->Q4.p, lo, hi);
+    hipEvent_t e0 = d->profiling ? d->ev0 : nullptr, e1 = d->profiling ? d->ev1 : nullptr;
+    hipExtLaunchKernelGGL(kern, dim3(gl), dim3(BLOCK), 0, d->stream, e0, e1, 0, d->nbr.p, d->nnb.p, d->flags.p,
+                          d->partials.p, (const double4*)d->Rs4.p, (const int*)d->key.p, (const int*)d->cellStart.p,
+                          (const int*)d->body.p, (const int*)d->perm.p, (const uint8_t*)d->interact.p,
+                          (const int*)d->exOff.p, (const int*)d->exItem.p, d->nexcl, d->ntypes, M, xRc2, d->cap, N, Np,
+                          (const int4*)d->Q4.p, lo, hi);
     launch_check("build_list");
-    if (d->profiling) HIP_CHECK(hipEventRecord(d->ev1, d->stream));
     hipLaunchKernelGGL(k_reduce_partials<1>, dim3(1), dim3(BLOCK), 0, d->stream, d->result.p + 8, d->partials.p, gl);
     HIP_CHECK(hipMemcpyAsync(d->hresult + 8, d->result.p + 8, sizeof(double), hipMemcpyDeviceToHost, d->stream));
     int* hf = reinterpret_cast<int*>(d->hresult + 16);
@@ -1457,7 +1459,9 @@ void dev_compute_pairs(Device* d, const PairCall& c, ForceSums* out) {
   }
   const DevPair* pt = d->pairs.p + (size_t)c.layer * c.ntypes * c.ntypes;
   const DevCoul* cp = d->couls.p + c.layer;
-  if (d->profiling) HIP_CHECK(hipEventRecord(d->ev0, d->stream));
+  // with profiling on, the launch itself records the kernel's start and end
+  // (hipExtLaunchKernelGGL), so the time excludes the host's launch latency
+  hipEvent_t e0 = d->profiling ? d->ev0 : nullptr, e1 = d->profiling ? d->ev1 : nullptr;
   bool U = c.ntypes == 1;
   bool LJ = lj_path(d, c.layer);
   // threads per atom: enough wavefronts to fill the chip (at least 2^18
@@ -1483,8 +1487,10 @@ void dev_compute_pairs(Device* d, const PairCall& c, ForceSums* out) {
     mc.unit = c.L / two32;
     mc.L = c.L;
 #define LAUNCH_MX(E, U, J, T)                                                                                      \
-  hipLaunchKernelGGL((k_pairs_mx<E, U, J, T>), dim3(gp), dim3(BLOCK), 0, d->stream, F, d->partials.p, d->Q4.p,     \
-                     d->R.p, d->Rs4.p, d->nbr.p, d->nnb.p, d->qf.p, pt, cp, c.ntypes, mc, N, Np, ea)
+  hipExtLaunchKernelGGL((k_pairs_mx<E, U, J, T>), dim3(gp), dim3(BLOCK), 0, d->stream, e0, e1, 0, F,                \
+                        d->partials.p, (const int4*)d->Q4.p, (const double*)d->R.p, (const double4*)d->Rs4.p,     \
+                        (const int*)d->nbr.p, (const int*)d->nnb.p, (const float*)d->qf.p, pt, cp, c.ntypes, mc, N, \
+                        Np, ea)
 #define GENERIC_MX(E, U)                                                                                           \
   do {                                                                                                            \
     if (tpa == 16) LAUNCH_MX(E, U, false, 16); else if (tpa == 4) LAUNCH_MX(E, U, false, 4);                      \
@@ -1502,8 +1508,10 @@ void dev_compute_pairs(Device* d, const PairCall& c, ForceSums* out) {
 #undef LAUNCH_MX
   } else {
 #define LAUNCH_PAIRS(E, U, J, T)                                                                                    \
-  hipLaunchKernelGGL((k_pairs<E, false, U, J, T>), dim3(gp), dim3(BLOCK), 0, d->stream, F, d->partials.p, d->Rs4.p, \
-                     d->nbr.p, d->nnb.p, d->charge.p, d->chargedFlag.p, pt, cp, c.ntypes, c.L, c.invL2, c.Rc2, N, Np, ea)
+  hipExtLaunchKernelGGL((k_pairs<E, false, U, J, T>), dim3(gp), dim3(BLOCK), 0, d->stream, e0, e1, 0, F,           \
+                        d->partials.p, (const double4*)d->Rs4.p, (const int*)d->nbr.p, (const int*)d->nnb.p,       \
+                        (const double*)d->charge.p, (const int*)d->chargedFlag.p, pt, cp, c.ntypes, c.L, c.invL2,   \
+                        c.Rc2, N, Np, ea)
 #define GENERIC(E, U)                                                                                              \
   do {                                                                                                            \
     if (tpa == 16) LAUNCH_PAIRS(E, U, false, 16); else if (tpa == 4) LAUNCH_PAIRS(E, U, false, 4);                \
@@ -1521,7 +1529,6 @@ void dev_compute_pairs(Device* d, const PairCall& c, ForceSums* out) {
 #undef LAUNCH_PAIRS
   }
   launch_check("pairs");
-  if (d->profiling) HIP_CHECK(hipEventRecord(d->ev1, d->stream));
   unsigned long long seq = ++d->seq;
   hipLaunchKernelGGL(k_reduce_post<4>, dim3(1), dim3(BLOCK), 0, d->stream, d->hpostDev, d->hseqDev, seq, d->partials.p, gp);
   launch_check("reduce");
