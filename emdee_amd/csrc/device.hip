@@ -147,7 +147,7 @@ __device__ __forceinline__ void scale_slot(const Scaled& sc, const double* __res
     }
     bool fin = isfinite(v[0]) && isfinite(v[1]) && isfinite(v[2]);
     sc.Q4[s] = make_int4((int)u[0], (int)u[1], (int)u[2], fin ? t : (t | Q4_NONFINITE));
-    sc.qf[s] = sc.charged[s] ? (float)sc.charge[s] : 0.0f;
+    if (sc.qf) sc.qf[s] = sc.charged[s] ? (float)sc.charge[s] : 0.0f;
   }
 }
 
@@ -1303,7 +1303,8 @@ bool dev_needs_rebuild(Device* d, double skinSq, double L, bool mixed, int layer
   // the mixed LJ kernel reads fixed-point positions only (its FP64 band
   // test divides R by L itself)
   bool needRs4 = !mixed || !lj_path(d, layer);
-  Scaled sc{ needRs4 ? d->Rs4.p : nullptr, mixed ? d->Q4.p : nullptr, d->qf.p, d->type.p, d->charge.p,
+  // (the FP32 charges change only with the slot order: the list build writes them)
+  Scaled sc{ needRs4 ? d->Rs4.p : nullptr, mixed ? d->Q4.p : nullptr, nullptr, d->type.p, d->charge.p,
              d->chargedFlag.p, L };
   hipLaunchKernelGGL(k_disp_a, dim3(g), dim3(BLOCK), 0, d->stream, d->d2.p, d->partials.p, d->R.p, d->R0.p, d->perm.p,
                      N, d->Npad, sc);
