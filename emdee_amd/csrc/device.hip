@@ -187,6 +187,7 @@ __global__ void k_disp_a_reduce(double* __restrict__ trig, const double* __restr
   __shared__ double smax[BLOCK];
   __shared__ double sidx[BLOCK];
   double v = -1.0, idx = 2147483647.0;
+#pragma unroll 8
   for (int b = threadIdx.x; b < nb; b += BLOCK) {
     double ov = part[2 * b], oi = part[2 * b + 1];
     if (ov > v || (ov == v && oi < idx)) { v = ov; idx = oi; }
@@ -224,6 +225,7 @@ __global__ void k_disp_c_reduce(double* __restrict__ trig, const double* __restr
                                 unsigned long long seq) {
   __shared__ double smax[BLOCK];
   double v = 0.0;
+#pragma unroll 8
   for (int b = threadIdx.x; b < nb; b += BLOCK) v = fmax(v, part[b]);
   smax[threadIdx.x] = v;
   __syncthreads();

@@ -66,6 +66,7 @@ __global__ void k_reduce_partials(double* __restrict__ out, const double* __rest
   double v[NV];
 #pragma unroll
   for (int k = 0; k < NV; k++) v[k] = 0;
+#pragma unroll 8
   for (int b = threadIdx.x; b < nblocks; b += BLOCK) {
 #pragma unroll
     for (int k = 0; k < NV; k++) v[k] += partials[NV * b + k];
@@ -91,6 +92,7 @@ __global__ void k_reduce_post(double* __restrict__ host, unsigned long long* __r
   double v[NV];
 #pragma unroll
   for (int k = 0; k < NV; k++) v[k] = 0;
+#pragma unroll 8
   for (int b = threadIdx.x; b < nblocks; b += BLOCK) {
 #pragma unroll
     for (int k = 0; k < NV; k++) v[k] += partials[NV * b + k];

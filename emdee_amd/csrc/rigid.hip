@@ -606,6 +606,7 @@ template <int NV>
 __global__ void k_body_reduce(double* __restrict__ out, const double* __restrict__ part, int nblocks) {
   double v[NV];
   for (int k = 0; k < NV; k++) v[k] = 0;
+#pragma unroll 8
   for (int b = threadIdx.x; b < nblocks; b += BLOCK)
     for (int k = 0; k < NV; k++) v[k] += part[NV * b + k];
   block_sum<NV>(v);
