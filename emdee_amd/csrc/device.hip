@@ -183,27 +183,25 @@ __global__ void k_disp_a(double* __restrict__ d2, double* __restrict__ part, con
     part[2 * blockIdx.x + 1] = (double)idx;
   }
 }
+// (max, first index) of a set of pairs: the larger value wins, ties go to the
+// smaller index, so any combination order gives the same pair.
+__device__ __forceinline__ void max_first(double& v, double& idx, double ov, double oi) {
+  if (ov > v || (ov == v && oi < idx)) { v = ov; idx = oi; }
+}
 __global__ void k_disp_a_reduce(double* __restrict__ trig, const double* __restrict__ part, int nb) {
-  __shared__ double smax[BLOCK];
-  __shared__ double sidx[BLOCK];
+  __shared__ double smax[BLOCK / 64], sidx[BLOCK / 64];
   double v = -1.0, idx = 2147483647.0;
 #pragma unroll 8
-  for (int b = threadIdx.x; b < nb; b += BLOCK) {
-    double ov = part[2 * b], oi = part[2 * b + 1];
-    if (ov > v || (ov == v && oi < idx)) { v = ov; idx = oi; }
-  }
-  smax[threadIdx.x] = v; sidx[threadIdx.x] = idx;
+  for (int b = threadIdx.x; b < nb; b += BLOCK) max_first(v, idx, part[2 * b], part[2 * b + 1]);
+  for (int off = 32; off > 0; off >>= 1) max_first(v, idx, __shfl_down(v, off, 64), __shfl_down(idx, off, 64));
+  int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  if (lane == 0) { smax[w] = v; sidx[w] = idx; }
   __syncthreads();
-  for (int h = BLOCK / 2; h > 0; h >>= 1) {
-    if (threadIdx.x < h) {
-      double ov = smax[threadIdx.x + h], oi = sidx[threadIdx.x + h];
-      if (ov > smax[threadIdx.x] || (ov == smax[threadIdx.x] && oi < sidx[threadIdx.x])) {
-        smax[threadIdx.x] = ov; sidx[threadIdx.x] = oi;
-      }
-    }
-    __syncthreads();
+  if (threadIdx.x == 0) {
+    for (int k = 1; k < BLOCK / 64; k++) max_first(v, idx, smax[k], sidx[k]);
+    trig[0] = v;
+    trig[1] = idx;
   }
-  if (threadIdx.x == 0) { trig[0] = smax[0]; trig[1] = sidx[0]; }
 }
 __global__ void k_disp_c(double* __restrict__ part, const double* __restrict__ d2, const int* __restrict__ perm,
                          const double* __restrict__ trig, int N) {
@@ -223,19 +221,17 @@ __global__ void k_disp_c(double* __restrict__ part, const double* __restrict__ d
 __global__ void k_disp_c_reduce(double* __restrict__ trig, const double* __restrict__ part, int nb,
                                 double* __restrict__ host, unsigned long long* __restrict__ hostSeq,
                                 unsigned long long seq) {
-  __shared__ double smax[BLOCK];
+  __shared__ double smax[BLOCK / 64];
   double v = 0.0;
 #pragma unroll 8
   for (int b = threadIdx.x; b < nb; b += BLOCK) v = fmax(v, part[b]);
-  smax[threadIdx.x] = v;
+  for (int off = 32; off > 0; off >>= 1) v = fmax(v, __shfl_down(v, off, 64));
+  if ((threadIdx.x & 63) == 0) smax[threadIdx.x >> 6] = v;
   __syncthreads();
-  for (int h = BLOCK / 2; h > 0; h >>= 1) {
-    if (threadIdx.x < h) smax[threadIdx.x] = fmax(smax[threadIdx.x], smax[threadIdx.x + h]);
-    __syncthreads();
-  }
   if (threadIdx.x == 0) {   // posts {max, k, next} to the host (as k_reduce_post)
-    trig[2] = smax[0];
-    host[0] = trig[0]; host[1] = trig[1]; host[2] = smax[0];
+    for (int k = 1; k < BLOCK / 64; k++) v = fmax(v, smax[k]);
+    trig[2] = v;
+    host[0] = trig[0]; host[1] = trig[1]; host[2] = v;
     __threadfence_system();
     *(volatile unsigned long long*)hostSeq = seq;
   }
