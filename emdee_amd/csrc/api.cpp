@@ -87,7 +87,10 @@ struct System {
   bool atomEnergies = false;                 // next force evaluation writes atom energies
   int precision = 0;
   Device* dev = nullptr;
-  std::vector<double> mirror;                // host buffer for EmDee_memory_address
+  // EmDee_memory_address views: host arrays (coordinates, momenta, forces,
+  // layerForces) kept equal to the engine's state around every call
+  struct View { std::vector<double> data, shadow; bool armed = false; };
+  View views[4];
   std::shared_ptr<SharedPhase> phase;        // set once the system shares its phase space
   unsigned long long phaseSeen = 0;          // version of the shared state this system holds
 
@@ -145,11 +148,49 @@ void phase_leave(System* me) {
 // Keeps a shared phase space coherent around an entry point: the state is
 // brought up to date on entry and, for a call that changes it, published on
 // exit.
+// EmDee_memory_address (EmDeeCode.f90:212-235) hands the host pointers into
+// the reference's own arrays.  Here the arrays live on the GPU in cell order,
+// so the pointers are host views: on entry to a call, values the host wrote
+// through a view go to the engine (as writes through the reference's pointers
+// do: nothing is marked stale); on exit, every view is refreshed.
+enum { VIEW_R, VIEW_P, VIEW_F, VIEW_LAYER_F };
+void view_fill(System* me, int k) {
+  System::View& v = me->views[k];
+  size_t n = 3 * (size_t)me->natoms;
+  v.data.resize(k == VIEW_LAYER_F ? n * me->nlayers : n);
+  if (k == VIEW_R) dev_download_positions(me->dev, v.data.data());
+  else if (k == VIEW_P) dev_download_momenta(me->dev, v.data.data());
+  else if (k == VIEW_F) dev_download_forces(me->dev, me->layer - 1, v.data.data());
+  else for (int l = 0; l < me->nlayers; l++) dev_download_forces(me->dev, l, v.data.data() + n * l);
+  v.shadow = v.data;
+}
+void views_push(System* me) {
+  size_t n = 3 * (size_t)me->natoms;
+  for (int k = 0; k < 4; k++) {
+    System::View& v = me->views[k];
+    if (!v.armed || v.data == v.shadow) continue;
+    if (k == VIEW_R) dev_upload_positions(me->dev, v.data.data());
+    else if (k == VIEW_P) dev_upload_momenta(me->dev, v.data.data());
+    else if (k == VIEW_F) dev_upload_forces(me->dev, me->layer - 1, v.data.data());
+    else for (int l = 0; l < me->nlayers; l++) dev_upload_forces(me->dev, l, v.data.data() + n * l);
+    v.shadow = v.data;
+  }
+}
+void views_pull(System* me) {
+  for (int k = 0; k < 4; k++) if (me->views[k].armed) view_fill(me, k);
+}
+
 struct PhaseScope {
   System* me;
   bool writes;
-  PhaseScope(System* s, bool w) : me(s), writes(w) { phase_acquire(me); }
-  ~PhaseScope() { if (writes) phase_publish(me); }
+  PhaseScope(System* s, bool w) : me(s), writes(w) {
+    phase_acquire(me);
+    if (me->initialized) views_push(me);
+  }
+  ~PhaseScope() {
+    if (writes) phase_publish(me);
+    if (me->initialized) views_pull(me);
+  }
 };
 
 void mark_stale(System* me, tEmDee* md) {
@@ -477,20 +518,22 @@ tEmDee EmDee_system(int threads, int layers, double rc, double skin, int N, int*
   return md;
 }
 
-// EmDeeCode.f90:212-235.  The engine's arrays live in GPU memory; the
-// address returned is a host snapshot taken at the time of the call.
+// EmDeeCode.f90:212-235.  The address is a host view of the engine's array
+// (see views_push/views_pull): it stays valid and current for the life of the
+// system, and values written through it reach the engine at the next call.
 void* EmDee_memory_address(tEmDee md, char* option) {
   System* me = S(md);
-  PhaseScope ps(me, false);
   std::string item(option);
-  me->mirror.assign(3 * (size_t)me->natoms * (item == "layerForces" ? me->nlayers : 1), 0.0);
-  if (item == "coordinates") dev_download_positions(me->dev, me->mirror.data());
-  else if (item == "momenta") dev_download_momenta(me->dev, me->mirror.data());
-  else if (item == "forces") dev_download_forces(me->dev, me->layer - 1, me->mirror.data());
-  else if (item == "layerForces") {
-    for (int l = 0; l < me->nlayers; l++) dev_download_forces(me->dev, l, me->mirror.data() + 3 * (size_t)me->natoms * l);
-  } else fatal("memory address retrieving", "invalid option " + item);
-  return me->mirror.data();
+  int k = item == "coordinates" ? VIEW_R : item == "momenta" ? VIEW_P : item == "forces" ? VIEW_F
+        : item == "layerForces" ? VIEW_LAYER_F : -1;
+  if (k < 0) fatal("memory address retrieving", "invalid option " + item);
+  PhaseScope ps(me, false);
+  if (!me->views[k].armed) {
+    me->views[k].armed = true;
+    if (me->initialized) view_fill(me, k);
+    else me->views[k].data.assign(3 * (size_t)me->natoms * (k == VIEW_LAYER_F ? me->nlayers : 1), 0.0);
+  }
+  return me->views[k].data.data();
 }
 
 // EmDeeCode.f90:239-269
@@ -788,7 +831,10 @@ void EmDee_upload(tEmDee* md, char* option, double* address) {
 }
 
 // EmDeeCode.f90:929-946
-void EmDee_switch_model_layer(tEmDee* md, int layer) { switch_layer(S(*md), md, layer); }
+void EmDee_switch_model_layer(tEmDee* md, int layer) {
+  PhaseScope ps(S(*md), false);
+  switch_layer(S(*md), md, layer);
+}
 
 // pi = B(q) (2 I omega): tBody_assign_momenta, ArBee.f90:356-362
 static void body_pi_from_omega(const double* q, const double* MoI, const double* w, double* pi) {
@@ -952,7 +998,10 @@ void EmDee_verlet_step(tEmDee* md, double dt) {
 }
 
 // EmDeeCode.f90:1215-1277
-void EmDee_compute_forces(tEmDee* md) { compute_forces(md); }
+void EmDee_compute_forces(tEmDee* md) {
+  PhaseScope ps(S(*md), false);
+  compute_forces(md);
+}
 
 // EmDeeCode.f90:1281-1395 (analysis observable, not part of the force engine)
 void EmDee_rdf(tEmDee md, int bins, double Rc, int pairs, int itype[], int jtype[], double g[]) {
