@@ -164,14 +164,22 @@ k_hmc(const ReplicaDesc* __restrict__ desc, const int* __restrict__ order, doubl
           cz = (int)(((unsigned long long)wi * M) >> 32);
       double4 ri = Rr[i];
       double fx = 0, fy = 0, fz = 0, ei = 0;
+      // cells are numbered x fastest, so the cells x-w..x+w of one (y, z) row
+      // are one contiguous slot range (two where the row wraps, visited in
+      // the order of increasing dx); w = 0 takes every atom at once
       for (int dz = -w; dz <= w; dz++) {
-        int z = (cz + dz + M) % M;
+        int z = cz + dz; z = z < 0 ? z + M : (z >= M ? z - M : z);
         for (int dy = -w; dy <= w; dy++) {
-          int y = (cy + dy + M) % M;
-          for (int dx = -w; dx <= w; dx++) {
-            int x = (cx + dx + M) % M;
-            int c = (w == 0) ? 0 : x + M * (y + M * z);
-            int t0 = (w == 0) ? 0 : cstart[c], t1 = (w == 0) ? N : cstart[c + 1];
+          int y = cy + dy; y = y < 0 ? y + M : (y >= M ? y - M : y);
+          int base = M * (y + M * z), xa = cx - w, xb = cx + w;
+          int r0a, r0b, r1a = 0, r1b = -1;   // cell ranges [r0a, r0b] then [r1a, r1b]
+          if (xa < 0) { r0a = xa + M; r0b = M - 1; r1a = 0; r1b = xb; }
+          else if (xb >= M) { r0a = xa; r0b = M - 1; r1a = 0; r1b = xb - M; }
+          else { r0a = xa; r0b = xb; }
+          for (int part = 0; part < 2; part++) {
+            int ca = part == 0 ? r0a : r1a, cb = part == 0 ? r0b : r1b;
+            if (cb < ca) continue;
+            int t0 = (w == 0) ? 0 : cstart[base + ca], t1 = (w == 0) ? N : cstart[base + cb + 1];
             for (int t = t0; t < t1; t++) {
               int j = (w == 0) ? t : clist[t];
               if (j == i) continue;
