@@ -157,11 +157,18 @@ k_hmc(const ReplicaDesc* __restrict__ desc, const int* __restrict__ order, doubl
   auto forces = [&]() -> double {
     build_cells();
     const int w = (M >= 2 * DIV + 1) ? DIV : 0;   // w = 0: one cell holds every atom
+    // cutoff^2 in cell units, with margins far above the FP32 rounding of the
+    // culling below (which can then only keep extra cells)
+    const float rcell2 = (float)(d.Rc2s * (double)M * (double)M * (1.0 + 1e-5) + 1e-5);
     double e = 0.0;
     for (int i = threadIdx.x; i < N; i += BS) {
       unsigned int ui = qx[i], vi = qy[i], wi = qz[i];
-      int cx = (int)(((unsigned long long)ui * M) >> 32), cy = (int)(((unsigned long long)vi * M) >> 32),
-          cz = (int)(((unsigned long long)wi * M) >> 32);
+      unsigned long long px = (unsigned long long)ui * M, py = (unsigned long long)vi * M,
+                         pz = (unsigned long long)wi * M;
+      int cx = (int)(px >> 32), cy = (int)(py >> 32), cz = (int)(pz >> 32);
+      // position of atom i inside its cell, in cell units
+      const float ox = (float)(unsigned int)px * 2.3283064e-10f, oy = (float)(unsigned int)py * 2.3283064e-10f,
+                  oz = (float)(unsigned int)pz * 2.3283064e-10f;
       double4 ri = Rr[i];
       double fx = 0, fy = 0, fz = 0, ei = 0;
       // cells are numbered x fastest, so the cells x-w..x+w of one (y, z) row
@@ -169,9 +176,18 @@ k_hmc(const ReplicaDesc* __restrict__ desc, const int* __restrict__ order, doubl
       // the order of increasing dx); w = 0 takes every atom at once
       for (int dz = -w; dz <= w; dz++) {
         int z = cz + dz; z = z < 0 ? z + M : (z >= M ? z - M : z);
+        float gz = fmaxf(dz > 0 ? dz - oz : (dz < 0 ? oz - dz - 1 : 0.0f), 0.0f);
         for (int dy = -w; dy <= w; dy++) {
           int y = cy + dy; y = y < 0 ? y + M : (y >= M ? y - M : y);
           int base = M * (y + M * z), xa = cx - w, xb = cx + w;
+          if (w > 0) {   // rows and cells beyond the cutoff sphere hold no partner
+            float gy = fmaxf(dy > 0 ? dy - oy : (dy < 0 ? oy - dy - 1 : 0.0f), 0.0f);
+            float rem = rcell2 - gy * gy - gz * gz;
+            if (rem <= 0.0f) continue;
+            float ex = sqrtf(rem) + 1e-4f;
+            xa = cx + max((int)floorf(ox - ex), -w);
+            xb = cx + min((int)floorf(ox + ex), w);
+          }
           int r0a, r0b, r1a = 0, r1b = -1;   // cell ranges [r0a, r0b] then [r1a, r1b]
           if (xa < 0) { r0a = xa + M; r0b = M - 1; r1a = 0; r1b = xb; }
           else if (xb >= M) { r0a = xa; r0b = M - 1; r1a = 0; r1b = xb - M; }
@@ -390,8 +406,10 @@ EmDeeHMC* EmDeeHMC_create(int nrep, const int* natoms, const double* box, const 
     d.off = off; d.N = natoms[r]; d.M = 1; d.L = box[r]; d.kT = kT[r];
     double two32 = 4294967296.0, rc2s = (rc * rc) / (box[r] * box[r]), rc2u = rc2s * two32 * two32;
     d.K = (float)(two32 * two32 / (box[r] * box[r]));
-    d.lo = (float)(rc2u * (1.0 - 2e-6));
-    d.hi = (float)(rc2u * (1.0 + 2e-6));
+    // FP32 rounding plus the quantisation of the fixed-point coordinates
+    const double q = 1.0 / two32, band = 2e-6 + 4.0 * q / std::sqrt(rc2s) + 3.0 * q * q / rc2s;
+    d.lo = (float)(rc2u * (1.0 - band));
+    d.hi = (float)(rc2u * (1.0 + band));
     d.Rc2s = rc2s;
     d.unit = box[r] / two32;
     h->offsets[r] = off;
