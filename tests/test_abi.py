@@ -67,8 +67,9 @@ def test_reference_c_host_compiles_unchanged(tmp_path):
     assert r.returncode == 0, r.stderr
 
 
-@pytest.mark.skipif(not (os.path.exists(REF + "/test/test_pair_lj_cut.f90") and shutil.which("amdflang")
-                         or os.path.exists("/opt/rocm/bin/amdflang")), reason="reference tests or amdflang absent")
+@pytest.mark.skipif(not (os.path.exists(REF + "/test/test_pair_lj_cut.f90")
+                         and (shutil.which("amdflang") or os.path.exists("/opt/rocm/bin/amdflang"))),
+                    reason="reference tests or amdflang absent")
 def test_reference_fortran_host_compiles_unchanged(tmp_path):
     fc = shutil.which("amdflang") or "/opt/rocm/bin/amdflang"
     common = REF + "/test/common"
