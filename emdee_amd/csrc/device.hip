@@ -998,15 +998,15 @@ k_pairs_mx(double* __restrict__ F, double* __restrict__ partials, const int4* __
 // ---- integrator -------------------------------------------------------------
 // boost, EmDeeData.f90:864-896 (free atoms): P = CP*P + CF*F;
 // kinetic_energies, EmDeeData.f90:900-922: 2KE_x = sum invMass*P_x^2.
-template <bool KE, bool UNIF>
+template <bool KE>
 __global__ void k_boost(double* __restrict__ P, double* __restrict__ partials, const double* __restrict__ F,
                         const double* __restrict__ invMass, const int* __restrict__ freeFlag,
-                        double CP, double CF, int translate, int N, int Npad, double invMass0) {
+                        double CP, double CF, int translate, int N, int Npad) {
   int blk = blockIdx.x;
   int s = blk * BLOCK + threadIdx.x;
   double v[3] = { 0, 0, 0 };
-  if (s < N && (UNIF || freeFlag[s])) {
-    double im = UNIF ? invMass0 : invMass[s];
+  if (s < N && freeFlag[s]) {
+    double im = invMass[s];
 #pragma unroll
     for (int x = 0; x < 3; x++) {
       double p = P[x * Npad + s];
@@ -1019,15 +1019,57 @@ __global__ void k_boost(double* __restrict__ P, double* __restrict__ partials, c
     if (threadIdx.x == 0) { partials[3 * blk] = v[0]; partials[3 * blk + 1] = v[1]; partials[3 * blk + 2] = v[2]; }
   }
 }
+// The same kick for free atoms of one mass, two neighbouring slots per
+// thread (16-byte loads and stores; the second slot is left alone past N).
+template <bool KE>
+__global__ void k_boost_u2(double* __restrict__ P, double* __restrict__ partials, const double* __restrict__ F,
+                           double CP, double CF, int translate, int N, int Npad, double im) {
+  int blk = blockIdx.x;
+  int s = 2 * (blk * BLOCK + threadIdx.x);
+  double v[3] = { 0, 0, 0 };
+  if (s < N) {
+    bool two = s + 1 < N;
+#pragma unroll
+    for (int x = 0; x < 3; x++) {
+      double2* px = reinterpret_cast<double2*>(P + (size_t)x * Npad + s);
+      double2 p = *px;
+      if (translate) {
+        double2 f = *reinterpret_cast<const double2*>(F + (size_t)x * Npad + s);
+        p.x = CP * p.x + CF * f.x;
+        if (two) p.y = CP * p.y + CF * f.y;
+        *px = p;
+      }
+      if (KE) v[x] = im * (p.x * p.x) + (two ? im * (p.y * p.y) : 0.0);
+    }
+  }
+  if (KE) {
+    block_sum<3>(v);
+    if (threadIdx.x == 0) { partials[3 * blk] = v[0]; partials[3 * blk + 1] = v[1]; partials[3 * blk + 2] = v[2]; }
+  }
+}
 // move, EmDeeData.f90:823-860 (free atoms): R = CR*R + CP*P*invMass.
-template <bool UNIF>
 __global__ void k_displace(double* __restrict__ R, const double* __restrict__ P, const double* __restrict__ invMass,
-                           const int* __restrict__ freeFlag, double CR, double CP, int N, int Npad, double invMass0) {
+                           const int* __restrict__ freeFlag, double CR, double CP, int N, int Npad) {
   int s = blockIdx.x * BLOCK + threadIdx.x;
-  if (s >= N || (!UNIF && !freeFlag[s])) return;
-  double im = UNIF ? invMass0 : invMass[s];
+  if (s >= N || !freeFlag[s]) return;
+  double im = invMass[s];
 #pragma unroll
   for (int x = 0; x < 3; x++) R[x * Npad + s] = CR * R[x * Npad + s] + CP * P[x * Npad + s] * im;
+}
+// The same move when every atom is free and of one mass, two slots per thread.
+__global__ void k_displace_u2(double* __restrict__ R, const double* __restrict__ P, double CR, double CP, int N,
+                              int Npad, double im) {
+  int s = 2 * (blockIdx.x * BLOCK + threadIdx.x);
+  if (s >= N) return;
+  bool two = s + 1 < N;
+#pragma unroll
+  for (int x = 0; x < 3; x++) {
+    double2* rx = reinterpret_cast<double2*>(R + (size_t)x * Npad + s);
+    double2 r = *rx, p = *reinterpret_cast<const double2*>(P + (size_t)x * Npad + s);
+    r.x = CR * r.x + CP * p.x * im;
+    if (two) r.y = CR * r.y + CP * p.y * im;
+    *rx = r;
+  }
 }
 
 // ---- shadow terms of EmDee_verlet_step, free atoms (EmDeeCode.f90:1167-1204)
@@ -1549,18 +1591,28 @@ void dev_boost(Device* d, int layer, double CP, double CF, bool translate, doubl
   if (N == 0) { if (twoKE) twoKE[0] = twoKE[1] = twoKE[2] = 0; return; }
   int g = div_up(N, BLOCK);
   const double* F = d->F.p + 3 * (size_t)Np * layer;
+  const int g2 = div_up(div_up(N, 2), BLOCK);
   if (twoKE) {
-    hipLaunchKernelGGL((d->uniformFree ? k_boost<true, true> : k_boost<true, false>), dim3(g), dim3(BLOCK), 0, d->stream,
-                       d->P.p, d->partials.p, F, d->invMass.p, d->freeFlag.p, CP, CF, (int)translate, N, Np,
-                       d->invMass0);
+    if (d->uniformFree) {
+      hipLaunchKernelGGL(k_boost_u2<true>, dim3(g2), dim3(BLOCK), 0, d->stream, d->P.p, d->partials.p, F, CP, CF,
+                         (int)translate, N, Np, d->invMass0);
+      g = g2;
+    } else {
+      hipLaunchKernelGGL(k_boost<true>, dim3(g), dim3(BLOCK), 0, d->stream, d->P.p, d->partials.p, F, d->invMass.p,
+                         d->freeFlag.p, CP, CF, (int)translate, N, Np);
+    }
     unsigned long long seq = ++d->seq;
     hipLaunchKernelGGL(k_reduce_post<3>, dim3(1), dim3(BLOCK), 0, d->stream, d->hpostDev, d->hseqDev, seq, d->partials.p, g);
     launch_check("boost");
     post_wait(d, seq);
     for (int x = 0; x < 3; x++) twoKE[x] = d->hpost[x];
   } else if (translate) {
-    hipLaunchKernelGGL((d->uniformFree ? k_boost<false, true> : k_boost<false, false>), dim3(g), dim3(BLOCK), 0,
-                       d->stream, d->P.p, d->partials.p, F, d->invMass.p, d->freeFlag.p, CP, CF, 1, N, Np, d->invMass0);
+    if (d->uniformFree)
+      hipLaunchKernelGGL(k_boost_u2<false>, dim3(g2), dim3(BLOCK), 0, d->stream, d->P.p, d->partials.p, F, CP, CF, 1, N,
+                         Np, d->invMass0);
+    else
+      hipLaunchKernelGGL(k_boost<false>, dim3(g), dim3(BLOCK), 0, d->stream, d->P.p, d->partials.p, F, d->invMass.p,
+                         d->freeFlag.p, CP, CF, 1, N, Np);
   }
   launch_check("boost");
 }
@@ -1570,8 +1622,12 @@ void dev_kinetic(Device* d, double* twoKE) { dev_boost(d, 0, 1.0, 0.0, false, tw
 void dev_displace(Device* d, double CR, double CP) {
   int N = d->N;
   if (N == 0) return;
-  hipLaunchKernelGGL((d->uniformFree ? k_displace<true> : k_displace<false>), dim3(div_up(N, BLOCK)), dim3(BLOCK), 0,
-                     d->stream, d->R.p, d->P.p, d->invMass.p, d->freeFlag.p, CR, CP, N, d->Npad, d->invMass0);
+  if (d->uniformFree)
+    hipLaunchKernelGGL(k_displace_u2, dim3(div_up(div_up(N, 2), BLOCK)), dim3(BLOCK), 0, d->stream, d->R.p, d->P.p, CR,
+                       CP, N, d->Npad, d->invMass0);
+  else
+    hipLaunchKernelGGL(k_displace, dim3(div_up(N, BLOCK)), dim3(BLOCK), 0, d->stream, d->R.p, d->P.p, d->invMass.p,
+                       d->freeFlag.p, CR, CP, N, d->Npad);
   launch_check("displace");
 }
 
