@@ -912,15 +912,26 @@ k_pairs_mx(double* __restrict__ F, double* __restrict__ partials, const int4* __
         flush(a);
       };
       // three groups of four in flight per iteration; each group is still
-      // summed in FP32 and flushed to FP64 on its own
+      // summed in FP32 and flushed to FP64 on its own.  The indices of the
+      // next twelve entries are loaded before this block's terms, so their
+      // latency hides behind the arithmetic.
       int k = 0;
+      int jn[12];
+      if (n4 >= 12) {
+#pragma unroll
+        for (int u = 0; u < 12; u++) jn[u] = __builtin_nontemporal_load(col + (size_t)u * Npad);
+      }
       for (; k + 12 <= n4; k += 12) {
         int j[12];
         int4 q[12];
 #pragma unroll
-        for (int u = 0; u < 12; u++) j[u] = __builtin_nontemporal_load(col + (size_t)(k + u) * Npad);
+        for (int u = 0; u < 12; u++) j[u] = jn[u];
 #pragma unroll
         for (int u = 0; u < 12; u++) q[u] = Q4[j[u]];
+        if (k + 24 <= n4) {
+#pragma unroll
+          for (int u = 0; u < 12; u++) jn[u] = __builtin_nontemporal_load(col + (size_t)(k + 12 + u) * Npad);
+        }
         group4(j, q);
         group4(j + 4, q + 4);
         group4(j + 8, q + 8);

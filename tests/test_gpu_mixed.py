@@ -92,3 +92,53 @@ def test_same_pair_set_as_fp64(gpu):
     gpu.compute_forces(mg)
     assert mg.Energy.Potential == pytest.approx(E_mixed, rel=1e-6)
     assert np.array_equal(R, gpu.download(mg, "coordinates", (cfg["N"], 3)))
+
+
+def fcc_box(nc, density, jitter, seed):
+    """4 nc^3 atoms on an fcc lattice that spans a cubic box at the given
+    density, each displaced by a seeded Gaussian of deviation `jitter`."""
+    N = 4 * nc ** 3
+    L = (N / density) ** (1.0 / 3.0)
+    base = np.array([[0, 0, 0], [0.5, 0.5, 0], [0.5, 0, 0.5], [0, 0.5, 0.5]])
+    g = np.stack(np.meshgrid(*(np.arange(nc),) * 3, indexing="ij"), -1).reshape(-1, 1, 3)
+    R = ((g + base) * (L / nc)).reshape(-1, 3)
+    R += np.random.default_rng(seed).normal(0.0, jitter, R.shape)
+    return R, L
+
+
+@pytest.mark.parametrize("shifted", [False, True])
+def test_large_lj_fast_path(gpu, oracle, shifted):
+    """131,072 atoms of one type, lj_cut (plain and shifted): the size from
+    which the mixed form runs its dedicated LJ kernel (blocks of twelve
+    entries, cutoff-band entries decided in FP64).  Forces, energy and virial
+    against the oracle, before and after twenty steps with list rebuilds."""
+    R, L = fcc_box(32, 0.8442, 0.08, seed=17)
+    N = len(R)
+
+    def make(lib, mixed):
+        md = lib.system(1, 1, 2.5, 0.3, N, None, None, None)
+        model = lib.EmDee_pair_lj_cut(1.0, 1.0)
+        if shifted:
+            model = lib.EmDee_shifted(model)
+        lib.EmDee_set_pair_model(md, 1, 1, model, 0.0)
+        if mixed:
+            lib.EmDeeGPU_set_precision(md, 1)
+        lib.upload(md, "box", np.array([L]))
+        lib.upload(md, "coordinates", R)
+        lib.random_momenta(md, 1.0, True, 5)
+        return md
+    mg, mo = make(gpu, True), make(oracle, False)
+    for stage in range(2):
+        if stage:
+            for _ in range(20):
+                for lib, md in ((gpu, mg), (oracle, mo)):
+                    lib.boost(md, 1.0, 0.0, 0.0025)
+                    lib.displace(md, 1.0, 0.0, 0.005)
+                    lib.boost(md, 1.0, 0.0, 0.0025)
+        gpu.compute_forces(mg)
+        oracle.compute_forces(mo)
+        ferr, eerr, werr = errors(gpu, oracle, mg, mo, N)
+        dW = abs(mg.Virial - mo.Virial) / N
+        print(f"mixed LJ N={N} shifted={shifted} stage {stage}: force rms {ferr:.2e} energy {eerr:.2e} "
+              f"virial per atom {dW:.1e}")
+        assert ferr < FORCE_RMS and eerr < 1e-6 and dW < 2e-5
