@@ -306,13 +306,16 @@ void initialize_system(System* me, tEmDee* md) {
   }
 }
 
-// EmDeeCode.f90:1215-1277
-void compute_forces(tEmDee* md) {
+// EmDeeCode.f90:1215-1277, in two parts: forces_begin queues the list
+// update and the pair kernels; forces_end adds the other terms and fills the
+// energy and virial records.  With `defer`, and a layer of pair terms only,
+// forces_begin returns true without waiting for the pair sums: the caller
+// queues more work (the kick of EmDee_boost) and reads them later with one
+// wait for both.
+static bool forces_begin(tEmDee* md, bool defer, ForceSums& fs) {
   System* me = S(*md);
   if (!me->initialized) fatal("force computation", "box and coordinates have not been defined");
   phase_acquire(me);
-  double E[5] = { 0, 0, 0, 0, 0 }, W[5] = { 0, 0, 0, 0, 0 };
-  enum { PAIR_ = 0, COUL_ = 1, LONG_ = 2, BOND_ = 3, ANGLE_ = 4 };
   // handle_neighbor_lists: neighbor_lists.f90:175-195
   md->Time.Neighbor -= wtime();
   if (dev_needs_rebuild(me->dev, me->skinSq, me->L, me->precision != 0, me->layer - 1)) {
@@ -333,13 +336,25 @@ void compute_forces(tEmDee* md) {
   c.invL2 = 1.0 / (me->L * me->L);
   c.Rc2 = (me->useInRc[l] ? me->InRcSq : me->RcSq) * c.invL2;
   c.scaled = 1;
-  ForceSums fs{ 0, 0, 0, 0 };
+  fs = ForceSums{ 0, 0, 0, 0 };
+  bool pairsOnly = me->nbonds + me->nangles == 0 && !me->coul[l - 1].requires_kspace &&
+                   !(me->surrogate && me->surrogateLayer == l) && me->nbodies == 0;
+  c.defer = defer && pairsOnly && me->pairs_exist[l];
   if (me->pairs_exist[l]) {
     dev_compute_pairs(me->dev, c, &fs);
   } else {
     std::vector<double> zero(3 * (size_t)me->natoms, 0.0);
     dev_upload_forces(me->dev, l - 1, zero.data());
   }
+  return c.defer != 0;
+}
+
+static void forces_end(tEmDee* md, const ForceSums& fs) {
+  System* me = S(*md);
+  double E[5] = { 0, 0, 0, 0, 0 }, W[5] = { 0, 0, 0, 0, 0 };
+  enum { PAIR_ = 0, COUL_ = 1, LONG_ = 2, BOND_ = 3, ANGLE_ = 4 };
+  bool compute = md->Options.Compute || me->atomEnergies;
+  int l = me->layer;
   E[PAIR_] = fs.Epair; E[COUL_] = fs.Ecoul; W[PAIR_] = fs.Wpair; W[COUL_] = fs.Wcoul;
   // compute_bonds / compute_angles, EmDeeData.f90:443-550
   bool kspace = me->coul[l - 1].requires_kspace;
@@ -381,6 +396,12 @@ void compute_forces(tEmDee* md) {
   double t = wtime();
   md->Time.Pair += t;
   md->Time.Total = t - me->startTime;
+}
+
+void compute_forces(tEmDee* md) {
+  ForceSums fs;
+  forces_begin(md, false, fs);
+  forces_end(md, fs);
 }
 
 void set_kinetic(tEmDee* md, const double* twoKEt, const double* twoKEr) {
@@ -925,11 +946,22 @@ void EmDee_boost(tEmDee* md, double lambda, double alpha, double dt) {
   double CF = phi_fn(alpha * dt) * dt;
   double CP = 1.0 - alpha * CF;
   CF = lambda * CF;
-  if (lambda != 0.0 && !me->forcesUpToDate[me->layer]) compute_forces(md);
+  // a force evaluation that is due is queued before the kick, and its sums
+  // are read after the kick's kinetic energy (one wait for both)
+  ForceSums fs;
+  bool pending = false;
+  if (lambda != 0.0 && !me->forcesUpToDate[me->layer]) {
+    pending = forces_begin(md, true, fs);
+    if (!pending) forces_end(md, fs);
+  }
   bool compute = md->Options.Compute, translate = md->Options.Translate, rotate = md->Options.Rotate;
   double twoKEt[3] = { 0, 0, 0 }, kb[6] = { 0, 0, 0, 0, 0, 0 };
   dev_bodies_boost(me->dev, me->layer - 1, CP, CF, translate, rotate, compute ? kb : nullptr);
   dev_boost(me->dev, me->layer - 1, CP, CF, translate, compute ? twoKEt : nullptr);
+  if (pending) {
+    dev_pairs_result(me->dev, &fs);
+    forces_end(md, fs);
+  }
   if (compute) {
     if (translate) for (int x = 0; x < 3; x++) md->Kinetic.TransPart[x] = 0.5 * (kb[x] + twoKEt[x]);
     if (rotate) {

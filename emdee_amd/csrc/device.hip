@@ -49,11 +49,11 @@ void warning(const std::string& msg) { fprintf(stderr, "WARNING: %s.\n", msg.c_s
 static void post_wait(Device* d, unsigned long long seq) {
   volatile unsigned long long* f = d->hseq;
   for (int k = 0; k < (1 << 20); k++) {
-    if (*f == seq) { std::atomic_thread_fence(std::memory_order_acquire); return; }
+    if (*f >= seq) { std::atomic_thread_fence(std::memory_order_acquire); return; }
     __builtin_ia32_pause();
   }
   HIP_CHECK(hipStreamSynchronize(d->stream));
-  if (*f != seq) fatal("device", "a posted result did not arrive");
+  if (*f < seq) fatal("device", "a posted result did not arrive");
   std::atomic_thread_fence(std::memory_order_acquire);
 }
 
@@ -1581,20 +1581,29 @@ void dev_compute_pairs(Device* d, const PairCall& c, ForceSums* out) {
 #undef LAUNCH_PAIRS
   }
   launch_check("pairs");
-  unsigned long long seq = ++d->seq;
-  hipLaunchKernelGGL(k_reduce_post<4>, dim3(1), dim3(BLOCK), 0, d->stream, d->hpostDev, d->hseqDev, seq, d->partials.p, gp);
+  // the pair sums have a slot of their own, so that a later post (the
+  // kinetic energy of the kick that follows a deferred call) leaves them
+  d->pairSeq = ++d->seq;
+  hipLaunchKernelGGL(k_reduce_post<4>, dim3(1), dim3(BLOCK), 0, d->stream, d->hpostDev + POST_PAIRS, d->hseqDev,
+                     d->pairSeq, d->partials.p, gp);
   launch_check("reduce");
-  post_wait(d, seq);
+  if (!c.defer) dev_pairs_result(d, out);
+}
+
+void dev_pairs_result(Device* d, ForceSums* out) {
+  if (d->N == 0) { *out = ForceSums{ 0, 0, 0, 0 }; return; }
+  post_wait(d, d->pairSeq);
   if (d->profiling) {
     float ms = 0;
     HIP_CHECK(hipEventSynchronize(d->ev1));
     HIP_CHECK(hipEventElapsedTime(&ms, d->ev0, d->ev1));
     d->pairMs += ms; d->pairLaunches++;
   }
-  out->Epair = d->hpost[0];
-  out->Ecoul = d->hpost[1];
-  out->Wpair = d->hpost[2];
-  out->Wcoul = d->hpost[3];
+  const double* h = d->hpost + POST_PAIRS;
+  out->Epair = h[0];
+  out->Ecoul = h[1];
+  out->Wpair = h[2];
+  out->Wcoul = h[3];
 }
 
 void dev_boost(Device* d, int layer, double CP, double CF, bool translate, double* twoKE) {

@@ -108,6 +108,10 @@ __global__ void k_reduce_post(double* __restrict__ host, unsigned long long* __r
 
 // ------------------------------------------------------------ device state
 
+// Slot of the pair sums in the posted-results block (64 doubles); the other
+// posts use slot 0.
+constexpr int POST_PAIRS = 8;
+
 template <typename T>
 struct DBuf {
   T* p = nullptr;
@@ -178,6 +182,7 @@ struct Device {
   unsigned long long* hseq = nullptr;
   unsigned long long* hseqDev = nullptr;
   unsigned long long seq = 0;
+  unsigned long long pairSeq = 0;   // post of the last pair evaluation (slot POST_PAIRS)
   DBuf<double> d2;
   DBuf<double> trig;      // [0] max, [1] first original index holding it, [2] max before it
   // rigid bodies; iperm: original atom -> slot

@@ -81,6 +81,7 @@ struct PairCall {
   int mixed;          // FP32 pair arithmetic with FP64 accumulation
   double L, invL2, Rc2;   // box and scaled squared cutoff
   int scaled;         // scaled positions are current (written by the trigger or the list build)
+  int defer = 0;      // return once the kernels are queued; dev_pairs_result reads the sums
 };
 
 // Sums returned by one force evaluation (indices of EmDeeCode.f90:1219).
@@ -122,6 +123,8 @@ int dev_neighbor_pairs(Device* d, int max, int* out);             // half set, 1
 
 // Forces of one layer: fills F(layer) and returns the reduced sums.
 void dev_compute_pairs(Device* d, const PairCall& c, ForceSums* out);
+// Sums of the last dev_compute_pairs call made with defer set (waits for them).
+void dev_pairs_result(Device* d, ForceSums* out);
 // Adds bonded forces (computed on the host, original order) into F(layer).
 void dev_add_forces(Device* d, int layer, const double* F);
 
