@@ -316,14 +316,6 @@ static bool forces_begin(tEmDee* md, bool defer, ForceSums& fs) {
   System* me = S(*md);
   if (!me->initialized) fatal("force computation", "box and coordinates have not been defined");
   phase_acquire(me);
-  // handle_neighbor_lists: neighbor_lists.f90:175-195
-  md->Time.Neighbor -= wtime();
-  if (dev_needs_rebuild(me->dev, me->skinSq, me->L, me->precision != 0, me->layer - 1)) {
-    dev_rebuild(me->dev, me->L, me->xRc, std::sqrt(me->xInRcSq));
-    md->Builds += 1;
-  }
-  md->Time.Neighbor += wtime();
-  md->Time.Pair -= wtime();
   dev_set_atom_energies(me->dev, me->atomEnergies);
   bool compute = md->Options.Compute || me->atomEnergies;
   int l = me->layer;
@@ -340,8 +332,29 @@ static bool forces_begin(tEmDee* md, bool defer, ForceSums& fs) {
   bool pairsOnly = me->nbonds + me->nangles == 0 && !me->coul[l - 1].requires_kspace &&
                    !(me->surrogate && me->surrogateLayer == l) && me->nbodies == 0;
   c.defer = defer && pairsOnly && me->pairs_exist[l];
+  // handle_neighbor_lists: neighbor_lists.f90:175-195.  When the recent
+  // trigger values make a rebuild unlikely, the pair kernels are queued
+  // behind the trigger kernels before the answer is read, so the GPU does
+  // not wait for the host in between; after a rebuild they run again on the
+  // new list, and their sums replace the first ones.
+  md->Time.Neighbor -= wtime();
+  dev_trigger_launch(me->dev, me->L, me->precision != 0, l - 1);
+  bool ahead = me->pairs_exist[l] && dev_trigger_unlikely(me->dev, me->skinSq);
+  if (ahead) {
+    PairCall ca = c;
+    ca.defer = 1;
+    dev_compute_pairs(me->dev, ca, &fs);
+  }
+  bool rebuild = dev_trigger_result(me->dev, me->skinSq);
+  if (rebuild) {
+    dev_rebuild(me->dev, me->L, me->xRc, std::sqrt(me->xInRcSq));
+    md->Builds += 1;
+  }
+  md->Time.Neighbor += wtime();
+  md->Time.Pair -= wtime();
   if (me->pairs_exist[l]) {
-    dev_compute_pairs(me->dev, c, &fs);
+    if (!ahead || rebuild) dev_compute_pairs(me->dev, c, &fs);
+    else if (!c.defer) dev_pairs_result(me->dev, &fs);
   } else {
     std::vector<double> zero(3 * (size_t)me->natoms, 0.0);
     dev_upload_forces(me->dev, l - 1, zero.data());

@@ -1346,9 +1346,9 @@ static bool lj_path(const Device* d, int layer) {
   return hp.kind == PAIR_LJ_CUT && (hp.modifier == MOD_NONE || hp.modifier == MOD_SHIFTED);
 }
 
-bool dev_needs_rebuild(Device* d, double skinSq, double L, bool mixed, int layer) {
+void dev_trigger_launch(Device* d, double L, bool mixed, int layer) {
   int N = d->N;
-  if (N == 0) return false;
+  if (N == 0) return;
   int g = div_up(N, BLOCK);
   if (mixed) { d->Q4.alloc(d->Npad); d->qf.alloc(d->Npad); }
   // the mixed LJ kernel reads fixed-point positions only (its FP64 band
@@ -1361,14 +1361,38 @@ bool dev_needs_rebuild(Device* d, double skinSq, double L, bool mixed, int layer
                      N, d->Npad, sc);
   hipLaunchKernelGGL(k_disp_a_reduce, dim3(1), dim3(BLOCK), 0, d->stream, d->trig.p, d->partials.p, g);
   hipLaunchKernelGGL(k_disp_c, dim3(g), dim3(BLOCK), 0, d->stream, d->partials.p, d->d2.p, d->perm.p, d->trig.p, N);
-  unsigned long long seq = ++d->seq;
+  d->trigSeq = ++d->seq;
   hipLaunchKernelGGL(k_disp_c_reduce, dim3(1), dim3(BLOCK), 0, d->stream, d->trig.p, d->partials.p, g, d->hpostDev,
-                     d->hseqDev, seq);
+                     d->hseqDev, d->trigSeq);
   launch_check("trigger");
-  post_wait(d, seq);
+}
+
+bool dev_trigger_result(Device* d, double skinSq) {
+  if (d->N == 0) return false;
+  post_wait(d, d->trigSeq);
   double maximum = d->hpost[0], next = (d->hpost[1] == 0.0) ? maximum : d->hpost[2];
   double approach = maximum + 2 * std::sqrt(maximum * next) + next;
-  return approach > skinSq;
+  bool rebuild = approach > skinSq;
+  // history for dev_trigger_unlikely; a rebuild starts the displacements anew
+  d->trigPrev = rebuild ? 0.0 : d->trigLast;
+  d->trigLast = rebuild ? 0.0 : approach;
+  return rebuild;
+}
+
+// sqrt(approach) is the sum of two displacements and grows about linearly
+// with time; a pair evaluation may be queued ahead of the trigger's answer
+// when one and a half times the last step's growth still stays clear of the
+// threshold.  A wrong guess costs one pair evaluation, never a result.
+bool dev_trigger_unlikely(const Device* d, double skinSq) {
+  if (d->trigLast < 0.0 || d->trigPrev < 0.0) return false;
+  double s1 = std::sqrt(d->trigLast), s0 = std::sqrt(d->trigPrev);
+  double p = s1 + 1.5 * std::max(s1 - s0, 0.0);
+  return p * p < skinSq;
+}
+
+bool dev_needs_rebuild(Device* d, double skinSq, double L, bool mixed, int layer) {
+  dev_trigger_launch(d, L, mixed, layer);
+  return dev_trigger_result(d, skinSq);
 }
 
 static void ensure_cells(Device* d, int M) {
