@@ -106,28 +106,31 @@ def fcc_box(nc, density, jitter, seed):
     return R, L
 
 
-@pytest.mark.parametrize("shifted", [False, True])
-def test_large_lj_fast_path(gpu, oracle, shifted):
+@pytest.mark.parametrize("mixed,shifted", [(True, False), (True, True), (False, False)])
+def test_large_lj_fast_path(gpu, oracle, mixed, shifted):
     """131,072 atoms of one type, lj_cut (plain and shifted): the size from
-    which the mixed form runs its dedicated LJ kernel (blocks of twelve
-    entries, cutoff-band entries decided in FP64).  Forces, energy and virial
-    against the oracle, before and after twenty steps with list rebuilds."""
+    which both forms run their dedicated LJ kernels (in the mixed form,
+    blocks of twelve entries with cutoff-band entries decided in FP64) and
+    pair evaluations are queued ahead of the rebuild trigger's answer.
+    Forces, energy, virial, kinetic energy and rebuild count against the
+    oracle, before and after twenty steps with list rebuilds."""
     R, L = fcc_box(32, 0.8442, 0.08, seed=17)
     N = len(R)
+    tol = FORCE_RMS if mixed else 1e-9
 
-    def make(lib, mixed):
+    def make(lib, gpu_mixed):
         md = lib.system(1, 1, 2.5, 0.3, N, None, None, None)
         model = lib.EmDee_pair_lj_cut(1.0, 1.0)
         if shifted:
             model = lib.EmDee_shifted(model)
         lib.EmDee_set_pair_model(md, 1, 1, model, 0.0)
-        if mixed:
+        if gpu_mixed:
             lib.EmDeeGPU_set_precision(md, 1)
         lib.upload(md, "box", np.array([L]))
         lib.upload(md, "coordinates", R)
         lib.random_momenta(md, 1.0, True, 5)
         return md
-    mg, mo = make(gpu, True), make(oracle, False)
+    mg, mo = make(gpu, mixed), make(oracle, False)
     for stage in range(2):
         if stage:
             for _ in range(20):
@@ -139,6 +142,8 @@ def test_large_lj_fast_path(gpu, oracle, shifted):
         oracle.compute_forces(mo)
         ferr, eerr, werr = errors(gpu, oracle, mg, mo, N)
         dW = abs(mg.Virial - mo.Virial) / N
-        print(f"mixed LJ N={N} shifted={shifted} stage {stage}: force rms {ferr:.2e} energy {eerr:.2e} "
-              f"virial per atom {dW:.1e}")
-        assert ferr < FORCE_RMS and eerr < 1e-6 and dW < 2e-5
+        kerr = abs(mg.Kinetic.Total - mo.Kinetic.Total) / mo.Kinetic.Total
+        print(f"LJ N={N} mixed={mixed} shifted={shifted} stage {stage}: force rms {ferr:.2e} energy {eerr:.2e} "
+              f"virial per atom {dW:.1e} kinetic {kerr:.1e} builds {mg.Builds}/{mo.Builds}")
+        assert ferr < tol and eerr < 1e-6 and dW < 2e-5 and kerr < 1e-6
+        assert mg.Builds == mo.Builds
