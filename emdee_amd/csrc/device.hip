@@ -1390,10 +1390,6 @@ bool dev_trigger_unlikely(const Device* d, double skinSq) {
   return p * p < skinSq;
 }
 
-bool dev_needs_rebuild(Device* d, double skinSq, double L, bool mixed, int layer) {
-  dev_trigger_launch(d, L, mixed, layer);
-  return dev_trigger_result(d, skinSq);
-}
 
 static void ensure_cells(Device* d, int M) {
   if (M == d->M) return;

@@ -114,16 +114,14 @@ void dev_download_momenta(Device* d, double* P);
 void dev_upload_forces(Device* d, int layer, const double* F);
 void dev_download_forces(Device* d, int layer, double* F);
 
-// Neighbor lists.
-// Also writes the scaled positions of the current coordinates (and, when
-// mixed, their fixed-point copy) for the pair kernels of the same evaluation.
-bool dev_needs_rebuild(Device* d, double skinSq, double L, bool mixed, int layer);
-// The same in two parts: queue the trigger kernels (they also write the
-// scaled positions of the step), then wait for the answer.
+// Neighbor lists.  The maximum_approach_sq rebuild trigger in two parts:
+// queue its kernels, which also write the scaled positions of the current
+// coordinates (and, when mixed, their fixed-point copy) for the pair kernels
+// of the same evaluation; then wait for the answer (true: rebuild).
 void dev_trigger_launch(Device* d, double L, bool mixed, int layer);
 bool dev_trigger_result(Device* d, double skinSq);
 // Whether the last steps' trigger values make a rebuild at this step unlikely.
-bool dev_trigger_unlikely(const Device* d, double skinSq);                 // maximum_approach_sq trigger
+bool dev_trigger_unlikely(const Device* d, double skinSq);
 void dev_rebuild(Device* d, double L, double xRc, double xInRc);  // sort + full Verlet list
 int dev_neighbor_pairs(Device* d, int max, int* out);             // half set, 1-based i<j
 
