@@ -435,17 +435,19 @@ __device__ __forceinline__ void scan_range(int a, int b, int i, double xi, doubl
         if (!in && (r2 < hi || odd)) in = exact_inside(Rs4, i, j, xRc2);
         return in && j != i;
       }
-      double4 rj = Rs4[j];
-      double r2;
-      {
-#pragma clang fp contract(off)
-        double d0 = xi - rj.x, d1 = yi - rj.y, d2 = zi - rj.z;
-        d0 = d0 - rint(d0); d1 = d1 - rint(d1); d2 = d2 - rint(d2);
-        r2 = d0 * d0 + d1 * d1 + d2 * d2;
-      }
-      if (!(r2 < xRc2 && j != i)) return false;
+      // the same fixed-point prefilter with the FP64 test in the band, then
+      // the body, type and exclusion filters for the candidates in range
+      int4 qj = Q4[j];
+      float dx = (float)(int)((unsigned int)qi.x - (unsigned int)qj.x);
+      float dy = (float)(int)((unsigned int)qi.y - (unsigned int)qj.y);
+      float dz = (float)(int)((unsigned int)qi.z - (unsigned int)qj.z);
+      float r2 = dx * dx + dy * dy + dz * dz;
+      bool odd = ((qi.w | qj.w) & Q4_NONFINITE) != 0;
+      bool in = r2 < lo && !odd;
+      if (!in && (r2 < hi || odd)) in = exact_inside(Rs4, i, j, xRc2);
+      if (!in || j == i) return false;
       if (body[j] == bi) return false;
-      if (!interact[(int)rj.w * ntypes + ti]) return false;
+      if (!interact[(qj.w & Q4_TYPE) * ntypes + ti]) return false;
       if (e1 > e0) {
         int oj = perm[j];
         for (int e = e0; e < e1; e++) if (exItem[e] == oj) return false;
@@ -546,7 +548,7 @@ k_build_list(int* __restrict__ nbr, int* __restrict__ nnb, int* __restrict__ fla
     int c = key[i];
     int cx = c % M, cy = (c / M) % M, cz = c / (M * M);
     int ti = (int)ri.w, bi = body[i], oi = perm[i];
-    int4 qi = SIMPLE ? Q4[i] : int4{ 0, 0, 0, 0 };
+    int4 qi = (SIMPLE || S > 1) ? Q4[i] : int4{ 0, 0, 0, 0 };
     int e0 = 0, e1 = 0;
     if (nexcl) { e0 = exOff[oi]; e1 = exOff[oi + 1]; }
     // Conservative culling: with f the fractional position of atom i inside
