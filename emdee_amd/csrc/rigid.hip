@@ -504,7 +504,9 @@ __global__ void k_body_boost(BodyState B, const double* __restrict__ F, const in
 // move for bodies (EmDeeData.f90:823-860): translation of the centre of mass,
 // then rotation and member positions.  Member positions are written only by
 // the rotation branch, as in the reference.
-__global__ void k_body_move(BodyState B, double* __restrict__ R, const int* __restrict__ iperm, double CR, double CP,
+// (256-thread launch bounds: the rotation's long FP64 chains keep their
+// values in registers instead of spilling at the default 1024-thread bound)
+__global__ void __launch_bounds__(BLOCK) k_body_move(BodyState B, double* __restrict__ R, const int* __restrict__ iperm, double CR, double CP,
                             double dt, int translate, int rotate, int mode, int Npad) {
   int b = blockIdx.x * blockDim.x + threadIdx.x;
   if (b >= B.nb) return;
@@ -633,7 +635,7 @@ __device__ void virtual_rotation(const BodyState& B, int b, double dt, int mode,
 
 // pre_force, body part: r0 = 2.5 rcm + dt/2 invMass (pcm - dt/2 F);
 // q0 = q(-dt)/2 - 3 q.  sh holds [3][nb] r0 then [4][nb] q0.
-__global__ void k_body_shadow_pre(BodyState B, double* __restrict__ sh, double dt, int mode) {
+__global__ void __launch_bounds__(BLOCK) k_body_shadow_pre(BodyState B, double* __restrict__ sh, double dt, int mode) {
   int b = blockIdx.x * blockDim.x + threadIdx.x;
   if (b >= B.nb) return;
   int nb = B.nb;
@@ -645,7 +647,7 @@ __global__ void k_body_shadow_pre(BodyState B, double* __restrict__ sh, double d
 }
 
 // post_force, body part: partials of {Ks_t, Ks_r, Us}.
-__global__ void k_body_shadow_post(BodyState B, const double* __restrict__ sh, double dt, int mode) {
+__global__ void __launch_bounds__(BLOCK) k_body_shadow_post(BodyState B, const double* __restrict__ sh, double dt, int mode) {
   int b = blockIdx.x * blockDim.x + threadIdx.x;
   double v[3] = { 0.0, 0.0, 0.0 };
   if (b < B.nb) {
