@@ -188,7 +188,10 @@ __global__ void k_disp_a(double* __restrict__ d2, double* __restrict__ part, con
 __device__ __forceinline__ void max_first(double& v, double& idx, double ov, double oi) {
   if (ov > v || (ov == v && oi < idx)) { v = ov; idx = oi; }
 }
-__global__ void k_disp_a_reduce(double* __restrict__ trig, const double* __restrict__ part, int nb) {
+// ... and posts {max, first index} to the host as well (as k_reduce_post)
+__global__ void k_disp_a_reduce(double* __restrict__ trig, const double* __restrict__ part, int nb,
+                                double* __restrict__ host, unsigned long long* __restrict__ hostSeq,
+                                unsigned long long seq) {
   __shared__ double smax[BLOCK / 64], sidx[BLOCK / 64];
   double v = -1.0, idx = 2147483647.0;
 #pragma unroll 8
@@ -201,6 +204,10 @@ __global__ void k_disp_a_reduce(double* __restrict__ trig, const double* __restr
     for (int k = 1; k < BLOCK / 64; k++) max_first(v, idx, smax[k], sidx[k]);
     trig[0] = v;
     trig[1] = idx;
+    host[0] = v;
+    host[1] = idx;
+    __threadfence_system();
+    *(volatile unsigned long long*)hostSeq = seq;
   }
 }
 __global__ void k_disp_c(double* __restrict__ part, const double* __restrict__ d2, const int* __restrict__ perm,
@@ -1383,19 +1390,39 @@ void dev_trigger_launch(Device* d, double L, bool mixed, int layer) {
              d->chargedFlag.p, L };
   hipLaunchKernelGGL(k_disp_a, dim3(g), dim3(BLOCK), 0, d->stream, d->d2.p, d->partials.p, d->R.p, d->R0.p, d->perm.p,
                      N, d->Npad, sc);
-  hipLaunchKernelGGL(k_disp_a_reduce, dim3(1), dim3(BLOCK), 0, d->stream, d->trig.p, d->partials.p, g);
-  hipLaunchKernelGGL(k_disp_c, dim3(g), dim3(BLOCK), 0, d->stream, d->partials.p, d->d2.p, d->perm.p, d->trig.p, N);
   d->trigSeq = ++d->seq;
-  hipLaunchKernelGGL(k_disp_c_reduce, dim3(1), dim3(BLOCK), 0, d->stream, d->trig.p, d->partials.p, g, d->hpostDev,
+  hipLaunchKernelGGL(k_disp_a_reduce, dim3(1), dim3(BLOCK), 0, d->stream, d->trig.p, d->partials.p, g, d->hpostDev,
                      d->hseqDev, d->trigSeq);
   launch_check("trigger");
 }
 
+// With next <= max, approach lies in [max, 4 max] (the computed sums too, up
+// to a few roundings), so pass C, which finds `next`, runs only when max
+// alone does not decide: max <= skin^2 < 4 max.  The decision is the one of
+// the full formula in every case.
 bool dev_trigger_result(Device* d, double skinSq) {
   if (d->N == 0) return false;
   post_wait(d, d->trigSeq);
-  double maximum = d->hpost[0], next = (d->hpost[1] == 0.0) ? maximum : d->hpost[2];
-  double approach = maximum + 2 * std::sqrt(maximum * next) + next;
+  double maximum = d->hpost[0], approach;
+  if (maximum > skinSq) {
+    approach = maximum;                    // a lower bound, enough to decide
+  } else if (d->hpost[1] != 0.0 && 4.0 * maximum * (1.0 + 1e-12) < skinSq) {
+    approach = 4.0 * maximum;              // an upper bound, enough to decide
+  } else {
+    double next = maximum;
+    if (d->hpost[1] != 0.0) {              // pass C: max over atoms before the first maximum
+      int N = d->N, g = div_up(N, BLOCK);
+      hipLaunchKernelGGL(k_disp_c, dim3(g), dim3(BLOCK), 0, d->stream, d->partials.p, d->d2.p, d->perm.p, d->trig.p,
+                         N);
+      unsigned long long seq = ++d->seq;
+      hipLaunchKernelGGL(k_disp_c_reduce, dim3(1), dim3(BLOCK), 0, d->stream, d->trig.p, d->partials.p, g,
+                         d->hpostDev, d->hseqDev, seq);
+      launch_check("trigger");
+      post_wait(d, seq);
+      next = d->hpost[2];
+    }
+    approach = maximum + 2 * std::sqrt(maximum * next) + next;
+  }
   bool rebuild = approach > skinSq;
   // history for dev_trigger_unlikely; a rebuild starts the displacements anew
   d->trigPrev = rebuild ? 0.0 : d->trigLast;
