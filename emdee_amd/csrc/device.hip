@@ -151,20 +151,25 @@ __device__ __forceinline__ void scale_slot(const Scaled& sc, const double* __res
   }
 }
 
+// d_k of slot s, |R - R0|^2 summed in this order with no contraction (pass
+// C recomputes it rather than storing it: it runs only near the threshold)
+__device__ __forceinline__ double disp_sq(const double* __restrict__ R, const double* __restrict__ R0, int s, int Npad) {
+#pragma clang fp contract(off)
+  double dx = R[s] - R0[s], dy = R[Npad + s] - R0[Npad + s], dz = R[2 * Npad + s] - R0[2 * Npad + s];
+  return dx * dx + dy * dy + dz * dz;
+}
+
 // Pass A also writes the scaled positions of the current coordinates (sc),
 // which the pair kernels of the same force evaluation read.
-__global__ void k_disp_a(double* __restrict__ d2, double* __restrict__ part, const double* __restrict__ R,
-                         const double* __restrict__ R0, const int* __restrict__ perm, int N, int Npad, Scaled sc) {
+__global__ void k_disp_a(double* __restrict__ part, const double* __restrict__ R, const double* __restrict__ R0,
+                         const int* __restrict__ perm, int N, int Npad, Scaled sc) {
   __shared__ double smax[BLOCK / 64];
   __shared__ int sidx[BLOCK / 64];
   int s = blockIdx.x * BLOCK + threadIdx.x;
   double v = -1.0;
   int idx = INT_MAX;
   if (s < N) {
-#pragma clang fp contract(off)
-    double dx = R[s] - R0[s], dy = R[Npad + s] - R0[Npad + s], dz = R[2 * Npad + s] - R0[2 * Npad + s];
-    v = dx * dx + dy * dy + dz * dz;
-    d2[s] = v;
+    v = disp_sq(R, R0, s, Npad);
     idx = perm[s];
     scale_slot(sc, R, s, Npad);
   }
@@ -210,13 +215,13 @@ __global__ void k_disp_a_reduce(double* __restrict__ trig, const double* __restr
     *(volatile unsigned long long*)hostSeq = seq;
   }
 }
-__global__ void k_disp_c(double* __restrict__ part, const double* __restrict__ d2, const int* __restrict__ perm,
-                         const double* __restrict__ trig, int N) {
+__global__ void k_disp_c(double* __restrict__ part, const double* __restrict__ R, const double* __restrict__ R0,
+                         const int* __restrict__ perm, const double* __restrict__ trig, int N, int Npad) {
   __shared__ double smax[BLOCK / 64];
   int s = blockIdx.x * BLOCK + threadIdx.x;
   double k = trig[1];
   double v = 0.0;
-  if (s < N && (double)perm[s] < k) v = d2[s];
+  if (s < N && (double)perm[s] < k) v = disp_sq(R, R0, s, Npad);
   for (int off = 32; off > 0; off >>= 1) v = fmax(v, __shfl_down(v, off, 64));
   if ((threadIdx.x & 63) == 0) smax[threadIdx.x >> 6] = v;
   __syncthreads();
@@ -1218,7 +1223,6 @@ Device* dev_create(const DeviceSetup& s) {
   *d->hseq = 0;
   d->result.alloc(64);
   d->partials.alloc(64 * (size_t)(Np / BLOCK + 1));   // up to 16 threads per atom x 4 sums
-  d->d2.alloc(Np);
   d->trig.alloc(4);
   HIP_CHECK(hipMemsetAsync(d->Rs4.p, 0, sizeof(double4) * Np, d->stream));
   d->key.alloc(Np); d->order.alloc(Np); d->nnb.alloc(Np); d->flags.alloc(4);
@@ -1262,7 +1266,7 @@ Device* dev_create(const DeviceSetup& s) {
 void dev_destroy(Device* d) {
   if (!d) return;
   (void)hipStreamSynchronize(d->stream);
-  for (DBuf<double>* b : { &d->R, &d->P, &d->R0, &d->invMass, &d->charge, &d->F, &d->tmp3, &d->tmpF, &d->tmp1, &d->stage, &d->partials, &d->result, &d->d2, &d->trig, &d->shadow }) b->release();
+  for (DBuf<double>* b : { &d->R, &d->P, &d->R0, &d->invMass, &d->charge, &d->F, &d->tmp3, &d->tmpF, &d->tmp1, &d->stage, &d->partials, &d->result, &d->trig, &d->shadow }) b->release();
   d->Rs4.release(); d->Q4.release(); d->qf.release(); d->eatom.release();
   for (DBuf<int>* b : { &d->type, &d->body, &d->perm, &d->freeFlag, &d->chargedFlag, &d->tmpI, &d->key, &d->cellCount, &d->cellStart, &d->order, &d->blockSums, &d->nbr, &d->nnb, &d->flags, &d->exOff, &d->exItem, &d->iperm }) b->release();
   d->pairs.release(); d->couls.release(); d->interact.release();
@@ -1388,8 +1392,8 @@ void dev_trigger_launch(Device* d, double L, bool mixed, int layer) {
   // (the FP32 charges change only with the slot order: the list build writes them)
   Scaled sc{ needRs4 ? d->Rs4.p : nullptr, mixed ? d->Q4.p : nullptr, nullptr, d->type.p, d->charge.p,
              d->chargedFlag.p, L };
-  hipLaunchKernelGGL(k_disp_a, dim3(g), dim3(BLOCK), 0, d->stream, d->d2.p, d->partials.p, d->R.p, d->R0.p, d->perm.p,
-                     N, d->Npad, sc);
+  hipLaunchKernelGGL(k_disp_a, dim3(g), dim3(BLOCK), 0, d->stream, d->partials.p, d->R.p, d->R0.p, d->perm.p, N,
+                     d->Npad, sc);
   d->trigSeq = ++d->seq;
   hipLaunchKernelGGL(k_disp_a_reduce, dim3(1), dim3(BLOCK), 0, d->stream, d->trig.p, d->partials.p, g, d->hpostDev,
                      d->hseqDev, d->trigSeq);
@@ -1412,8 +1416,8 @@ bool dev_trigger_result(Device* d, double skinSq) {
     double next = maximum;
     if (d->hpost[1] != 0.0) {              // pass C: max over atoms before the first maximum
       int N = d->N, g = div_up(N, BLOCK);
-      hipLaunchKernelGGL(k_disp_c, dim3(g), dim3(BLOCK), 0, d->stream, d->partials.p, d->d2.p, d->perm.p, d->trig.p,
-                         N);
+      hipLaunchKernelGGL(k_disp_c, dim3(g), dim3(BLOCK), 0, d->stream, d->partials.p, d->R.p, d->R0.p, d->perm.p,
+                         d->trig.p, N, d->Npad);
       unsigned long long seq = ++d->seq;
       hipLaunchKernelGGL(k_disp_c_reduce, dim3(1), dim3(BLOCK), 0, d->stream, d->trig.p, d->partials.p, g,
                          d->hpostDev, d->hseqDev, seq);

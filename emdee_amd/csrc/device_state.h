@@ -185,7 +185,6 @@ struct Device {
   unsigned long long pairSeq = 0;   // post of the last pair evaluation (slot POST_PAIRS)
   unsigned long long trigSeq = 0;   // post of the last rebuild trigger (slot 0)
   double trigLast = -1.0, trigPrev = -1.0;   // trigger values of the last two steps (-1: none)
-  DBuf<double> d2;
   DBuf<double> trig;      // [0] max, [1] first original index holding it, [2] max before it
   // rigid bodies; iperm: original atom -> slot
   BodyState bodies;
