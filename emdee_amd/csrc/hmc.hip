@@ -198,11 +198,11 @@ k_hmc(const ReplicaDesc* __restrict__ desc, const int* __restrict__ order, doubl
             int t0 = (w == 0) ? 0 : cstart[base + ca], t1 = (w == 0) ? N : cstart[base + cb + 1];
             for (int t = t0; t < t1; t++) {
               int j = (w == 0) ? t : clist[t];
-              if (j == i) continue;
               if (MIXED) {
+                // the atom itself (zero separation) is masked, not branched over
                 float ddx = (float)(int)(ui - qx[j]), ddy = (float)(int)(vi - qy[j]), ddz = (float)(int)(wi - qz[j]);
                 float r2 = ddx * ddx + ddy * ddy + ddz * ddz;
-                bool in = r2 < d.lo;
+                bool in = r2 < d.lo && j != i;
                 if (r2 >= d.lo && r2 < d.hi) {
                   double4 rj = Rr[j];
                   double ex = ri.x / L - rj.x / L, ey = ri.y / L - rj.y / L, ez = ri.z / L - rj.z / L;
@@ -225,6 +225,7 @@ k_hmc(const ReplicaDesc* __restrict__ desc, const int* __restrict__ order, doubl
                   ei += (double)(c0 * sr6 * (sr6 - 1.0f));
                 }
               } else {
+                if (j == i) continue;
                 double4 rj = Rr[j];
                 double ex = ri.x / L - rj.x / L, ey = ri.y / L - rj.y / L, ez = ri.z / L - rj.z / L;
                 ex = ex - rint(ex); ey = ey - rint(ey); ez = ez - rint(ez);
