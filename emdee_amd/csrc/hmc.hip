@@ -30,6 +30,7 @@
 #include <cstdio>
 #include <numeric>
 #include <string>
+#include <type_traits>
 #include <vector>
 
 #include "../../include/emdee_gpu.h"
@@ -153,8 +154,10 @@ k_hmc(const ReplicaDesc* __restrict__ desc, const int* __restrict__ order, doubl
     __syncthreads();
   };
 
-  // forces F = -dU/dR and the potential energy of the replica
-  auto forces = [&]() -> double {
+  // forces F = -dU/dR and, with E_ true, the potential energy of the replica
+  // (only the last step of a trajectory needs it)
+  auto forces = [&](auto E_) -> double {
+    constexpr bool WANT_E = decltype(E_)::value;
     build_cells();
     const int w = (M >= 2 * DIV + 1) ? DIV : 0;   // w = 0: one cell holds every atom
     // cutoff^2 in cell units, with margins far above the FP32 rounding of the
@@ -222,7 +225,7 @@ k_hmc(const ReplicaDesc* __restrict__ desc, const int* __restrict__ order, doubl
                   fx += (double)(g * ddx);
                   fy += (double)(g * ddy);
                   fz += (double)(g * ddz);
-                  ei += (double)(c0 * sr6 * (sr6 - 1.0f));
+                  if (WANT_E) ei += (double)(c0 * sr6 * (sr6 - 1.0f));
                 }
               } else {
                 if (j == i) continue;
@@ -242,7 +245,7 @@ k_hmc(const ReplicaDesc* __restrict__ desc, const int* __restrict__ order, doubl
                   fx += g * ex;
                   fy += g * ey;
                   fz += g * ez;
-                  ei += 4.0 * (sr12 - sr6);
+                  if (WANT_E) ei += 4.0 * (sr12 - sr6);
                 }
               }
             }
@@ -253,11 +256,11 @@ k_hmc(const ReplicaDesc* __restrict__ desc, const int* __restrict__ order, doubl
       Fr[i] = make_double4(sc * fx, sc * fy, sc * fz, 0.0);
       e += ei;
     }
-    return 0.5 * block_total<BS>(e, red);
+    return WANT_E ? 0.5 * block_total<BS>(e, red) : 0.0;
   };
 
   if (mode == 0) {
-    double u = forces();
+    double u = forces(std::true_type());
     if (threadIdx.x == 0) U[r] = u;
     return;
   }
@@ -292,7 +295,8 @@ k_hmc(const ReplicaDesc* __restrict__ desc, const int* __restrict__ order, doubl
         Rr[i] = x;
       }
       __syncthreads();
-      U1 = forces();
+      if (s == nsteps - 1) U1 = forces(std::true_type());
+      else forces(std::false_type());
       for (int i = threadIdx.x; i < N; i += BS) {
         double4 p = Pr[i], f = Fr[i];
         p.x = p.x + hdt * f.x; p.y = p.y + hdt * f.y; p.z = p.z + hdt * f.z;
