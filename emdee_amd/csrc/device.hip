@@ -869,13 +869,21 @@ __device__ __forceinline__ void pair_body_mx(const DevPair& p, const DevCoul& cm
   a.fz += g * dz;
 }
 
-template <bool ENERGY, bool UNIFORM, bool LJ, int TPA>
+// LJ: 1 for the one-type LJ path (lj_path); 2 for layers whose pair models
+// are all lj_cut (no modifier or an energy shift) or none, with any Coulomb
+// model: each list entry then reads a 16-byte record of FP32 coefficients
+// (ljt: c0, c1, c2, eshift; zeros for pair none) and the pair's Coulomb
+// factor (ljk: kCoul, or 0 without Coulomb) instead of the full model record,
+// and evaluates the same terms as pair_body_mx (an uncharged pair adds zero
+// Coulomb terms); 0 for every other layer.
+template <bool ENERGY, bool UNIFORM, int LJ, int TPA>
 __global__ void __launch_bounds__(BLOCK)
 k_pairs_mx(double* __restrict__ F, double* __restrict__ partials, const int4* __restrict__ Q4,
            const double* __restrict__ R, const double4* __restrict__ Rs4, const int* __restrict__ nbr,
            const int* __restrict__ nnb, const float* __restrict__ qf, const DevPair* __restrict__ pairs,
            const DevCoul* __restrict__ coulp,
-           int ntypes, MixedCut mc, int N, int Npad, double* __restrict__ eatom) {
+           int ntypes, MixedCut mc, int N, int Npad, double* __restrict__ eatom, const float4* __restrict__ ljt,
+           const float* __restrict__ ljk) {
   int blk = xcd_block();
   int gt = blk * BLOCK + threadIdx.x;
   int i = TPA == 1 ? gt : gt / TPA, sub = TPA == 1 ? 0 : gt % TPA;
@@ -885,6 +893,7 @@ k_pairs_mx(double* __restrict__ F, double* __restrict__ partials, const int4* __
     const int4 qi = Q4[i];
     const float qqi = qf[i];
     const DevPair* prow = pairs + (qi.w & Q4_TYPE) * ntypes;
+    const int trow = (qi.w & Q4_TYPE) * ntypes;
     DevPair pu;
     if (UNIFORM) pu = pairs[0];   // uniform address: scalar loads, no VGPRs
     const int n = nnb[i];
@@ -898,8 +907,27 @@ k_pairs_mx(double* __restrict__ F, double* __restrict__ partials, const int4* __
       if (!in && r2 < mc.hi) in = exact_inside(Rs4, i, j, mc.Rc2s);
       if (in) {
         float invR2 = mc.K * __builtin_amdgcn_rcpf(r2);
-        if (LJ) {
+        if (LJ == 1) {
           pair_body_mx<ENERGY, true>(pu, cm, false, 0.0f, dx, dy, dz, invR2, a);
+        } else if (LJ == 2) {
+          const int t = trow + (qj.w & Q4_TYPE);
+          const float4 c = ljt[t];
+          const float qiqj = ljk[t] * qqi * qf[j];
+          float sr2 = c.z * invR2, sr6 = sr2 * sr2 * sr2, sr12 = sr6 * sr6;
+          float W = c.y * (sr12 + sr12 - sr6);
+          if (ENERGY) a.Ep += c.x * (sr12 - sr6) + c.w;
+          a.Wp += W;
+          float invR = sqrtf(invR2), E, Wq;
+          coul_model<float, ENERGY>(cm, invR, invR2, E, Wq);
+          apply_modifier<float>(cm.modifier, float(cm.eshift), float(cm.fshift), float(cm.factor), float(cm.Rm2fac),
+                                invR, invR2, E, Wq);
+          if (ENERGY) a.Ec += qiqj * E;
+          Wq = qiqj * Wq;
+          a.Wc += Wq;
+          float g = (W + Wq) * invR2;
+          a.fx += g * dx;
+          a.fy += g * dy;
+          a.fz += g * dz;
         } else if (UNIFORM) {
           bool jc = qqi != 0.0f && pu.coulomb && qf[j] != 0.0f;
           float qiqj = jc ? float(pu.kCoul) * qqi * qf[j] : 0.0f;
@@ -917,7 +945,7 @@ k_pairs_mx(double* __restrict__ F, double* __restrict__ partials, const int4* __
       Wp += (double)a.Wp; Wc += (double)a.Wc;
       if (ENERGY) { Ep += (double)a.Ep; Ec += (double)a.Ec; }
     };
-    if (LJ) {
+    if (LJ == 1) {
       // Branch-free groups of four: every entry is evaluated and masked, and
       // only a group holding an entry in the cutoff band takes the FP64 test.
       const float c0 = float(pu.c0), c1 = float(pu.c1), c2 = float(pu.c2), es = float(pu.eshift);
@@ -1255,6 +1283,9 @@ Device* dev_create(const DeviceSetup& s) {
   d->hostType = s.type;
   d->pairs.alloc((size_t)s.nlayers * s.ntypes * s.ntypes);
   d->couls.alloc(s.nlayers);
+  d->ljt.alloc((size_t)s.nlayers * s.ntypes * s.ntypes);
+  d->ljk.alloc((size_t)s.nlayers * s.ntypes * s.ntypes);
+  d->ljcOk.assign(s.nlayers, 0);
   d->interact.alloc((size_t)s.ntypes * s.ntypes);
   HIP_CHECK(hipMemsetAsync(d->interact.p, 0, s.ntypes * s.ntypes, d->stream));
   d->exOff.alloc(s.N + 1);
@@ -1269,7 +1300,7 @@ void dev_destroy(Device* d) {
   for (DBuf<double>* b : { &d->R, &d->P, &d->R0, &d->invMass, &d->charge, &d->F, &d->tmp3, &d->tmpF, &d->tmp1, &d->stage, &d->partials, &d->result, &d->trig, &d->shadow }) b->release();
   d->Rs4.release(); d->Q4.release(); d->qf.release(); d->eatom.release();
   for (DBuf<int>* b : { &d->type, &d->body, &d->perm, &d->freeFlag, &d->chargedFlag, &d->tmpI, &d->key, &d->cellCount, &d->cellStart, &d->order, &d->blockSums, &d->nbr, &d->nnb, &d->flags, &d->exOff, &d->exItem, &d->iperm }) b->release();
-  d->pairs.release(); d->couls.release(); d->interact.release();
+  d->pairs.release(); d->couls.release(); d->interact.release(); d->ljt.release(); d->ljk.release();
   dev_ewald_release(d);
   dev_structs_release(d);
   if (d->hstage) (void)hipHostFree(d->hstage);
@@ -1321,6 +1352,22 @@ void dev_set_models(Device* d, int layer, const std::vector<DevPair>& pairs, con
   std::copy(pairs.begin(), pairs.begin() + nt2, d->hostPairs.begin() + nt2 * layer);
   HIP_CHECK(hipMemcpyAsync(d->pairs.p + nt2 * layer, pairs.data(), sizeof(DevPair) * nt2, hipMemcpyHostToDevice, d->stream));
   HIP_CHECK(hipMemcpyAsync(d->couls.p + layer, &coul, sizeof(DevCoul), hipMemcpyHostToDevice, d->stream));
+  // compact FP32 records for the mixed kernels when every pair model of the
+  // layer is lj_cut (no modifier or an energy shift) or none
+  std::vector<float4> t(nt2);
+  std::vector<float> k(nt2);
+  bool ok = true;
+  for (size_t q = 0; q < nt2; q++) {
+    const DevPair& p = pairs[q];
+    bool lj = p.kind == PAIR_LJ_CUT, none = p.kind == PAIR_NONE;
+    ok = ok && (lj || none) && (p.modifier == MOD_NONE || p.modifier == MOD_SHIFTED);
+    float es = p.modifier == MOD_SHIFTED ? (float)p.eshift : 0.0f;
+    t[q] = lj ? make_float4((float)p.c0, (float)p.c1, (float)p.c2, es) : make_float4(0.0f, 0.0f, 0.0f, es);
+    k[q] = p.coulomb ? (float)p.kCoul : 0.0f;
+  }
+  d->ljcOk[layer] = ok;
+  HIP_CHECK(hipMemcpyAsync(d->ljt.p + nt2 * layer, t.data(), sizeof(float4) * nt2, hipMemcpyHostToDevice, d->stream));
+  HIP_CHECK(hipMemcpyAsync(d->ljk.p + nt2 * layer, k.data(), sizeof(float) * nt2, hipMemcpyHostToDevice, d->stream));
   HIP_CHECK(hipStreamSynchronize(d->stream));
 }
 
@@ -1613,25 +1660,33 @@ void dev_compute_pairs(Device* d, const PairCall& c, ForceSums* out) {
     mc.Rc2s = c.Rc2;
     mc.unit = c.L / two32;
     mc.L = c.L;
+    const float4* ljt = d->ljt.p + (size_t)c.layer * c.ntypes * c.ntypes;
+    const float* ljk = d->ljk.p + (size_t)c.layer * c.ntypes * c.ntypes;
+    const bool ljc = d->ljcOk[c.layer] != 0;
 #define LAUNCH_MX(E, U, J, T)                                                                                      \
   hipExtLaunchKernelGGL((k_pairs_mx<E, U, J, T>), dim3(gp), dim3(BLOCK), 0, d->stream, e0, e1, 0, F,                \
                         d->partials.p, (const int4*)d->Q4.p, (const double*)d->R.p, (const double4*)d->Rs4.p,     \
                         (const int*)d->nbr.p, (const int*)d->nnb.p, (const float*)d->qf.p, pt, cp, c.ntypes, mc, N, \
-                        Np, ea)
+                        Np, ea, ljt, ljk)
+#define GENERIC_MX_J(E, U, J)                                                                                      \
+  do {                                                                                                            \
+    if (tpa == 16) LAUNCH_MX(E, U, J, 16); else if (tpa == 4) LAUNCH_MX(E, U, J, 4);                              \
+    else LAUNCH_MX(E, U, J, 1);                                                                                   \
+  } while (0)
 #define GENERIC_MX(E, U)                                                                                           \
   do {                                                                                                            \
-    if (tpa == 16) LAUNCH_MX(E, U, false, 16); else if (tpa == 4) LAUNCH_MX(E, U, false, 4);                      \
-    else LAUNCH_MX(E, U, false, 1);                                                                               \
+    if (ljc) GENERIC_MX_J(E, U, 2); else GENERIC_MX_J(E, U, 0);                                                   \
   } while (0)
     switch (variant) {
-      case 7: LAUNCH_MX(true, true, true, 1); break;
+      case 7: LAUNCH_MX(true, true, 1, 1); break;
       case 6: GENERIC_MX(true, true); break;
       case 4: GENERIC_MX(true, false); break;
-      case 3: LAUNCH_MX(false, true, true, 1); break;
+      case 3: LAUNCH_MX(false, true, 1, 1); break;
       case 2: GENERIC_MX(false, true); break;
       default: GENERIC_MX(false, false); break;
     }
 #undef GENERIC_MX
+#undef GENERIC_MX_J
 #undef LAUNCH_MX
   } else {
 #define LAUNCH_PAIRS(E, U, J, T)                                                                                    \

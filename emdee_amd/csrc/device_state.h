@@ -183,6 +183,11 @@ struct Device {
   unsigned long long* hseqDev = nullptr;
   unsigned long long seq = 0;
   unsigned long long pairSeq = 0;   // post of the last pair evaluation (slot POST_PAIRS)
+  // per layer: FP32 records {c0, c1, c2, eshift} and Coulomb factors of the
+  // type pairs, used by the mixed kernels when ljcOk (k_pairs_mx, LJ = 2)
+  DBuf<float4> ljt;
+  DBuf<float> ljk;
+  std::vector<char> ljcOk;
   unsigned long long trigSeq = 0;   // post of the last rebuild trigger (slot 0)
   double trigLast = -1.0, trigPrev = -1.0;   // trigger values of the last two steps (-1: none)
   DBuf<double> trig;      // [0] max, [1] first original index holding it, [2] max before it
