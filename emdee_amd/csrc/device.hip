@@ -712,6 +712,34 @@ __device__ __forceinline__ void pair_body(const DevPair& p, const DevCoul& cm, b
   a.fz += (double)(g * T(dz));
 }
 
+// pair_body for a layer of lj_cut/none pair models (FP64): c = {c0, c1, c2,
+// eshift} of the type pair (zeros for pair none; eshift zero without the
+// shifted modifier), the same terms in the same order.
+template <bool ENERGY>
+__device__ __forceinline__ void pair_body_ljc(const double4& c, const DevCoul& cm, bool jc, double qiqj, double dx,
+                                              double dy, double dz, double r2, double invL2, PairAcc& a) {
+  double invR2 = invL2 * fast_rcp(r2);
+  double sr2 = c.z * invR2, sr6 = sr2 * sr2 * sr2, sr12 = sr6 * sr6;
+  double E = c.x * (sr12 - sr6), W = c.y * (sr12 + sr12 - sr6);
+  E = E + c.w;
+  if (ENERGY) a.Ep += E;
+  a.Wp += W;
+  double Wsum = W;
+  if (jc) {
+    double invR = sqrt(invR2);
+    coul_model<double, ENERGY>(cm, invR, invR2, E, W);
+    apply_modifier<double>(cm.modifier, cm.eshift, cm.fshift, cm.factor, cm.Rm2fac, invR, invR2, E, W);
+    if (ENERGY) a.Ec += qiqj * E;
+    double Wq = qiqj * W;
+    a.Wc += Wq;
+    Wsum = Wsum + Wq;
+  }
+  double g = Wsum * invR2;
+  a.fx += g * dx;
+  a.fy += g * dy;
+  a.fz += g * dz;
+}
+
 // UNIFORM: a single atom type, so the model record is loaded once.
 // TPA threads per atom (consecutive lanes) split an atom's list for small
 // systems that would not fill the chip with one thread per atom; their
@@ -723,20 +751,35 @@ __device__ __forceinline__ double group_sum(double v) {
   return v;
 }
 
-template <bool ENERGY, bool MIXED, bool UNIFORM, bool LJ, int TPA>
+// LJ: 1 for the one-type LJ path; 2 for layers of lj_cut/none pair models
+// with any Coulomb model, whose compact FP64 records (ljt: c0, c1, c2,
+// eshift; ljk: kCoul or 0) are staged in LDS; 0 for every other layer.
+template <bool ENERGY, bool MIXED, bool UNIFORM, int LJ, int TPA>
 __global__ void __launch_bounds__(BLOCK)
 k_pairs(double* __restrict__ F, double* __restrict__ partials,
         const double4* __restrict__ Rs4, const int* __restrict__ nbr, const int* __restrict__ nnb,
         const double* __restrict__ charge, const int* __restrict__ charged,
         const DevPair* __restrict__ pairs, const DevCoul* __restrict__ coulp,
-        int ntypes, double L, double invL2, double Rc2, int N, int Npad, double* __restrict__ eatom) {
+        int ntypes, double L, double invL2, double Rc2, int N, int Npad, double* __restrict__ eatom,
+        const double4* __restrict__ ljt, const double* __restrict__ ljk) {
   using T = typename std::conditional<MIXED, float, double>::type;
   int blk = xcd_block();
   int gt = blk * BLOCK + threadIdx.x;
   int i = TPA == 1 ? gt : gt / TPA, sub = TPA == 1 ? 0 : gt % TPA;
   PairAcc a = { 0, 0, 0, 0, 0, 0, 0 };
   __shared__ DevPair spair[PAIR_LDS_TYPES * PAIR_LDS_TYPES];
-  const bool lds = !UNIFORM && stage_pairs(spair, pairs, ntypes);
+  __shared__ double4 sljt[PAIR_LDS_TYPES * PAIR_LDS_TYPES];
+  __shared__ double sljk[PAIR_LDS_TYPES * PAIR_LDS_TYPES];
+  bool lds = false;
+  if (LJ == 2) {
+    lds = ntypes <= PAIR_LDS_TYPES;
+    if (lds) {
+      for (int t = threadIdx.x; t < ntypes * ntypes; t += blockDim.x) { sljt[t] = ljt[t]; sljk[t] = ljk[t]; }
+      __syncthreads();
+    }
+  } else if (!UNIFORM) {
+    lds = stage_pairs(spair, pairs, ntypes);
+  }
   if (i < N) {
     const DevCoul cm = *coulp;
     const double4 ri = Rs4[i];
@@ -758,8 +801,17 @@ k_pairs(double* __restrict__ F, double* __restrict__ partials,
         r2 = dx * dx + dy * dy + dz * dz;
       }
       if (r2 < Rc2) {
-        if (LJ) {
+        if (LJ == 1) {
           pair_body<T, ENERGY, true>(pu, cm, false, 0.0, dx, dy, dz, r2, invL2, a);
+        } else if (LJ == 2) {
+          const int t = ti * ntypes + (int)rj.w;
+          double4 c;
+          double kc;
+          if (lds) { c = sljt[t]; kc = sljk[t]; } else { c = ljt[t]; kc = ljk[t]; }
+          double qj = charge[j];
+          bool jc = ich && kc != 0.0 && fabs(qj) > Q_EPS;
+          double qiqj = jc ? kc * qi * qj : 0.0;
+          pair_body_ljc<ENERGY>(c, cm, jc, qiqj, dx, dy, dz, r2, invL2, a);
         } else if (UNIFORM) {
           bool jc = ich && pu.coulomb && charged[j];
           double qiqj = jc ? pu.kCoul * qi * charge[j] : 0.0;
@@ -1285,6 +1337,8 @@ Device* dev_create(const DeviceSetup& s) {
   d->couls.alloc(s.nlayers);
   d->ljt.alloc((size_t)s.nlayers * s.ntypes * s.ntypes);
   d->ljk.alloc((size_t)s.nlayers * s.ntypes * s.ntypes);
+  d->ljt64.alloc((size_t)s.nlayers * s.ntypes * s.ntypes);
+  d->ljk64.alloc((size_t)s.nlayers * s.ntypes * s.ntypes);
   d->ljcOk.assign(s.nlayers, 0);
   d->interact.alloc((size_t)s.ntypes * s.ntypes);
   HIP_CHECK(hipMemsetAsync(d->interact.p, 0, s.ntypes * s.ntypes, d->stream));
@@ -1300,7 +1354,8 @@ void dev_destroy(Device* d) {
   for (DBuf<double>* b : { &d->R, &d->P, &d->R0, &d->invMass, &d->charge, &d->F, &d->tmp3, &d->tmpF, &d->tmp1, &d->stage, &d->partials, &d->result, &d->trig, &d->shadow }) b->release();
   d->Rs4.release(); d->Q4.release(); d->qf.release(); d->eatom.release();
   for (DBuf<int>* b : { &d->type, &d->body, &d->perm, &d->freeFlag, &d->chargedFlag, &d->tmpI, &d->key, &d->cellCount, &d->cellStart, &d->order, &d->blockSums, &d->nbr, &d->nnb, &d->flags, &d->exOff, &d->exItem, &d->iperm }) b->release();
-  d->pairs.release(); d->couls.release(); d->interact.release(); d->ljt.release(); d->ljk.release();
+  d->pairs.release(); d->couls.release(); d->interact.release(); d->ljt.release(); d->ljk.release(); d->ljt64.release();
+  d->ljk64.release();
   dev_ewald_release(d);
   dev_structs_release(d);
   if (d->hstage) (void)hipHostFree(d->hstage);
@@ -1354,20 +1409,29 @@ void dev_set_models(Device* d, int layer, const std::vector<DevPair>& pairs, con
   HIP_CHECK(hipMemcpyAsync(d->couls.p + layer, &coul, sizeof(DevCoul), hipMemcpyHostToDevice, d->stream));
   // compact FP32 records for the mixed kernels when every pair model of the
   // layer is lj_cut (no modifier or an energy shift) or none
+  // (and their FP64 originals for the FP64 kernels)
   std::vector<float4> t(nt2);
   std::vector<float> k(nt2);
+  std::vector<double4> t64(nt2);
+  std::vector<double> k64(nt2);
   bool ok = true;
   for (size_t q = 0; q < nt2; q++) {
     const DevPair& p = pairs[q];
     bool lj = p.kind == PAIR_LJ_CUT, none = p.kind == PAIR_NONE;
     ok = ok && (lj || none) && (p.modifier == MOD_NONE || p.modifier == MOD_SHIFTED);
-    float es = p.modifier == MOD_SHIFTED ? (float)p.eshift : 0.0f;
-    t[q] = lj ? make_float4((float)p.c0, (float)p.c1, (float)p.c2, es) : make_float4(0.0f, 0.0f, 0.0f, es);
-    k[q] = p.coulomb ? (float)p.kCoul : 0.0f;
+    double es = p.modifier == MOD_SHIFTED ? p.eshift : 0.0;
+    t64[q] = lj ? make_double4(p.c0, p.c1, p.c2, es) : make_double4(0.0, 0.0, 0.0, es);
+    k64[q] = p.coulomb ? p.kCoul : 0.0;
+    t[q] = make_float4((float)t64[q].x, (float)t64[q].y, (float)t64[q].z, (float)t64[q].w);
+    k[q] = (float)k64[q];
   }
   d->ljcOk[layer] = ok;
   HIP_CHECK(hipMemcpyAsync(d->ljt.p + nt2 * layer, t.data(), sizeof(float4) * nt2, hipMemcpyHostToDevice, d->stream));
   HIP_CHECK(hipMemcpyAsync(d->ljk.p + nt2 * layer, k.data(), sizeof(float) * nt2, hipMemcpyHostToDevice, d->stream));
+  HIP_CHECK(hipMemcpyAsync(d->ljt64.p + nt2 * layer, t64.data(), sizeof(double4) * nt2, hipMemcpyHostToDevice,
+                           d->stream));
+  HIP_CHECK(hipMemcpyAsync(d->ljk64.p + nt2 * layer, k64.data(), sizeof(double) * nt2, hipMemcpyHostToDevice,
+                           d->stream));
   HIP_CHECK(hipStreamSynchronize(d->stream));
 }
 
@@ -1689,25 +1753,33 @@ void dev_compute_pairs(Device* d, const PairCall& c, ForceSums* out) {
 #undef GENERIC_MX_J
 #undef LAUNCH_MX
   } else {
+    const double4* ljt = d->ljt64.p + (size_t)c.layer * c.ntypes * c.ntypes;
+    const double* ljk = d->ljk64.p + (size_t)c.layer * c.ntypes * c.ntypes;
+    const bool ljc = d->ljcOk[c.layer] != 0;
 #define LAUNCH_PAIRS(E, U, J, T)                                                                                    \
   hipExtLaunchKernelGGL((k_pairs<E, false, U, J, T>), dim3(gp), dim3(BLOCK), 0, d->stream, e0, e1, 0, F,           \
                         d->partials.p, (const double4*)d->Rs4.p, (const int*)d->nbr.p, (const int*)d->nnb.p,       \
                         (const double*)d->charge.p, (const int*)d->chargedFlag.p, pt, cp, c.ntypes, c.L, c.invL2,   \
-                        c.Rc2, N, Np, ea)
+                        c.Rc2, N, Np, ea, ljt, ljk)
+#define GENERIC_J(E, U, J)                                                                                         \
+  do {                                                                                                            \
+    if (tpa == 16) LAUNCH_PAIRS(E, U, J, 16); else if (tpa == 4) LAUNCH_PAIRS(E, U, J, 4);                        \
+    else LAUNCH_PAIRS(E, U, J, 1);                                                                                \
+  } while (0)
 #define GENERIC(E, U)                                                                                              \
   do {                                                                                                            \
-    if (tpa == 16) LAUNCH_PAIRS(E, U, false, 16); else if (tpa == 4) LAUNCH_PAIRS(E, U, false, 4);                \
-    else LAUNCH_PAIRS(E, U, false, 1);                                                                            \
+    if (ljc) GENERIC_J(E, U, 2); else GENERIC_J(E, U, 0);                                                         \
   } while (0)
     switch (variant) {
-      case 7: LAUNCH_PAIRS(true, true, true, 1); break;
+      case 7: LAUNCH_PAIRS(true, true, 1, 1); break;
       case 6: GENERIC(true, true); break;
       case 4: GENERIC(true, false); break;
-      case 3: LAUNCH_PAIRS(false, true, true, 1); break;
+      case 3: LAUNCH_PAIRS(false, true, 1, 1); break;
       case 2: GENERIC(false, true); break;
       default: GENERIC(false, false); break;
     }
 #undef GENERIC
+#undef GENERIC_J
 #undef LAUNCH_PAIRS
   }
   launch_check("pairs");

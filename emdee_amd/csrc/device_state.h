@@ -187,6 +187,8 @@ struct Device {
   // type pairs, used by the mixed kernels when ljcOk (k_pairs_mx, LJ = 2)
   DBuf<float4> ljt;
   DBuf<float> ljk;
+  DBuf<double4> ljt64;   // the same records in FP64 for the FP64 kernels
+  DBuf<double> ljk64;
   std::vector<char> ljcOk;
   unsigned long long trigSeq = 0;   // post of the last rebuild trigger (slot 0)
   double trigLast = -1.0, trigPrev = -1.0;   // trigger values of the last two steps (-1: none)
