@@ -459,7 +459,7 @@ __device__ __forceinline__ void scan_range(int a, int b, int i, double xi, doubl
       if (!in && (r2 < hi || odd)) in = exact_inside(Rs4, i, j, xRc2);
       if (!in || j == i) return false;
       if (body[j] == bi) return false;
-      if (!interact[(qj.w & Q4_TYPE) * ntypes + ti]) return false;
+      if (interact && !interact[(qj.w & Q4_TYPE) * ntypes + ti]) return false;   // null: all types interact
       if (e1 > e0) {
         int oj = perm[j];
         for (int e = e0; e < e1; e++) if (exItem[e] == oj) return false;
@@ -526,7 +526,7 @@ __device__ __forceinline__ void scan_range(int a, int b, int i, double xi, doubl
     if (r2 < xRc2 && j != i) {
       if (!SIMPLE) {
         if (body[j] == bi) return;
-        if (!interact[(int)rj.w * ntypes + ti]) return;
+        if (interact && !interact[(int)rj.w * ntypes + ti]) return;
         if (e1 > e0) {
           int oj = perm[j];
           for (int e = e0; e < e1; e++) if (exItem[e] == oj) return;
@@ -1640,7 +1640,8 @@ void dev_rebuild(Device* d, double L, double xRc, double xInRc) {
     hipEvent_t e0 = d->profiling ? d->ev0 : nullptr, e1 = d->profiling ? d->ev1 : nullptr;
     hipExtLaunchKernelGGL(kern, dim3(gl), dim3(BLOCK), 0, d->stream, e0, e1, 0, d->nbr.p, d->nnb.p, d->flags.p,
                           d->partials.p, (const double4*)d->Rs4.p, (const int*)d->key.p, (const int*)d->cellStart.p,
-                          (const int*)d->body.p, (const int*)d->perm.p, (const uint8_t*)d->interact.p,
+                          (const int*)d->body.p, (const int*)d->perm.p,
+                          d->allInteract ? nullptr : (const uint8_t*)d->interact.p,
                           (const int*)d->exOff.p, (const int*)d->exItem.p, d->nexcl, d->ntypes, M, xRc2, d->cap, N, Np,
                           (const int4*)d->Q4.p, lo, hi);
     launch_check("build_list");
