@@ -13,6 +13,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <string>
+#include <unordered_map>
 #include <vector>
 
 #include "../../include/emdee.h"
@@ -426,21 +427,25 @@ void set_kinetic(tEmDee* md, const double* twoKEt, const double* twoKEr) {
   md->Kinetic.UpToDate = true;
 }
 
-// EmDeeData.f90:268-355 (body bookkeeping; rigid-body dynamics are rejected
-// at initialization by this build)
+// EmDeeData.f90:268-355 (body bookkeeping).  Bodies are numbered as
+// clean_body_indices numbers them: ids that occur more than once, in order of
+// first appearance; ids that occur once (and ids <= 0) mark free atoms.  The
+// reference finds the rank of an id by a linear search over the ids seen so
+// far; a hash map gives the same rank in O(1).
 void allocate_rigid_bodies(System* me, const int* bodies) {
   int N = me->natoms;
   me->atomBody.assign(N, 0);
   if (bodies) {
     std::vector<int> index(N + 1, 0), saved(N + 1, 0), amount(N + 1, 0), first(N + 1, 0);
+    std::unordered_map<int, int> rank;
+    rank.reserve((size_t)N);
     int n = 0;
     for (int i = 1; i <= N; i++) {
       int ib = bodies[i - 1];
       if (ib > 0) {
-        int j = 0; bool not_found = true;
-        while (not_found && j < n) { j++; not_found = saved[j] != ib; }
-        if (not_found) { n++; amount[n] = 1; saved[n] = ib; first[n] = i; index[i] = 0; }
-        else { amount[j]++; index[i] = j; index[first[j]] = j; }
+        auto it = rank.find(ib);
+        if (it == rank.end()) { n++; rank.emplace(ib, n); amount[n] = 1; saved[n] = ib; first[n] = i; index[i] = 0; }
+        else { int j = it->second; amount[j]++; index[i] = j; index[first[j]] = j; }
       }
     }
     int i = 0;

@@ -10,10 +10,11 @@
 //     consecutive words.  A full list (every pair stored twice) lets one
 //     thread own one atom's force: no atomics, deterministic sums.
 //   * the list is rebuilt when the reference trigger fires
-//     (neighbor_lists.f90:41-59, 175-195): cells of side >= (rc+skin)/2, a
+//     (neighbor_lists.f90:41-59, 175-195): cells of side >= (rc+skin)/3, a
 //     counting sort of atoms by cell (stable: ascending original index within
-//     a cell), then a 125-cell scan per atom with the reference's FP64
-//     distance test on R/L.
+//     a cell), then a scan of the 7 x 7 rows of cells around each atom's cell
+//     (each row one contiguous slot range, rows outside the cutoff sphere
+//     culled) with the reference's FP64 distance test on R/L.
 //
 // All launches go to one stream per system.  Host syncs happen only where
 // the C ABI must return host-visible values (energies, virial, builds).

@@ -15,7 +15,8 @@
  *
  * Pinning.  tests/test_oracle_*.py check this restatement against (i) the
  * four [U, W, U+K] energy pins held by the reference's own tests
- * (test/test_pair_lj_*.f90:46), (ii) brute-force all-pairs neighbor sets,
+ * (test/test_pair_lj_*.f90:46; the fourth under the earlier shifted-smoothed
+ * rule, see oracle_set_legacy_shifted_smoothing), (ii) brute-force all-pairs neighbor sets,
  * (iii) closed forms of every pair, Coulomb and bonded model, (iv) textbook
  * Ewald results (NaCl Madelung constant) and (v) rigid-body invariants.
  * Functions without such a check are listed as "parity unpinned" in the
@@ -775,6 +776,16 @@ static double nb_energy(const omodel_t *m, double invR, double invR2) {
   return is_pair(m) ? pair_energy(m, invR, invR2) : coul_energy(m, invR, invR2);
 }
 
+/* Earlier shifted-smoothed rule, kept only so that the checker can reproduce
+   the energy pin of test/test_pair_lj_shifted_smoothed.f90:46, which was
+   produced by an earlier version of the library than the src/ snapshot: the
+   energy shift is -E(rc) (not the snapshot's -(E(Rm) + E(rc))/2,
+   modelClass_nonbonded.f90:281-290) and it is added to the pair energy only
+   inside the switching region (the snapshot adds it at every distance,
+   apply_modifier.f90:17-20,38-41).  Off unless a test turns it on. */
+static int legacy_shifted_smoothing = 0;
+void oracle_set_legacy_shifted_smoothing(int on) { legacy_shifted_smoothing = on; }
+
 /* modelClass_nonbonded.f90:245-296 */
 static void modifier_setup(omodel_t *m, double cutoff) {
   double Ec = 0, Wc = 0, Es, Ws;
@@ -789,12 +800,12 @@ static void modifier_setup(omodel_t *m, double cutoff) {
     case MOD_SHIFTED: m->eshift = -Ec; break;
     case MOD_SHIFTED_FORCE: m->eshift = -(Ec + Wc); m->fshift = Wc / cutoff; break;
     case MOD_SMOOTHED: case MOD_SHIFTED_SMOOTHED:
-      if (shifting) { nb_compute(m, &Es, &Ws, ONE / m->Rm, ONE / m->RmSq); m->eshift = -HALF * (Es + Ec); }
+      if (shifting) { nb_compute(m, &Es, &Ws, ONE / m->Rm, ONE / m->RmSq); m->eshift = legacy_shifted_smoothing ? -Ec : -HALF * (Es + Ec); }
       m->factor = ONE / (cutoff - m->Rm);
       m->Rm2fac = m->factor * m->Rm;
       break;
     case MOD_SQUARE_SMOOTHED: case MOD_SHIFTED_SQUARE_SMOOTHED:
-      if (shifting) { nb_compute(m, &Es, &Ws, ONE / m->Rm, ONE / m->RmSq); m->eshift = -HALF * (Es + Ec); }
+      if (shifting) { nb_compute(m, &Es, &Ws, ONE / m->Rm, ONE / m->RmSq); m->eshift = legacy_shifted_smoothing ? -Ec : -HALF * (Es + Ec); }
       m->factor = ONE / (cutoff * cutoff - m->RmSq);
       m->Rm2fac = m->factor * m->RmSq;
       break;
@@ -1785,8 +1796,8 @@ static void apply_modifier(const omodel_t *m, int compute, double invR, double i
       if (compute) *Eij = *Eij + m->eshift + rFc;
       break;
     case MOD_SMOOTHED: case MOD_SHIFTED_SMOOTHED:
-      if (compute) *Eij = *Eij + m->eshift;
       r2fac = m->factor / invR;
+      if (compute && (!legacy_shifted_smoothing || r2fac > m->Rm2fac)) *Eij = *Eij + m->eshift;
       if (r2fac > m->Rm2fac) {
         if (!compute) { *Eij = is_p ? pair_energy(m, invR, invR2) : coul_energy(m, invR, invR2); *Eij = *Eij + m->eshift; }
         u = r2fac - m->Rm2fac; u2 = u * u; u3 = u * u2;
@@ -1797,8 +1808,8 @@ static void apply_modifier(const omodel_t *m, int compute, double invR, double i
       }
       break;
     case MOD_SQUARE_SMOOTHED: case MOD_SHIFTED_SQUARE_SMOOTHED:
-      if (compute) *Eij = *Eij + m->eshift;
       r2fac = m->factor / invR2;
+      if (compute && (!legacy_shifted_smoothing || r2fac > m->Rm2fac)) *Eij = *Eij + m->eshift;
       if (r2fac > m->Rm2fac) {
         if (!compute) { *Eij = is_p ? pair_energy(m, invR, invR2) : coul_energy(m, invR, invR2); *Eij = *Eij + m->eshift; }
         u = r2fac - m->Rm2fac; u2 = u * u; u3 = u * u2;

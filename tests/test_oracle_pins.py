@@ -39,3 +39,25 @@ def test_square_smoothed_pin_with_switch_width(oracle):
     # as literally written (skin = Rc - 1 = 2) the pin does not hold
     bad = smoothed_run(oracle, oracle.EmDee_square_smoothed, 2.0)
     assert np.max(np.abs(bad - np.array(pins["pins"]["lj_smoothed"]["expected"]))) > 1.0
+
+
+def test_shifted_square_smoothed_pin_under_earlier_shift(oracle):
+    """test_pair_lj_shifted_smoothed.f90:46 was produced by an earlier library
+    than the src/ snapshot.  Its trajectory and virial are the snapshot's
+    square-smoothed arithmetic with switch width 1.0 (as for the smoothed pin)
+    and the energy shift -E(rc); its energy adds that shift only inside the
+    switching region.  Under that rule the checker reproduces all three
+    numbers; under the snapshot's rule (shift -(E(Rm) + E(rc))/2 at every
+    distance, modelClass_nonbonded.f90:281-290) it cannot, and the gap is
+    large, so the pin discriminates between the two rules."""
+    pins = hosts.load_pins()
+    expected = np.array(pins["pins"]["lj_shifted_smoothed"]["expected"])
+    oracle.lib.oracle_set_legacy_shifted_smoothing(1)
+    try:
+        got = smoothed_run(oracle, oracle.EmDee_shifted_square_smoothed, 1.0)
+    finally:
+        oracle.lib.oracle_set_legacy_shifted_smoothing(0)
+    assert np.all(np.abs(got - expected) <= pins["tolerance"]), got - expected
+    snapshot = smoothed_run(oracle, oracle.EmDee_shifted_square_smoothed, 1.0)
+    # the snapshot's shift moves the forces inside the switching region
+    assert np.max(np.abs(snapshot - expected)) > 100.0
