@@ -9,6 +9,7 @@ import glob
 import os
 import subprocess
 import sys
+from concurrent.futures import ThreadPoolExecutor
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
@@ -32,17 +33,22 @@ def _run(cmd):
     return r
 
 
-def build(verbose=False):
+def build(verbose=False, jobs=None):
+    """Compiles every source (in parallel, one process per file) and links."""
     os.makedirs(OBJ, exist_ok=True)
-    objs = []
+    cmds, objs = [], []
     for src in sorted(glob.glob(os.path.join(CSRC, "*.hip"))):
         o = os.path.join(OBJ, os.path.basename(src) + ".o")
-        _run([HIPCC] + HIP_FLAGS + ["-c", "-o", o, src])
+        cmds.append([HIPCC] + HIP_FLAGS + ["-c", "-o", o, src])
         objs.append(o)
     for src in sorted(glob.glob(os.path.join(CSRC, "*.cpp"))):
         o = os.path.join(OBJ, os.path.basename(src) + ".o")
-        _run(["g++"] + CXX_FLAGS + ["-c", "-o", o, src])
+        cmds.append(["g++"] + CXX_FLAGS + ["-c", "-o", o, src])
         objs.append(o)
+    jobs = jobs or min(len(cmds), max(1, min(8, os.cpu_count() or 1)))
+    with ThreadPoolExecutor(max_workers=jobs) as pool:
+        for f in [pool.submit(_run, c) for c in cmds]:
+            f.result()
     tmp = LIB + ".tmp"
     _run([HIPCC, "-shared", "-fPIC", "--offload-arch=gfx950", "-Wl,-Bsymbolic", "-o", tmp] + objs)
     os.replace(tmp, LIB)
