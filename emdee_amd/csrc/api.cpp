@@ -1087,6 +1087,37 @@ void EmDee_rdf(tEmDee md, int bins, double Rc, int pairs, int itype[], int jtype
 
 // ---- extensions (include/emdee_gpu.h) --------------------------------------
 
+// EmDee_rdf with no allocation (no counterpart in the reference): the same
+// checks, counts and normalisation as EmDee_rdf above, with the device
+// buffers in the library's static scratch array and no host containers.
+void EmDeeGPU_rdf_noalloc(tEmDee md, int bins, double Rc, int pairs, int itype[], int jtype[], double g[]) {
+  const char* task = "radial distribution calculation";
+  const double Pi4_3 = 4.188790204786391;
+  System* me = S(md);
+  int nt = me->ntypes;
+  for (int p = 0; p < pairs; p++)
+    if (!ranged({ itype[p], jtype[p] }, nt)) fatal(task, "at least one provided type index is out of range");
+  if (bins < 1) fatal(task, "the number of bins must be positive");
+  if (pairs > RDF_NOALLOC_MAX_PAIRS) fatal(task, "more type pairs than the allocation-free form holds (64)");
+  int maxtype = 0;
+  for (int p = 0; p < pairs; p++) maxtype = std::max(maxtype, std::max(itype[p], jtype[p]));
+  if (maxtype > RDF_NOALLOC_MAX_TYPE) fatal(task, "type index above the allocation-free form's limit (31)");
+  if (pairs < 1) return;
+  int npt = (maxtype - 1) + maxtype * (maxtype - 1) / 2 + 1;   // symm1D(maxtype, maxtype)
+  if ((long)bins * npt > RDF_NOALLOC_MAX_COUNTS || (long)bins * pairs > RDF_NOALLOC_MAX_VALUES)
+    fatal(task, "more bins than the allocation-free form holds (65536 per type pair set)");
+  PhaseScope ps(me, false);
+  double NiNj[RDF_NOALLOC_MAX_PAIRS];
+  for (int p = 0; p < pairs; p++) {
+    long Ni = 0, Nj = 0;
+    for (int a = 0; a < me->natoms; a++) { Ni += me->atomType[a] == itype[p]; Nj += me->atomType[a] == jtype[p]; }
+    NiNj[p] = (double)Ni * (double)Nj;
+  }
+  double invL = 1.0 / me->L;
+  double norm = Pi4_3 * std::pow(Rc * invL / bins, 3);
+  dev_rdf_noalloc(me->dev, bins, Rc, me->L, itype, jtype, pairs, NiNj, npt, norm, g);
+}
+
 void EmDeeGPU_set_precision(tEmDee md, int mode) {
   System* me = S(md);
   me->precision = mode ? 1 : 0;

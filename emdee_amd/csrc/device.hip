@@ -27,6 +27,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <vector>
 
 #include "device_state.h"
@@ -1243,12 +1244,11 @@ __global__ void k_free_shadow_post(double* __restrict__ part, const double* __re
 // current neighbor lists, each pair once (original index a < b), with the
 // reference's distance arithmetic (Rs = invL*R, d - anint(d), no FMA so that
 // the bin of every pair is the reference's).  Integer atomics: exact counts.
-__global__ void k_rdf(unsigned long long* __restrict__ count, const double* __restrict__ R,
-                      const int* __restrict__ nbr, const int* __restrict__ nnb, const int* __restrict__ perm,
-                      const int* __restrict__ type, const uint8_t* __restrict__ pairOn, int nt, double invL,
-                      double Rc2, double binsByRc, int bins, int N, int Npad) {
-  int s = blockIdx.x * BLOCK + threadIdx.x;
-  if (s >= N) return;
+template <typename PairOn>
+__device__ __forceinline__ void rdf_atom(unsigned long long* __restrict__ count, const double* __restrict__ R,
+                                         const int* __restrict__ nbr, const int* __restrict__ nnb,
+                                         const int* __restrict__ perm, const int* __restrict__ type, PairOn on,
+                                         double invL, double Rc2, double binsByRc, int bins, int s, int Npad) {
   int a = perm[s], it = type[s] + 1;
   double ri[3] = { __dmul_rn(invL, R[s]), __dmul_rn(invL, R[Npad + s]), __dmul_rn(invL, R[2 * Npad + s]) };
   int n = nnb[s];
@@ -1256,7 +1256,7 @@ __global__ void k_rdf(unsigned long long* __restrict__ count, const double* __re
     int t = nbr[(size_t)k * Npad + s];
     if (perm[t] <= a) continue;
     int jt = type[t] + 1;
-    if (!pairOn[it * (nt + 1) + jt]) continue;
+    if (!on(it, jt)) continue;
     double r2 = 0.0;
     for (int x = 0; x < 3; x++) {
       double d = __dsub_rn(ri[x], __dmul_rn(invL, R[x * Npad + t]));
@@ -1265,11 +1265,70 @@ __global__ void k_rdf(unsigned long long* __restrict__ count, const double* __re
     }
     if (r2 < Rc2) {
       int bin = (int)__dmul_rn(sqrt(r2), binsByRc);
+      if (bin >= bins) continue;   // r rounds up to Rc (the reference would index past its last bin)
       int lo = min(it, jt) - 1, hi = max(it, jt) - 1;
       int sp = lo + (hi + 1) * hi / 2;   // symm1D - 1
       atomicAdd(&count[bin + (size_t)bins * sp], 1ULL);
     }
   }
+}
+
+__global__ void k_rdf(unsigned long long* __restrict__ count, const double* __restrict__ R,
+                      const int* __restrict__ nbr, const int* __restrict__ nnb, const int* __restrict__ perm,
+                      const int* __restrict__ type, const uint8_t* __restrict__ pairOn, int nt, double invL,
+                      double Rc2, double binsByRc, int bins, int N, int Npad) {
+  int s = blockIdx.x * BLOCK + threadIdx.x;
+  if (s >= N) return;
+  auto on = [pairOn, nt](int it, int jt) { return pairOn[it * (nt + 1) + jt] != 0; };
+  rdf_atom(count, R, nbr, nnb, perm, type, on, invL, Rc2, binsByRc, bins, s, Npad);
+}
+
+// ---- EmDeeGPU_rdf_noalloc: the same counts and normalisation as EmDee_rdf,
+// with every buffer in `scratch`, a file-scope device array (no allocation
+// per call).  The requested type pairs travel as a bit mask and the
+// per-pair normalisation as a table, both by value in the kernel arguments.
+constexpr int RDF_NOALLOC_COUNTS = RDF_NOALLOC_MAX_COUNTS;   // bins x type-pair slots
+constexpr int RDF_NOALLOC_VALUES = RDF_NOALLOC_MAX_VALUES;   // bins x requested pairs (the output g)
+constexpr int RDF_NOALLOC_PAIRS = RDF_NOALLOC_MAX_PAIRS;     // requested pairs per call
+constexpr int RDF_NOALLOC_TYPES = RDF_NOALLOC_MAX_TYPE;      // largest type index
+// 1 MiB on every GPU the library runs on, held until the process ends.
+static __device__ __align__(256) unsigned long long scratch[RDF_NOALLOC_COUNTS + RDF_NOALLOC_VALUES];
+
+struct RdfMask { unsigned int row[RDF_NOALLOC_TYPES + 1]; };   // bit jt of row[it]: pair (it, jt) requested
+struct RdfNorm {
+  int slot[RDF_NOALLOC_PAIRS];          // symm1D(i, j) - 1 of requested pair p
+  int same[RDF_NOALLOC_PAIRS];          // i == j
+  double NiNj[RDF_NOALLOC_PAIRS];       // (double) Ni * (double) Nj
+};
+
+__global__ void k_fill_u64(unsigned long long* __restrict__ p, int n) {
+  int i = blockIdx.x * BLOCK + threadIdx.x;
+  if (i < n) p[i] = 0ULL;
+}
+
+__global__ void k_rdf_masked(unsigned long long* __restrict__ count, const double* __restrict__ R,
+                             const int* __restrict__ nbr, const int* __restrict__ nnb, const int* __restrict__ perm,
+                             const int* __restrict__ type, RdfMask mask, double invL, double Rc2, double binsByRc,
+                             int bins, int N, int Npad) {
+  int s = blockIdx.x * BLOCK + threadIdx.x;
+  if (s >= N) return;
+  auto on = [&mask](int it, int jt) {
+    return it <= RDF_NOALLOC_TYPES && jt <= RDF_NOALLOC_TYPES && ((mask.row[it] >> jt) & 1u) != 0;
+  };
+  rdf_atom(count, R, nbr, nnb, perm, type, on, invL, Rc2, binsByRc, bins, s, Npad);
+}
+
+// g(bin, p) in the arithmetic order of EmDee_rdf's host normalisation
+// (EmDeeCode.f90:1336-1350): count / norm / (3 b (b - 1) + 1), doubled for
+// like pairs, over Ni Nj.
+__global__ void k_rdf_normalise(double* __restrict__ g, const unsigned long long* __restrict__ count, RdfNorm t,
+                                double norm, int bins, int pairs) {
+  int idx = blockIdx.x * BLOCK + threadIdx.x;
+  if (idx >= bins * pairs) return;
+  int b = idx % bins, p = idx / bins;
+  double r = __ddiv_rn(__ddiv_rn((double)(long long)count[b + (size_t)bins * t.slot[p]], norm),
+                       (double)(3 * (b + 1) * b + 1));
+  g[idx] = __ddiv_rn(t.same[p] ? __dmul_rn(2.0, r) : r, t.NiNj[p]);
 }
 
 // ------------------------------------------------------------ host side
@@ -1279,8 +1338,34 @@ void launch_check(const char* what) {
   if (e != hipSuccess) fatal("HIP kernel launch", std::string(what) + ": " + hipGetErrorString(e));
 }
 
+// Device address of `scratch` on the current device.  Each GPU holds its own
+// copy of the array at its own address, looked up once per device (with that
+// device current) by hipGetSymbolAddress; the host-side name `scratch` is not
+// a device address and is never passed to a kernel or a copy.
+namespace {
+constexpr int MAX_DEVICES = 64;
+std::mutex scratchMutex;
+void* scratchAddress[MAX_DEVICES] = {};
+}  // namespace
+
+static void* scratch_address() {
+  int dev = 0;
+  HIP_CHECK(hipGetDevice(&dev));
+  if (dev < 0 || dev >= MAX_DEVICES) fatal("system setup", "device index out of range");
+  std::lock_guard<std::mutex> lock(scratchMutex);
+  if (!scratchAddress[dev]) {
+    void* ptr = nullptr;
+    hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !ptr)
+      fatal("system setup", std::string("no device address for the scratch array: ") + hipGetErrorString(e));
+    scratchAddress[dev] = ptr;
+  }
+  return scratchAddress[dev];
+}
+
 Device* dev_create(const DeviceSetup& s) {
   Device* d = new Device();
+  d->scratch = scratch_address();
   d->N = s.N;
   d->Npad = div_up(std::max(s.N, 1), BLOCK) * BLOCK;
   d->ntypes = s.ntypes;
@@ -1906,6 +1991,42 @@ void dev_rdf(Device* d, int bins, double Rc, double L, const std::vector<uint8_t
   for (size_t k = 0; k < h.size(); k++) count[k] = (long long)h[k];
   c.release();
   on.release();
+}
+
+// EmDee_rdf with no allocation: counts and results in `scratch` (through the
+// address looked up at set-up), the pair mask and normalisation by value.
+// The caller has checked the limits (RDF_NOALLOC_*).  Calls that overlap on
+// one GPU share the array and race; each call clears the counts it uses.
+void dev_rdf_noalloc(Device* d, int bins, double Rc, double L, const int* itype, const int* jtype, int pairs,
+                     const double* NiNj, int npt, double norm, double* g) {
+  int N = d->N;
+  unsigned long long* count = static_cast<unsigned long long*>(d->scratch);
+  double* gdev = reinterpret_cast<double*>(count + RDF_NOALLOC_COUNTS);
+  RdfMask mask;
+  RdfNorm t;
+  for (int k = 0; k <= RDF_NOALLOC_TYPES; k++) mask.row[k] = 0u;
+  for (int p = 0; p < RDF_NOALLOC_PAIRS; p++) { t.slot[p] = 0; t.same[p] = 0; t.NiNj[p] = 1.0; }
+  for (int p = 0; p < pairs; p++) {
+    int i = itype[p], j = jtype[p];
+    mask.row[i] |= 1u << j;
+    mask.row[j] |= 1u << i;
+    int lo = std::min(i, j) - 1, hi = std::max(i, j) - 1;
+    t.slot[p] = lo + (hi + 1) * hi / 2;
+    t.same[p] = i == j;
+    t.NiNj[p] = NiNj[p];
+  }
+  int nc = bins * npt;
+  hipLaunchKernelGGL(k_fill_u64, dim3(div_up(nc, BLOCK)), dim3(BLOCK), 0, d->stream, count, nc);
+  if (N > 0 && d->built) {
+    double invL = 1.0 / L, invL2 = invL * invL, Rc2 = Rc * Rc * invL2, binsByRc = bins / (Rc * invL);
+    hipLaunchKernelGGL(k_rdf_masked, dim3(div_up(N, BLOCK)), dim3(BLOCK), 0, d->stream, count, d->R.p, d->nbr.p,
+                       d->nnb.p, d->perm.p, d->type.p, mask, invL, Rc2, binsByRc, bins, N, d->Npad);
+  }
+  hipLaunchKernelGGL(k_rdf_normalise, dim3(div_up(bins * pairs, BLOCK)), dim3(BLOCK), 0, d->stream, gdev, count, t,
+                     norm, bins, pairs);
+  launch_check("rdf (no allocation)");
+  HIP_CHECK(hipMemcpyAsync(g, gdev, sizeof(double) * bins * pairs, hipMemcpyDeviceToHost, d->stream));
+  HIP_CHECK(hipStreamSynchronize(d->stream));
 }
 
 void dev_set_atom_energies(Device* d, bool on) {

@@ -141,6 +141,7 @@ struct StructState;
 struct Device {
   int N = 0, Npad = 0, ntypes = 0, nlayers = 0;
   hipStream_t stream = nullptr;
+  void* scratch = nullptr;   // device address of this GPU's `scratch` array (device.hip)
   // per-atom SoA arrays (sorted order)
   DBuf<double> R, P, R0, invMass, charge, F;   // F holds nlayers * 3 * Npad
   DBuf<double4> Rs4;      // {x/L, y/L, z/L, type}: one 32-byte gather per neighbor

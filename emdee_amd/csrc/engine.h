@@ -160,6 +160,11 @@ void dev_shadow_post(Device* d, int layer, double dt, int mode, double* out3);
 // pairOn is the (nt+1)^2 table of requested type pairs (1-based types).
 void dev_rdf(Device* d, int bins, double Rc, double L, const std::vector<uint8_t>& pairOn, int npt,
              std::vector<long long>& count);
+// EmDee_rdf into g (bins x pairs) with no allocation; limits checked by the caller.
+constexpr int RDF_NOALLOC_MAX_COUNTS = 65536, RDF_NOALLOC_MAX_VALUES = 65536, RDF_NOALLOC_MAX_PAIRS = 64,
+              RDF_NOALLOC_MAX_TYPE = 31;
+void dev_rdf_noalloc(Device* d, int bins, double Rc, double L, const int* itype, const int* jtype, int pairs,
+                     const double* NiNj, int npt, double norm, double* g);
 
 // Atom energies: while on, force evaluations also write each atom's share of
 // the potential energy (half of each of its pair terms, its k-space energy

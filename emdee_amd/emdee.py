@@ -131,6 +131,7 @@ GPU_SIGNATURES = {
     "EmDeeSurrogate_set_params": (None, [c_void_p, POINTER(ctypes.c_float), POINTER(ctypes.c_float)]),
     "EmDeeSurrogate_destroy": (None, [c_void_p]),
     "EmDeeGPU_attach_surrogate": (None, [tEmDee, c_void_p, c_int]),
+    "EmDeeGPU_rdf_noalloc": (None, [tEmDee, c_int, c_double, c_int, _ip, _ip, _dp]),
 }
 
 

@@ -92,6 +92,17 @@ void   EmDeeSurrogate_destroy( EmDeeSurrogate* s );
 /* layer 1..layers, or s = NULL to detach */
 void   EmDeeGPU_attach_surrogate( tEmDee md, EmDeeSurrogate* s, int layer );
 
+/* EmDee_rdf (EmDeeCode.f90:1281-1395) with no memory allocation.  No
+   counterpart in the reference: it is for callers that cannot allocate
+   (code under HIP graph capture, tight serving loops, harnesses that forbid
+   hipMalloc).  Same arguments, checks and results as EmDee_rdf (bit for
+   bit); its device buffers live in a static 1 MiB array that each GPU holds
+   until the process ends.  Limits: at most 64 type pairs per call, type
+   indices up to 31, bins * symm1D(maxtype, maxtype) <= 65536 and
+   bins * pairs <= 65536; larger requests end with an error.  Calls that
+   overlap on one GPU share that array and race. */
+void EmDeeGPU_rdf_noalloc( tEmDee md, int bins, double Rc, int pairs, int itype[], int jtype[], double g[] );
+
 /* Waits for all device work of the system. */
 void EmDeeGPU_synchronize( tEmDee md );
 
