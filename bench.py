@@ -21,10 +21,15 @@ algorithm, OpenMP over the host cores) on the same system for a bounded
 number of steps.
 
 Also reported: batched HMC of 512 LJ replicas (effective samples/s), rigid
-SPC/E water of 98,304 atoms, and two-layer water (exact, r-RESPA 1:4 and
-short-layer step costs).
+SPC/E water of 98,304 atoms, two-layer water (exact, r-RESPA 1:4 and
+short-layer step costs), and the learned-electrostatics workload on 4096
+water molecules (water_hmc: HMC with the learned model as proposal force
+against exact-force HMC, effective samples/s of each and their ratio, the
+model's cost against the Ewald sum's, held-out force R^2, r-RESPA with the
+model as fast force).
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--natoms N] [--no-cpu] [--no-hmc] [--no-water]
+                    [--no-water-hmc]
 """
 import argparse
 import ctypes
@@ -236,6 +241,8 @@ def main():
     ap.add_argument("--hmc-cycles", type=int, default=60)
     ap.add_argument("--no-hmc", action="store_true")
     ap.add_argument("--no-water", action="store_true")
+    ap.add_argument("--no-water-hmc", action="store_true")
+    ap.add_argument("--water-hmc-cycles", type=int, default=120)
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build measures one GPU; --gpus {args.gpus} is not supported "
@@ -297,6 +304,13 @@ def main():
                  "two_layer_spce_12288": layers_bench(lib, 4096, 10)}
         print(f"# water: {water}", file=sys.stderr, flush=True)
 
+    water_hmc = None
+    if not args.no_water_hmc:
+        print("# learned-electrostatics water HMC: 4096 molecules", file=sys.stderr, flush=True)
+        from emdee_amd.surrogate import water_hmc_workload
+        water_hmc = water_hmc_workload(lib, 4096, kcut=1.2, cycles=args.water_hmc_cycles,
+                                       log=lambda *a: print("#", *a, file=sys.stderr, flush=True))
+
     line = {
         "metric": METRIC, "value": value, "unit": "atom-steps/s", "n_gpus": 1,
         "steps": args.steps, "warmup": args.warmup,
@@ -319,6 +333,7 @@ def main():
         "cpu_baseline": cpu,
         "hmc": hmc,
         "water": water,
+        "water_hmc": water_hmc,
     }
     print(json.dumps(line), flush=True)
 

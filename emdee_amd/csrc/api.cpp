@@ -313,6 +313,18 @@ void initialize_system(System* me, tEmDee* md) {
 // forces_begin returns true without waiting for the pair sums: the caller
 // queues more work (the kick of EmDee_boost) and reads them later with one
 // wait for both.
+// What an attached learned model needs of its system (surrogate.h).
+static SurrogateContext surrogate_context(System* me) {
+  SurrogateContext c;
+  c.L = me->L; c.Rc = me->Rc; c.skin = me->skin;
+  if (me->kspace_active) {
+    c.ewald = &me->ewald.dev;
+    c.alpha = me->ewald.alpha;
+    for (int l = me->nlayers; l >= 1; l--) if (me->coul[l - 1].requires_kspace) c.lambdaLayer = l;
+  }
+  return c;
+}
+
 static bool forces_begin(tEmDee* md, bool defer, ForceSums& fs) {
   System* me = S(*md);
   if (!me->initialized) fatal("force computation", "box and coordinates have not been defined");
@@ -385,7 +397,7 @@ static void forces_end(tEmDee* md, const ForceSums& fs) {
   }
   // learned model standing in for a costly term of this layer (no reference counterpart)
   if (me->surrogate && me->surrogateLayer == l) {
-    double Ws = 0, Es = dev_surrogate_apply(me->dev, me->surrogate, l - 1, me->L, &Ws);
+    double Ws = 0, Es = dev_surrogate_apply(me->dev, me->surrogate, l - 1, surrogate_context(me), &Ws);
     E[COUL_] += Es;
     W[COUL_] += Ws;
   }
@@ -1140,6 +1152,7 @@ void EmDeeGPU_atom_energies(tEmDee* md, double* e) {
 void EmDeeGPU_attach_surrogate(tEmDee md, EmDeeSurrogate* s, int layer) {
   System* me = S(md);
   if (s && (layer < 1 || layer > me->nlayers)) fatal("surrogate attachment", "selected layer is out of range");
+  if (s && surrogate_cutoff(s) > me->Rc) fatal("surrogate attachment", "the descriptor cutoff exceeds the system's cutoff");
   me->surrogate = s;
   me->surrogateLayer = s ? layer : 0;
   for (int l = 1; l <= me->nlayers; l++) me->forcesUpToDate[l] = 0;
@@ -1147,8 +1160,9 @@ void EmDeeGPU_attach_surrogate(tEmDee md, EmDeeSurrogate* s, int layer) {
 void EmDeeGPU_surrogate_sample(tEmDee md, EmDeeSurrogate* s, const double* labels) {
   System* me = S(md);
   if (!me->initialized) fatal("surrogate sampling", "box and coordinates have not been defined");
+  if (surrogate_cutoff(s) > me->Rc) fatal("surrogate sampling", "the descriptor cutoff exceeds the system's cutoff");
   PhaseScope ps(me, false);
-  dev_surrogate_add_sample(me->dev, s, me->L, labels);
+  dev_surrogate_add_sample(me->dev, s, surrogate_context(me), labels);
 }
 void EmDeeGPU_destroy(tEmDee* md) {
   if (!md || !md->Data) return;

@@ -1745,6 +1745,7 @@ void dev_rebuild(Device* d, double L, double xRc, double xInRc) {
   }
   d->listEntries = d->hresult[8];
   d->built = true;
+  d->listVersion++;
 }
 
 int dev_neighbor_pairs(Device* d, int max, int* out) {

@@ -161,6 +161,7 @@ struct Device {
   DBuf<int> nbr, nnb;
   DBuf<int> flags;        // [0] overflow max count
   bool built = false;
+  long listVersion = 0;   // incremented at every list build
   // models
   DBuf<DevPair> pairs;    // nlayers * nt * nt
   std::vector<DevPair> hostPairs;   // host copy of the pair table

@@ -31,7 +31,9 @@ namespace {
 const double TWO_PI = 2.0 * 3.14159265358979324;   // 2 pi with global.f90:38
 constexpr int EW_BLOCK = 64;                        // one wavefront per block
 constexpr int EW_ATOMS = EWALD_GROUP / EW_BLOCK;    // charges per thread in k_ew_structure
-constexpr int EW_TARGET_WAVES = 16384;              // k-range split target
+constexpr int EW_TARGET_WAVES = 16384;              // k-range split target of the structure factors
+constexpr int EW_TARGET_WAVES_F = 16384;            // ... and of the forces
+constexpr int EW_ADD_LANES = 16;                    // lanes per charge adding the partial forces
 }  // namespace
 
 struct EwaldState {
@@ -41,6 +43,7 @@ struct EwaldState {
   DBuf<double4> kv;
   DBuf<int> item, groupType, typeGroups;
   DBuf<double> q, lambda, Fpart, part, result, selfc;
+  DBuf<double> vfac;      // per k: 1 - k^2 / (2 alpha^2), when the virial of the sum is wanted
   DBuf<double2> T, Spart, S, sigma;
   std::vector<int> hostTypeGroups;
 };
@@ -55,7 +58,7 @@ void dev_ewald_release(Device* d) {
   if (!e) return;
   e->kn.release(); e->kv.release(); e->item.release(); e->groupType.release(); e->typeGroups.release();
   e->q.release(); e->lambda.release(); e->Fpart.release(); e->part.release(); e->result.release(); e->selfc.release();
-  e->T.release(); e->Spart.release(); e->S.release(); e->sigma.release();
+  e->vfac.release(); e->T.release(); e->Spart.release(); e->S.release(); e->sigma.release();
   delete e;
   d->ewald = nullptr;
 }
@@ -161,39 +164,63 @@ __global__ void __launch_bounds__(EW_BLOCK) k_ew_structure(double2* __restrict__
 }
 
 // S_I(k) (kspace_ewald.f90:226-228), sigma(k) = S(k) lambda (:284) and the
-// energy prefac * sum_I S_I . sigma_I (:287), one thread per k.
+// energy prefac * sum_I S_I . sigma_I (:287), one wavefront per k: lane g
+// adds the partials of groups g, g + 64, ... of each charged type, a fixed
+// shuffle tree completes S_I(k), and lane 0 forms sigma and the energy.  The
+// four k of a block are added in a fixed order into one partial per block.
+// With vfac (a partial k set standing in for part of the sum), the partials
+// also carry the virial of the energy, sum_k E_k (1 - k^2/(2 alpha^2)) =
+// -dE/dln(lambda) when positions and box scale by lambda: part[2 b + 1].
+constexpr int EW_SIGMA_MAXT = 8;
 __global__ void __launch_bounds__(BLOCK) k_ew_sigma(double2* __restrict__ S, double2* __restrict__ sigma,
                                                     double* __restrict__ part, const double2* __restrict__ Spart,
                                                     const int* __restrict__ typeGroups, const double* __restrict__ lam,
-                                                    const double4* __restrict__ kv, int nt, int nvecs) {
-  int k = blockIdx.x * blockDim.x + threadIdx.x;
-  double e[1] = { 0.0 };
+                                                    const double4* __restrict__ kv, const double* __restrict__ vfac,
+                                                    int nt, int nvecs) {
+  __shared__ double red[2][BLOCK / 64];
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const int k = blockIdx.x * (BLOCK / 64) + wv;
+  double e = 0.0, v = 0.0;
   if (k < nvecs) {
+    double sr[EW_SIGMA_MAXT], si[EW_SIGMA_MAXT];
     for (int t = 0; t < nt; t++) {
-      double sr = 0.0, si = 0.0;
-      for (int g = typeGroups[t]; g < typeGroups[t + 1]; g++) {
+      double a = 0.0, b = 0.0;
+      for (int g = typeGroups[t] + lane; g < typeGroups[t + 1]; g += 64) {
         double2 p = Spart[(size_t)g * nvecs + k];
-        sr += p.x;
-        si += p.y;
+        a += p.x;
+        b += p.y;
       }
-      S[(size_t)t * nvecs + k] = make_double2(sr, si);
-    }
-    double s = 0.0;
-    for (int t2 = 0; t2 < nt; t2++) {
-      double sr = 0.0, si = 0.0;
-      for (int t1 = 0; t1 < nt; t1++) {
-        double2 a = S[(size_t)t1 * nvecs + k];
-        sr += a.x * lam[t1 + nt * t2];
-        si += a.y * lam[t1 + nt * t2];
+      for (int off = 32; off > 0; off >>= 1) {
+        a += __shfl_xor(a, off, 64);
+        b += __shfl_xor(b, off, 64);
       }
-      sigma[(size_t)t2 * nvecs + k] = make_double2(sr, si);
-      double2 b = S[(size_t)t2 * nvecs + k];
-      s += b.x * sr + b.y * si;
+      sr[t] = a;
+      si[t] = b;
     }
-    e[0] = kv[k].w * s;
+    if (lane == 0) {
+      double s = 0.0;
+      for (int t = 0; t < nt; t++) S[(size_t)t * nvecs + k] = make_double2(sr[t], si[t]);
+      for (int t2 = 0; t2 < nt; t2++) {
+        double gr = 0.0, gi = 0.0;
+        for (int t1 = 0; t1 < nt; t1++) {
+          gr += sr[t1] * lam[t1 + nt * t2];
+          gi += si[t1] * lam[t1 + nt * t2];
+        }
+        sigma[(size_t)t2 * nvecs + k] = make_double2(gr, gi);
+        s += sr[t2] * gr + si[t2] * gi;
+      }
+      e = kv[k].w * s;
+      if (vfac) v = e * vfac[k];
+    }
   }
-  block_sum<1>(e);
-  if (threadIdx.x == 0) part[blockIdx.x] = e[0];
+  if (lane == 0) { red[0][wv] = e; red[1][wv] = v; }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double se = 0.0, sv = 0.0;
+    for (int w = 0; w < BLOCK / 64; w++) { se += red[0][w]; sv += red[1][w]; }
+    if (vfac) { part[2 * blockIdx.x] = se; part[2 * blockIdx.x + 1] = sv; }
+    else part[blockIdx.x] = se;
+  }
 }
 
 // Partial k-space forces over the k range of blockIdx.y:
@@ -249,38 +276,34 @@ __global__ void __launch_bounds__(EW_BLOCK) k_ew_forces(double* __restrict__ Fpa
 // F(slot of atom) += sum over k ranges (fixed order), kspace_ewald.f90:297;
 // with eatom, also the atom's reciprocal and self energies
 // (self: -lambda_II alpha q^2 / sqrt(pi), modelClass_kspace.f90:223-229).
+// EW_ADD_LANES lanes per charge: lane u adds the ranges u, u + 16, ... in
+// order, and a fixed butterfly adds the lanes' sums.
 __global__ void k_ew_force_add(double* __restrict__ F, double* __restrict__ eatom, const double* __restrict__ Fpart,
                                const int* __restrict__ item, const int* __restrict__ iperm,
                                const double* __restrict__ q, const int* __restrict__ groupType,
                                const double* __restrict__ selfc, int nslots, int ks, int Npad) {
-  int j = blockIdx.x * blockDim.x + threadIdx.x;
-  if (j >= nslots) return;
+  int gt = blockIdx.x * blockDim.x + threadIdx.x;
+  int j = gt / EW_ADD_LANES, u = gt % EW_ADD_LANES;
+  bool ok = j < nslots;
+  double f[4] = { 0.0, 0.0, 0.0, 0.0 };
+  if (ok) {
+    for (int y = u; y < ks; y += EW_ADD_LANES)
+#pragma unroll
+      for (int x = 0; x < 4; x++) f[x] += Fpart[((size_t)y * 4 + x) * nslots + j];
+  }
+#pragma unroll
+  for (int off = EW_ADD_LANES / 2; off > 0; off >>= 1)
+#pragma unroll
+    for (int x = 0; x < 4; x++) f[x] += __shfl_xor(f[x], off, EW_ADD_LANES);
+  if (!ok || u != 0) return;
   int i = item[j];
   if (i < 0) return;
-  double f[4] = { 0.0, 0.0, 0.0, 0.0 };
-  // the partials of four k ranges are loaded together (sixteen loads in
-  // flight), then added in range order
-  int y = 0;
-  for (; y + 4 <= ks; y += 4) {
-    double p[4][4];
-#pragma unroll
-    for (int u = 0; u < 4; u++)
-#pragma unroll
-      for (int x = 0; x < 4; x++) p[u][x] = Fpart[((size_t)(y + u) * 4 + x) * nslots + j];
-#pragma unroll
-    for (int u = 0; u < 4; u++)
-#pragma unroll
-      for (int x = 0; x < 4; x++) f[x] += p[u][x];
-  }
-  for (; y < ks; y++)
-    for (int x = 0; x < 4; x++) f[x] += Fpart[((size_t)y * 4 + x) * nslots + j];
   int s = iperm[i];
   for (int x = 0; x < 3; x++) F[x * (size_t)Npad + s] += f[x];
   if (eatom) eatom[s] += f[3] + selfc[groupType[j / EWALD_GROUP]] * q[j] * q[j];
 }
 
-void dev_ewald_setup(Device* d, const EwaldSetup& s) {
-  EwaldState* e = ewald_of(d);
+static void ewald_state_setup(EwaldState* e, const EwaldSetup& s) {
   e->nm = s.nm; e->nvecs = s.nvecs; e->nt = s.nt; e->nl = s.nl; e->nslots = s.nslots;
   e->ngroups = s.nslots / EWALD_GROUP;
   e->hostTypeGroups = s.typeGroups;
@@ -311,7 +334,7 @@ void dev_ewald_setup(Device* d, const EwaldSetup& s) {
   e->kchS = div_up(s.nvecs, e->ksS);
   e->ksS = div_up(s.nvecs, e->kchS);
   int nwF = s.nslots / EW_BLOCK;
-  e->ksF = std::max(1, std::min(s.nvecs, EW_TARGET_WAVES / std::max(1, nwF)));
+  e->ksF = std::max(1, std::min(s.nvecs, EW_TARGET_WAVES_F / std::max(1, nwF)));
   e->kchF = div_up(s.nvecs, e->ksF);
   e->ksF = div_up(s.nvecs, e->kchF);
   e->T.alloc(3 * (size_t)(s.nm + 1) * s.nslots);
@@ -319,48 +342,103 @@ void dev_ewald_setup(Device* d, const EwaldSetup& s) {
   e->S.alloc((size_t)s.nt * s.nvecs);
   e->sigma.alloc((size_t)s.nt * s.nvecs);
   e->Fpart.alloc((size_t)e->ksF * 4 * s.nslots);
+  if (s.nt > EW_SIGMA_MAXT) fatal("kspace set-up", "more than 8 charged types");
+  e->part.alloc(2 * (size_t)div_up(s.nvecs, BLOCK / 64));
+  e->result.alloc(2);
+}
+
+void dev_ewald_setup(Device* d, const EwaldSetup& s) {
+  EwaldState* e = ewald_of(d);
+  ewald_state_setup(e, s);
   std::vector<double> selfc((size_t)s.nl * s.nt);
   for (int l = 0; l < s.nl; l++)
     for (int t = 0; t < s.nt; t++)
       selfc[(size_t)l * s.nt + t] = -s.alpha * s.lambda[(size_t)l * s.nt * s.nt + t * s.nt + t] / std::sqrt(3.14159265358979324);
-  put(e->selfc, selfc);
-  e->part.alloc(div_up(s.nvecs, BLOCK));
-  e->result.alloc(1);
+  e->selfc.alloc(selfc.size());
+  HIP_CHECK(hipMemcpy(e->selfc.p, selfc.data(), selfc.size() * sizeof(double), hipMemcpyHostToDevice));
 }
 
-double dev_ewald_compute(Device* d, int layer, double L) {
-  EwaldState* e = d->ewald;
-  if (!e || e->nvecs == 0 || e->nslots == 0) return 0.0;
+// One evaluation of the sum over the state's k-vectors with the type-pair
+// weights lam and self-energy coefficients selfc (null: none): forces added
+// to F (slot order), atom shares to eatom (may be null); returns the energy
+// and, with the state's virial factors, its virial in *W.
+static double ewald_eval(Device* d, EwaldState* e, double* F, const double* lam, const double* selfc, double* eatom,
+                         double L, double* W) {
   // shapes the kernels assume
   if (e->nslots % EWALD_GROUP || e->ngroups * EWALD_GROUP != e->nslots || e->kchS * e->ksS < e->nvecs ||
       e->kchF * e->ksF < e->nvecs || e->nm < 1)
     fatal("kspace computation", "inconsistent k-space set-up");
+  if (eatom && !selfc) fatal("kspace computation", "atom energies need the self-energy coefficients");
   int Np = d->Npad;
-  double* F = d->F.p + 3 * (size_t)Np * layer;
+  const double* vf = W ? e->vfac.p : nullptr;
+  if (W && !vf) fatal("kspace computation", "no virial factors for this k set");
   hipLaunchKernelGGL(k_ew_tables, dim3(div_up(e->nslots, BLOCK)), dim3(BLOCK), 0, d->stream, e->T.p, e->item.p,
                      d->iperm.p, d->R.p, L, e->nm, e->nslots, Np);
   launch_check("ewald tables");
   hipLaunchKernelGGL(k_ew_structure, dim3(e->ngroups, e->ksS), dim3(EW_BLOCK), 0, d->stream, e->Spart.p, e->T.p,
                      e->q.p, e->kn.p, e->nm, e->nslots, e->nvecs, e->kchS);
   launch_check("ewald structure factors");
-  int gk = div_up(e->nvecs, BLOCK);
+  int gk = div_up(e->nvecs, BLOCK / 64);
   hipLaunchKernelGGL(k_ew_sigma, dim3(gk), dim3(BLOCK), 0, d->stream, e->S.p, e->sigma.p, e->part.p, e->Spart.p,
-                     e->typeGroups.p, e->lambda.p + (size_t)layer * e->nt * e->nt, e->kv.p, e->nt, e->nvecs);
+                     e->typeGroups.p, lam, e->kv.p, vf, e->nt, e->nvecs);
   launch_check("ewald sigma");
-  hipLaunchKernelGGL(k_reduce_partials<1>, dim3(1), dim3(BLOCK), 0, d->stream, e->result.p, e->part.p, gk);
+  if (vf) hipLaunchKernelGGL(k_reduce_partials<2>, dim3(1), dim3(BLOCK), 0, d->stream, e->result.p, e->part.p, gk);
+  else hipLaunchKernelGGL(k_reduce_partials<1>, dim3(1), dim3(BLOCK), 0, d->stream, e->result.p, e->part.p, gk);
   hipLaunchKernelGGL(k_ew_forces, dim3(e->nslots / EW_BLOCK, e->ksF), dim3(EW_BLOCK), 0, d->stream, e->Fpart.p,
                      e->T.p, e->q.p, e->kn.p, e->kv.p, e->sigma.p, e->groupType.p, e->nm, e->nslots, e->nvecs,
                      e->kchF);
   launch_check("ewald forces");
-  hipLaunchKernelGGL(k_ew_force_add, dim3(div_up(e->nslots, BLOCK)), dim3(BLOCK), 0, d->stream, F,
-                     d->wantAtomE ? d->eatom.p : nullptr, e->Fpart.p, e->item.p, d->iperm.p, e->q.p, e->groupType.p,
-                     e->selfc.p + (size_t)layer * e->nt, e->nslots, e->ksF, Np);
+  hipLaunchKernelGGL(k_ew_force_add, dim3(div_up((long)e->nslots * EW_ADD_LANES, BLOCK)), dim3(BLOCK), 0, d->stream, F,
+                     eatom, e->Fpart.p,
+                     e->item.p, d->iperm.p, e->q.p, e->groupType.p, selfc, e->nslots, e->ksF, Np);
   launch_check("ewald force add");
-  double E = 0;
-  HIP_CHECK(hipMemcpyAsync(d->hresult, e->result.p, sizeof(double), hipMemcpyDeviceToHost, d->stream));
+  HIP_CHECK(hipMemcpyAsync(d->hresult, e->result.p, 2 * sizeof(double), hipMemcpyDeviceToHost, d->stream));
   HIP_CHECK(hipStreamSynchronize(d->stream));
-  E = d->hresult[0];
-  return E;
+  if (W) *W = d->hresult[1];
+  return d->hresult[0];
+}
+
+double dev_ewald_compute(Device* d, int layer, double L) {
+  EwaldState* e = d->ewald;
+  if (!e || e->nvecs == 0 || e->nslots == 0) return 0.0;
+  double* F = d->F.p + 3 * (size_t)d->Npad * layer;
+  return ewald_eval(d, e, F, e->lambda.p + (size_t)layer * e->nt * e->nt, e->selfc.p + (size_t)layer * e->nt,
+                    d->wantAtomE ? d->eatom.p : nullptr, L, nullptr);
+}
+
+// A reciprocal-space sum over a partial k set (set-up by lowk_setup): one
+// weight table (layer 0 of s.lambda), no self energy, with its virial.
+EwaldState* dev_lowk_create(const EwaldSetup& s) {
+  if (s.nl != 1) fatal("kspace set-up", "a partial k set has one weight table");
+  EwaldState* e = new EwaldState();
+  ewald_state_setup(e, s);
+  std::vector<double> vf(s.nvecs), zero(s.nt, 0.0);
+  for (int k = 0; k < s.nvecs; k++) {
+    // kv = (2 prefac k, prefac): k^2 = |kv.xyz|^2 / (2 prefac)^2
+    double p2 = 2.0 * s.kv[4 * k + 3];
+    double kx = s.kv[4 * k] / p2, ky = s.kv[4 * k + 1] / p2, kz = s.kv[4 * k + 2] / p2;
+    vf[k] = 1.0 - (kx * kx + ky * ky + kz * kz) / (2.0 * s.alpha * s.alpha);
+  }
+  e->vfac.alloc(vf.size());
+  HIP_CHECK(hipMemcpy(e->vfac.p, vf.data(), vf.size() * sizeof(double), hipMemcpyHostToDevice));
+  e->selfc.alloc(s.nt);
+  HIP_CHECK(hipMemcpy(e->selfc.p, zero.data(), zero.size() * sizeof(double), hipMemcpyHostToDevice));
+  return e;
+}
+
+double dev_lowk_compute(Device* d, EwaldState* e, double* F, double* eatom, double L, double* W) {
+  if (!e || e->nvecs == 0 || e->nslots == 0) { *W = 0.0; return 0.0; }
+  return ewald_eval(d, e, F, e->lambda.p, e->selfc.p, eatom, L, W);
+}
+
+int dev_lowk_vectors(const EwaldState* e) { return e ? e->nvecs : 0; }
+
+void dev_lowk_release(EwaldState* e) {
+  if (!e) return;
+  e->kn.release(); e->kv.release(); e->item.release(); e->groupType.release(); e->typeGroups.release();
+  e->q.release(); e->lambda.release(); e->Fpart.release(); e->part.release(); e->result.release(); e->selfc.release();
+  e->vfac.release(); e->T.release(); e->Spart.release(); e->S.release(); e->sigma.release();
+  delete e;
 }
 
 }  // namespace emdee

@@ -171,4 +171,50 @@ EwaldHost ewald_initialize(double accuracy, double Rc, double L, int nlayers, co
   return h;
 }
 
+// The k-vectors of the same half space with |k| <= kcut (not the reference's
+// set: a partial sum over the longest wavelengths, used by the learned model
+// as an exact long-range channel), in the reference's enumeration order (n3
+// fastest, kspace_ewald.f90:115-138) and with its coefficients
+// (kspace_ewald.f90:174-180); charges, slots and the kCoul weights of layer
+// `layer` (1-based) are those of the full set-up.
+EwaldSetup lowk_setup(const EwaldSetup& full, int layer, double alpha, double L, double kcut) {
+  EwaldSetup e = full;
+  e.nl = 1;
+  e.alpha = alpha;
+  int nt = full.nt;
+  e.lambda.assign(full.lambda.begin() + (size_t)(layer - 1) * nt * nt, full.lambda.begin() + (size_t)layer * nt * nt);
+  double unit = 2.0 * PI_ / L, unitSq = unit * unit, kcutSq = kcut * kcut;
+  int nmax = std::max(1, (int)std::ceil(kcut / unit));
+  std::vector<int> n;
+  for (int n1 = 0; n1 <= nmax; n1++)
+    for (int n2 = -nmax; n2 <= nmax; n2++)
+      for (int n3 = -nmax; n3 <= nmax; n3++) {
+        bool upper = n1 > 0 || (n1 == 0 && (n2 > 0 || (n2 == 0 && n3 > 0)));
+        if (!upper) continue;
+        if (unitSq * n1 * n1 + unitSq * n2 * n2 + unitSq * n3 * n3 <= kcutSq) {
+          n.push_back(n1); n.push_back(n2); n.push_back(n3);
+        }
+      }
+  int nvecs = (int)n.size() / 3;
+  e.nm = nmax;
+  e.nvecs = nvecs;
+  e.kn.assign(4 * (size_t)nvecs, 0);
+  e.kv.assign(4 * (size_t)nvecs, 0.0);
+  double B = -0.25 / (alpha * alpha), fourPiByV = 4.0 * PI_ / (L * L * L);
+  for (int i = 0; i < nvecs; i++) {
+    double k[3];
+    for (int x = 0; x < 3; x++) k[x] = unit * n[3 * i + x];
+    double ksq = k[0] * k[0] + k[1] * k[1] + k[2] * k[2];
+    double prefac = fourPiByV * std::exp(ksq * B) / ksq;
+    for (int x = 0; x < 3; x++) e.kv[4 * i + x] = 2.0 * prefac * k[x];
+    e.kv[4 * i + 3] = prefac;
+    bool cont = i > 0 && n[3 * i] == n[3 * i - 3] && n[3 * i + 1] == n[3 * i - 2] && n[3 * i + 2] == n[3 * i - 1] + 1;
+    e.kn[4 * i] = n[3 * i];
+    e.kn[4 * i + 1] = n[3 * i + 1];
+    e.kn[4 * i + 2] = n[3 * i + 2];
+    e.kn[4 * i + 3] = cont ? 0 : 1;
+  }
+  return e;
+}
+
 }  // namespace emdee

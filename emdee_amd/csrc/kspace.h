@@ -53,6 +53,17 @@ double inverse_of_x_plus_ln_x(double y);
 void dev_ewald_setup(Device* d, const EwaldSetup& s);
 double dev_ewald_compute(Device* d, int layer, double L);
 
+// A partial k set: the vectors of the same half space with |k| <= kcut, the
+// charges and slots of `full` and its kCoul weights of `layer` (1-based).
+EwaldSetup lowk_setup(const EwaldSetup& full, int layer, double alpha, double L, double kcut);
+// Its device state (ewald.hip): forces added to F (slot order), atom shares
+// to eatom (may be null), energy returned and its virial in *W.
+struct EwaldState;
+EwaldState* dev_lowk_create(const EwaldSetup& s);
+double dev_lowk_compute(Device* d, EwaldState* e, double* F, double* eatom, double L, double* W);
+int dev_lowk_vectors(const EwaldState* e);
+void dev_lowk_release(EwaldState* e);
+
 // Bonded terms (bonded.hip): harmonic bonds and angles, EmDeeData.f90:443-550.
 struct DevStruct {
   int i, j, k, kind;          // 0-based original atoms; kind of the model

@@ -26,6 +26,7 @@
 
 #include "../../include/emdee_gpu.h"
 #include "device_state.h"
+#include "kspace.h"
 #include "surrogate.h"
 
 namespace emdee {
@@ -79,12 +80,31 @@ struct EmDeeSurrogate {
   DBuf<float> Xs, Ys;
   DBuf<int> Ss;
   long nrows = 0, caprows = 0;
-  // work buffers (rows padded to SG_ROWS)
+  // training work buffers (rows padded to SG_ROWS)
   long workRows = 0;
   DBuf<float> X, Z1, H1, Z2, H2, G2, G1, dX, e, dXsum, part;
   DBuf<double> Fw, red;          // results; per-block partials
-  DBuf<float4> lE;               // compact neighbour list (k_sg_list)
-  DBuf<int> lJ, lN, lOver;
+  // inference: compact list (rebuilt with the engine's list), normalised
+  // descriptors, atom energies and dE/dD (slot order), partial sums
+  DBuf<int> cJ, cN, cOver;
+  int cap = 0;
+  DBuf<int> rowSlot, grpSpecies;   // network rows grouped by species (k_sg_rows)
+  int ngroups = 0;
+  DBuf<float4> Rf;                 // single-precision positions and species
+  DBuf<float4> nearE, nearG;       // near list of the latest descriptors (SG_NEAR per atom)
+  DBuf<int> nearN, nearOver;
+  long nearRows = 0;
+  const void* listOwner = nullptr;   // Device whose list the compact list follows
+  long listVersion = -1;
+  long inferRows = 0;
+  DBuf<float> Xi, dEdD, eat;
+  DBuf<double> epart, wpart, eres;
+  // exact long-range channel: the reciprocal sum over |k| <= kcut (0: none)
+  double kcut = 0.0;
+  EwaldState* lowk = nullptr;
+  const void* lowkOwner = nullptr;
+  double lowkL = 0.0;
+  DBuf<double> lowkAtom;         // low-k atom energies of a sample (slot order)
   hipStream_t own = nullptr;     // training stream
   hipStream_t stream = nullptr;  // stream of the current operation
 };
@@ -105,77 +125,422 @@ struct ParamLayout {
 
 // ---- descriptors ------------------------------------------------------------
 
-__device__ inline void pair_sep(const double* __restrict__ R, int Npad, int i, int j, double L, double invL,
-                                float& dx, float& dy, float& dz) {
-  double ex = R[i] - R[j], ey = R[Npad + i] - R[Npad + j], ez = R[2 * Npad + i] - R[2 * Npad + j];
-  ex -= L * rint(ex * invL);
-  ey -= L * rint(ey * invL);
-  ez -= L * rint(ez * invL);
-  dx = (float)ex; dy = (float)ey; dz = (float)ez;
+
+// ---- inference: compact list, descriptors, fused MLP, forces ------------------
+//
+// The compact list of an atom holds its neighbours closer than rc + skin at
+// the positions of the engine's latest list build (R0), taken from the
+// engine's Verlet list in its order (sixteen lanes per atom, ordered by
+// ballot).  It is rebuilt only when the engine's list is: under the engine's
+// rebuild rule two atoms approach by at most the skin between builds, so
+// every pair closer than rc at any later step is in it (the same guarantee
+// the engine's own list relies on).
+constexpr int SG_TPA = 16;
+
+__global__ void __launch_bounds__(BLOCK) k_sg_list(int* __restrict__ cJ, int* __restrict__ cN, int* __restrict__ over,
+                                                   const double* __restrict__ R0, const int* __restrict__ nbr,
+                                                   const int* __restrict__ nnb, double rl2, double L, int N, int Npad,
+                                                   int cap) {
+  int gt = blockIdx.x * blockDim.x + threadIdx.x;
+  int i = gt / SG_TPA, sub = gt % SG_TPA, lane = threadIdx.x & 63;
+  bool valid = i < N;
+  int n = valid ? nnb[i] : 0;
+  const double invL = 1.0 / L;
+  double xi = valid ? R0[i] : 0.0, yi = valid ? R0[Npad + i] : 0.0, zi = valid ? R0[2 * Npad + i] : 0.0;
+  int base = 0;
+  // n is uniform over the atom's sixteen lanes, so each ballot below sees
+  // all of them
+  for (int k0 = 0; k0 < n; k0 += SG_TPA) {
+    int k = k0 + sub, j = 0;
+    bool keep = false;
+    if (k < n) {
+      j = nbr[(size_t)k * Npad + i];
+      double dx = xi - R0[j], dy = yi - R0[Npad + j], dz = zi - R0[2 * Npad + j];
+      dx -= L * rint(dx * invL);
+      dy -= L * rint(dy * invL);
+      dz -= L * rint(dz * invL);
+      keep = dx * dx + dy * dy + dz * dz < rl2;
+    }
+    unsigned long long m = __ballot(keep);
+    unsigned int gm = (unsigned int)(m >> (lane & ~(SG_TPA - 1))) & 0xffffu;
+    int c = base + __popc(gm & ((1u << sub) - 1u));
+    if (keep && c < cap) cJ[(size_t)i * cap + c] = j;
+    base += __popc(gm);
+  }
+  if (valid && sub == 0) {
+    cN[i] = min(base, cap);
+    if (base > cap) atomicMax(over, base);
+  }
 }
 
-// Compact neighbour list within rc, built from the engine's Verlet list at
-// every use: SG_SUBS consecutive lanes per atom scan interleaved parts of its
-// list and each appends its hits to its own segment; an entry keeps the
-// separation vector and distance (FP32) and the neighbour's slot with its
-// species in the low two bits.  Segments are interleaved: entry c of segment
-// s of atom i sits at (i SG_SEG + c) SG_SUBS + s, so the lanes of one atom
-// write and read neighbouring addresses.
-constexpr int SG_SUBS = 16, SG_SEG = 40;
-__device__ inline size_t sg_at(int i, int c, int s) { return ((size_t)i * SG_SEG + c) * SG_SUBS + s; }
-__global__ void __launch_bounds__(BLOCK) k_sg_list(float4* __restrict__ E, int* __restrict__ J, int* __restrict__ cnt,
-                                                   int* __restrict__ overflow, const double* __restrict__ R,
-                                                   const int* __restrict__ nbr, const int* __restrict__ nnb,
-                                                   const int* __restrict__ type, float rc, double L, int N,
-                                                   int Npad) {
-  int gt = blockIdx.x * blockDim.x + threadIdx.x;
-  int i = gt / SG_SUBS, sub = gt % SG_SUBS;
-  if (i >= N) return;
-  const float rc2 = rc * rc;
-  const double invL = 1.0 / L;
-  int n = nnb[i], c = 0;
-  for (int k = sub; k < n; k += SG_SUBS) {
-    int j = nbr[(size_t)k * Npad + i];
-    float dx, dy, dz;
-    pair_sep(R, Npad, i, j, L, invL, dx, dy, dz);
-    float r2 = dx * dx + dy * dy + dz * dz;
-    if (r2 < rc2) {
-      if (c < SG_SEG) { E[sg_at(i, c, sub)] = make_float4(dx, dy, dz, sqrtf(r2)); J[sg_at(i, c, sub)] = (j << 2) | type[j]; }
-      c++;
+// Rows of the network grouped by species (deterministic: ascending slot
+// within a species), each species padded to whole 64-row groups; rowSlot = -1
+// marks padding and grpSpecies[g] the species of group g (-1 past the end).
+// One block of 1024 threads: a scan of the species flags per species.
+__global__ void __launch_bounds__(1024) k_sg_rows(int* __restrict__ rowSlot, int* __restrict__ grpSpecies,
+                                                  const int* __restrict__ type, int nt, int N, int ngroups) {
+  __shared__ int part[1024];
+  __shared__ int base;
+  const int t = threadIdx.x, chunk = (N + 1023) / 1024;
+  const int a0 = min(N, t * chunk), a1 = min(N, a0 + chunk);
+  for (int g = t; g < ngroups; g += 1024) grpSpecies[g] = -1;
+  if (t == 0) base = 0;
+  __syncthreads();
+  for (int s = 0; s < nt; s++) {
+    int c = 0;
+    for (int a = a0; a < a1; a++) c += type[a] == s;
+    part[t] = c;
+    __syncthreads();
+    for (int off = 1; off < 1024; off <<= 1) {   // inclusive scan (Hillis-Steele)
+      int v = t >= off ? part[t - off] : 0;
+      __syncthreads();
+      part[t] += v;
+      __syncthreads();
+    }
+    int pos = base + part[t] - c;
+    for (int a = a0; a < a1; a++)
+      if (type[a] == s) rowSlot[pos++] = a;
+    const int cnt = part[1023], padded = (cnt + 63) & ~63;
+    for (int r = base + cnt + t; r < base + padded; r += 1024) rowSlot[r] = -1;
+    for (int g = base / 64 + t; g < (base + padded) / 64; g += 1024) grpSpecies[g] = s;
+    __syncthreads();
+    if (t == 0) base += padded;
+    __syncthreads();
+  }
+}
+
+// Single-precision positions and species for the gathers of the descriptor
+// kernel: one 16-byte load per neighbour.
+__global__ void k_sg_pos(float4* __restrict__ Rf, const double* __restrict__ R, const int* __restrict__ type, int N,
+                         int Npad) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < N) Rf[i] = make_float4((float)R[i], (float)R[Npad + i], (float)R[2 * Npad + i], __int_as_float(type[i]));
+}
+
+__device__ __forceinline__ float sg_wrap(float d, float L, float invL) { return d - L * rintf(d * invL); }
+
+// Near list: the entries of an atom's compact list closer than rc at the
+// current positions, at most SG_NEAR per atom: {r_ij / r, r} and
+// {fc, dfc, j, t_j} (cutoff function and its derivative, neighbour slot and
+// species), written by the descriptor kernel for the force kernel.
+constexpr int SG_NEAR = 256;
+constexpr int SG_G = 32;   // lanes per Gaussian set: lane g of a stream owns Gaussian g
+
+// Descriptors, one wavefront per atom.  Phase A: the 64 lanes test the
+// atom's compact-list entries (four index loads, then four position gathers,
+// in flight together) and compact those within rc, in list order (ballot
+// prefix), into LDS and into the near list.  Phase B: two streams of 32
+// lanes take the even and the odd near entries; lane g of a stream owns
+// Gaussian g and sums it over its entries (broadcast LDS reads), one
+// accumulator per neighbour species, so every evaluated Gaussian is of an
+// in-range pair; the two streams' sums are added in a fixed order.
+// Features are written normalised (mean != null) or raw.
+template <int NT>
+__global__ void __launch_bounds__(BLOCK) k_sg_desc(float* __restrict__ X, float4* __restrict__ nearE,
+                                                   float4* __restrict__ nearG, int* __restrict__ nearN,
+                                                   int* __restrict__ over, const int* __restrict__ cJ,
+                                                   const int* __restrict__ cN, const float4* __restrict__ Rf,
+                                                   const float* __restrict__ mean, const float* __restrict__ sd,
+                                                   SurrogateParams sp, float L, int N, int cap) {
+  constexpr int DIN = NT * SG_NB, AW = BLOCK / 64;
+  __shared__ float2 ent[AW][SG_NEAR];
+  __shared__ int entT[AW][SG_NEAR];
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const int i = blockIdx.x * AW + wv;
+  const bool valid = i < N;
+  const float invL = 1.0f / L, rc2 = sp.rc * sp.rc, kpi = PI_F / sp.rc, k2 = -sp.eta * 1.4426950408889634f;
+  const int n = valid ? cN[i] : 0;
+  const float4 ri = valid ? Rf[i] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+  int cnt = 0;
+  for (int c0 = 0; c0 < n; c0 += 64 * 4) {
+    int j[4];
+    float4 rj[4];
+#pragma unroll
+    for (int u = 0; u < 4; u++) {
+      int c = c0 + lane + 64 * u;
+      j[u] = c < n ? cJ[(size_t)i * cap + c] : -1;
+    }
+#pragma unroll
+    for (int u = 0; u < 4; u++) rj[u] = j[u] >= 0 ? Rf[j[u]] : ri;
+#pragma unroll
+    for (int u = 0; u < 4; u++) {
+      float dx = sg_wrap(ri.x - rj[u].x, L, invL), dy = sg_wrap(ri.y - rj[u].y, L, invL),
+            dz = sg_wrap(ri.z - rj[u].z, L, invL);
+      float r2 = dx * dx + dy * dy + dz * dz;
+      bool in = j[u] >= 0 && r2 < rc2;
+      unsigned long long m = __ballot(in);
+      int pos = cnt + __popcll(m & ((1ULL << lane) - 1ULL));
+      if (in && pos < SG_NEAR) {
+        float r = sqrtf(r2), fc = 0.5f * (__cosf(kpi * r) + 1.0f), dfc = -0.5f * kpi * __sinf(kpi * r);
+        float ir = 1.0f / r;
+        int tj = __float_as_int(rj[u].w);
+        ent[wv][pos] = make_float2(r, fc);
+        entT[wv][pos] = tj;
+        nearE[(size_t)i * SG_NEAR + pos] = make_float4(dx * ir, dy * ir, dz * ir, r);
+        nearG[(size_t)i * SG_NEAR + pos] = make_float4(fc, dfc, __int_as_float(j[u]), __int_as_float(tj));
+      }
+      cnt += __popcll(m);
     }
   }
-  cnt[i * SG_SUBS + sub] = min(c, SG_SEG);
-  if (c > SG_SEG) atomicMax(overflow, c);
-}
-
-// Descriptors from the compact list: 32 lanes per atom, lane b owns Gaussian b
-// of every neighbour species; entries are read once per atom (broadcast).
-template <int NT>
-__global__ void __launch_bounds__(BLOCK) k_sg_desc2(float* __restrict__ D, const float4* __restrict__ E,
-                                                    const int* __restrict__ J, const int* __restrict__ cnt,
-                                                    const int* __restrict__ type, const float* __restrict__ mean,
-                                                    const float* __restrict__ sd, SurrogateParams sp, int N) {
-  int gt = blockIdx.x * blockDim.x + threadIdx.x;
-  int i = gt / SG_NB, b = gt % SG_NB;
-  if (i >= N) return;
+  if (valid && lane == 0) {
+    nearN[i] = min(cnt, SG_NEAR);
+    if (cnt > SG_NEAR) atomicMax(over, cnt);
+  }
+  cnt = min(cnt, SG_NEAR);
+  __syncthreads();
+  const int g = lane & (SG_G - 1), st = lane >> 5;
   float acc[NT];
 #pragma unroll
   for (int t = 0; t < NT; t++) acc[t] = 0.0f;
-  const float mu = sp.mu0 + b * sp.dmu, kpi = PI_F / sp.rc;
-  for (int s = 0; s < SG_SUBS; s++) {
-    int n = cnt[i * SG_SUBS + s];
-    for (int c = 0; c < n; c++) {
-      float r = E[sg_at(i, c, s)].w;
-      int t = J[sg_at(i, c, s)] & 3;
-      float u = r - mu, g = __expf(-sp.eta * u * u) * 0.5f * (__cosf(kpi * r) + 1.0f);
+  const float mu = sp.mu0 + g * sp.dmu;
+  for (int e0 = st; e0 < cnt; e0 += 8) {
+    float2 v[4];
+    int tj[4];
 #pragma unroll
-      for (int q = 0; q < NT; q++) acc[q] += (q == t) ? g : 0.0f;
+    for (int u = 0; u < 4; u++) {
+      int e = e0 + 2 * u;
+      v[u] = ent[wv][min(e, cnt - 1)];
+      tj[u] = e < cnt ? entT[wv][e] : -1;
+    }
+#pragma unroll
+    for (int u = 0; u < 4; u++) {
+      float du = v[u].x - mu, gv = __builtin_amdgcn_exp2f(k2 * du * du) * v[u].y;
+#pragma unroll
+      for (int t = 0; t < NT; t++) acc[t] += (t == tj[u]) ? gv : 0.0f;
     }
   }
 #pragma unroll
+  for (int t = 0; t < NT; t++) acc[t] += __shfl_xor(acc[t], SG_G, 64);
+  if (!valid || st != 0) return;
+#pragma unroll
   for (int t = 0; t < NT; t++) {
-    int f = t * SG_NB + b;
-    D[(size_t)i * (NT * SG_NB) + f] = mean ? (acc[t] - mean[f]) / sd[f] : acc[t];   // normalised when mean given
+    int f = t * SG_NB + g;
+    X[(size_t)i * DIN + f] = mean ? (acc[t] - mean[f]) / sd[f] : acc[t];
+  }
+}
+
+struct SgNorm { float ymean[SG_MAXT], ystd[SG_MAXT]; };
+
+// The network of four 16-atom tiles of one species per workgroup (one tile
+// per wavefront), forward and backward.  The species' bf16 weights are staged
+// once in LDS (rows padded by four values, so the sixteen rows a fragment
+// load touches fall on distinct banks).  Activations are kept transposed
+// (features x atoms): the output fragment of v_mfma_f32_16x16x16_bf16 (lane
+// holds features 4 (lane/16) + r of atom lane % 16) is exactly the B operand
+// of the next product, so no activation leaves the registers.  Outputs: atom
+// energies, dE/dD of the raw descriptors (slot order) and the group's energy
+// sum (waves added in a fixed order).
+template <int DIN>
+struct SgLds {
+  static constexpr int S1 = DIN + 4, SH = SG_H + 4;
+  static constexpr int W1T = 0, W2T = W1T + SG_H * S1, W2 = W2T + SG_H * SH, W1 = W2 + SG_H * SH, SIZE = W1 + DIN * SH;
+};
+template <int NT>
+__global__ void __launch_bounds__(256) k_sg_mlp(float* __restrict__ eat, float* __restrict__ dEdD,
+                                                double* __restrict__ epart, const float* __restrict__ X,
+                                                const int* __restrict__ rowSlot, const int* __restrict__ grpSpecies,
+                                                const unsigned short* __restrict__ w1t,
+                                                const unsigned short* __restrict__ w2t,
+                                                const unsigned short* __restrict__ w2,
+                                                const unsigned short* __restrict__ w1,
+                                                const float* __restrict__ params, int P,
+                                                const float* __restrict__ xstd, SgNorm nrm) {
+  constexpr int DIN = NT * SG_NB, KT = DIN / 16, HT = SG_H / 16;
+  using Lay = SgLds<DIN>;
+  __shared__ unsigned short lw[Lay::SIZE];
+  __shared__ double wsum[4];
+  const int s = grpSpecies[blockIdx.x];
+  if (s < 0) {   // uniform over the block
+    if (threadIdx.x == 0) epart[blockIdx.x] = 0.0;
+    return;
+  }
+  // stage the species' weights: 8-byte chunks into padded rows
+  {
+    auto stage = [&](const unsigned short* src, int rows, int cols, int dst, int stride) {
+      const int per = cols / 8;
+      for (int c = threadIdx.x; c < rows * per; c += 256) {
+        int r = c / per, k = 8 * (c % per);
+        uint4 v = *reinterpret_cast<const uint4*>(src + (size_t)r * cols + k);
+        *reinterpret_cast<uint2*>(lw + dst + r * stride + k) = make_uint2(v.x, v.y);
+        *reinterpret_cast<uint2*>(lw + dst + r * stride + k + 4) = make_uint2(v.z, v.w);
+      }
+    };
+    stage(w1t + (size_t)s * DIN * SG_H, SG_H, DIN, Lay::W1T, Lay::S1);
+    stage(w2t + (size_t)s * SG_H * SG_H, SG_H, SG_H, Lay::W2T, Lay::SH);
+    stage(w2 + (size_t)s * SG_H * SG_H, SG_H, SG_H, Lay::W2, Lay::SH);
+    stage(w1 + (size_t)s * DIN * SG_H, DIN, SG_H, Lay::W1, Lay::SH);
+  }
+  __syncthreads();
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6, col = lane & 15, g = lane >> 4;
+  const int slot = rowSlot[blockIdx.x * 64 + wv * 16 + col];
+  const ParamLayout pl{ DIN, SG_H };
+  const float* p = params + (size_t)s * P;
+  bf16x4 xb[KT];
+#pragma unroll
+  for (int q = 0; q < KT; q++) {
+    float4 v = slot >= 0 ? *reinterpret_cast<const float4*>(X + (size_t)slot * DIN + 16 * q + 4 * g)
+                         : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    xb[q] = bf16x4{ (short)f2bf(v.x), (short)f2bf(v.y), (short)f2bf(v.z), (short)f2bf(v.w) };
+  }
+  f32x4 z1[HT], z2[HT];
+  bf16x4 hb[HT], gb[HT];
+#pragma unroll
+  for (int m = 0; m < HT; m++) {
+    f32x4 acc = f32x4{ 0.0f, 0.0f, 0.0f, 0.0f };
+#pragma unroll
+    for (int q = 0; q < KT; q++) {
+      bf16x4 a = *reinterpret_cast<const bf16x4*>(lw + Lay::W1T + (16 * m + col) * Lay::S1 + 16 * q + 4 * g);
+      acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, xb[q], acc, 0, 0, 0);
+    }
+#pragma unroll
+    for (int r = 0; r < 4; r++) {
+      float z = acc[r] + p[pl.b1() + 16 * m + 4 * g + r];
+      z1[m][r] = z;
+      hb[m][r] = (short)f2bf(silu(z));
+    }
+  }
+#pragma unroll
+  for (int m = 0; m < HT; m++) {
+    f32x4 acc = f32x4{ 0.0f, 0.0f, 0.0f, 0.0f };
+#pragma unroll
+    for (int q = 0; q < HT; q++) {
+      bf16x4 a = *reinterpret_cast<const bf16x4*>(lw + Lay::W2T + (16 * m + col) * Lay::SH + 16 * q + 4 * g);
+      acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, hb[q], acc, 0, 0, 0);
+    }
+#pragma unroll
+    for (int r = 0; r < 4; r++) z2[m][r] = acc[r] + p[pl.b2() + 16 * m + 4 * g + r];
+  }
+  float part = 0.0f;
+#pragma unroll
+  for (int m = 0; m < HT; m++)
+#pragma unroll
+    for (int r = 0; r < 4; r++) part += p[pl.w3() + 16 * m + 4 * g + r] * silu(z2[m][r]);
+  part += __shfl_xor(part, 16, 64);
+  part += __shfl_xor(part, 32, 64);
+  const float e = nrm.ystd[s] * (part + p[pl.b3()]) + nrm.ymean[s];
+  const float seed = slot >= 0 ? nrm.ystd[s] : 0.0f;
+#pragma unroll
+  for (int m = 0; m < HT; m++)
+#pragma unroll
+    for (int r = 0; r < 4; r++) gb[m][r] = (short)f2bf(seed * p[pl.w3() + 16 * m + 4 * g + r] * dsilu(z2[m][r]));
+  // G1 = (W2 G2) * SiLU'(Z1), into hb
+#pragma unroll
+  for (int m = 0; m < HT; m++) {
+    f32x4 acc = f32x4{ 0.0f, 0.0f, 0.0f, 0.0f };
+#pragma unroll
+    for (int q = 0; q < HT; q++) {
+      bf16x4 a = *reinterpret_cast<const bf16x4*>(lw + Lay::W2 + (16 * m + col) * Lay::SH + 16 * q + 4 * g);
+      acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, gb[q], acc, 0, 0, 0);
+    }
+#pragma unroll
+    for (int r = 0; r < 4; r++) hb[m][r] = (short)f2bf(acc[r] * dsilu(z1[m][r]));
+  }
+  // dE/dX = W1 G1
+  f32x4 dx[KT];
+#pragma unroll
+  for (int m = 0; m < KT; m++) {
+    dx[m] = f32x4{ 0.0f, 0.0f, 0.0f, 0.0f };
+#pragma unroll
+    for (int q = 0; q < HT; q++) {
+      bf16x4 a = *reinterpret_cast<const bf16x4*>(lw + Lay::W1 + (16 * m + col) * Lay::SH + 16 * q + 4 * g);
+      dx[m] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, hb[q], dx[m], 0, 0, 0);
+    }
+  }
+  if (slot >= 0) {
+#pragma unroll
+    for (int m = 0; m < KT; m++) {
+      int f = 16 * m + 4 * g;
+      float4 v = make_float4(dx[m][0] / xstd[f], dx[m][1] / xstd[f + 1], dx[m][2] / xstd[f + 2], dx[m][3] / xstd[f + 3]);
+      *reinterpret_cast<float4*>(dEdD + (size_t)slot * DIN + f) = v;
+    }
+    if (g == 0) eat[slot] = e;
+  }
+  double v = (g == 0 && slot >= 0) ? (double)e : 0.0;
+  v = wave_sum(v);
+  if (lane == 0) wsum[wv] = v;
+  __syncthreads();
+  if (threadIdx.x == 0) epart[blockIdx.x] = (wsum[0] + wsum[1]) + (wsum[2] + wsum[3]);
+}
+
+// Forces and virial from dE/dD over the near list, one wavefront per atom:
+// two streams of 32 lanes take the even and the odd near entries; lane g of
+// a stream owns Gaussian g and sums its share of
+// dE/dr_ij = sum_g (dE/dD_i[t_j][g] + dE/dD_j[t_i][g]) dG_g/dr (the pair
+// moves both atoms' descriptors) times -r_ij / r.  The records of the next
+// four entries are loaded while the current four are evaluated.  The 64
+// lanes' sums are combined once per atom by a fixed butterfly.  The list is
+// full, so each atom sums its own force (no atomics).  Virial: -1/2 dE/dr r
+// per entry.  The near list carries the unit vector r_ij / r, so no division
+// per pair.
+template <int NT>
+__global__ void __launch_bounds__(BLOCK) k_sg_force(double* __restrict__ F, double* __restrict__ wpart,
+                                                    const float* __restrict__ dEdD, const float4* __restrict__ nearE,
+                                                    const float4* __restrict__ nearG, const int* __restrict__ nearN,
+                                                    const int* __restrict__ type, SurrogateParams sp, int N,
+                                                    int Npad) {
+  constexpr int DIN = NT * SG_NB, AW = BLOCK / 64;
+  __shared__ double wred[AW];
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6, g = lane & (SG_G - 1), st = lane >> 5;
+  const int i = blockIdx.x * AW + wv;
+  float fx = 0.0f, fy = 0.0f, fz = 0.0f, w = 0.0f;
+  if (i < N) {
+    const float k2 = -sp.eta * 1.4426950408889634f, mu = sp.mu0 + g * sp.dmu, eta2 = 2.0f * sp.eta;
+    const int ti = type[i], cnt = nearN[i];
+    float ai[NT];
+#pragma unroll
+    for (int t = 0; t < NT; t++) ai[t] = dEdD[(size_t)i * DIN + t * SG_NB + g];
+    const float4* E = nearE + (size_t)i * SG_NEAR;
+    const float4* G = nearG + (size_t)i * SG_NEAR;
+    float4 ev[4], gv[4];
+    auto fetch = [&](int e0) {
+#pragma unroll
+      for (int u = 0; u < 4; u++) {
+        int e = min(e0 + 2 * u, cnt - 1);
+        ev[u] = E[e];
+        gv[u] = G[e];
+      }
+    };
+    if (cnt > 0) fetch(st);
+    for (int e0 = st; e0 < cnt; e0 += 8) {
+      float aj[4];
+      float4 ec[4] = { ev[0], ev[1], ev[2], ev[3] }, gc[4] = { gv[0], gv[1], gv[2], gv[3] };
+#pragma unroll
+      for (int u = 0; u < 4; u++) aj[u] = dEdD[(size_t)__float_as_int(gc[u].z) * DIN + ti * SG_NB + g];
+      if (e0 + 8 < cnt) fetch(e0 + 8);
+#pragma unroll
+      for (int u = 0; u < 4; u++) {
+        if (e0 + 2 * u >= cnt) break;
+        int tj = __float_as_int(gc[u].w);
+        float sel = ai[0];
+#pragma unroll
+        for (int t = 1; t < NT; t++) sel = (tj == t) ? ai[t] : sel;
+        float du = ec[u].w - mu, gauss = __builtin_amdgcn_exp2f(k2 * du * du);
+        float dEdr = (sel + aj[u]) * (gauss * (gc[u].y - eta2 * du * gc[u].x));
+        fx -= dEdr * ec[u].x;
+        fy -= dEdr * ec[u].y;
+        fz -= dEdr * ec[u].z;
+        w -= dEdr * ec[u].w;
+      }
+    }
+  }
+  double dx = fx, dy = fy, dz = fz, dw = 0.5 * (double)w;
+  dx = wave_sum(dx);
+  dy = wave_sum(dy);
+  dz = wave_sum(dz);
+  dw = wave_sum(dw);
+  if (lane == 0) {
+    if (i < N) {
+      F[i] += dx;
+      F[Npad + i] += dy;
+      F[2 * Npad + i] += dz;
+    }
+    wred[wv] = dw;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double s2 = 0.0;
+    for (int k = 0; k < AW; k++) s2 += wred[k];
+    wpart[blockIdx.x] = s2;
   }
 }
 
@@ -378,62 +743,6 @@ __global__ void k_sg_pack(unsigned short* __restrict__ w1t, unsigned short* __re
   }
 }
 
-// Forces from dE/dD over the compact list: 16 lanes per atom share its
-// entries; each entry sums its 32 Gaussian derivatives; the lanes' forces are
-// combined by a fixed butterfly.
-template <int NT>
-__global__ void __launch_bounds__(BLOCK) k_sg_force2(double* __restrict__ F, double* __restrict__ red,
-                                                     const float* __restrict__ dXn, const float* __restrict__ xstd,
-                                                     const float4* __restrict__ E, const int* __restrict__ J,
-                                                     const int* __restrict__ cnt, const int* __restrict__ type,
-                                                     SurrogateParams sp, int N, int Npad) {
-  constexpr int TPA = 16, din = NT * SG_NB;
-  int gt = blockIdx.x * blockDim.x + threadIdx.x;
-  int i = gt / TPA, sub = gt % TPA;
-  double fx = 0, fy = 0, fz = 0, w[1] = { 0.0 };
-  static_assert(TPA == SG_SUBS, "one lane per list segment");
-  if (i < N) {
-    const float kpi = PI_F / sp.rc;
-    int ti = type[i];
-    {
-      int n = cnt[i * SG_SUBS + sub];
-      for (int c = 0; c < n; c++) {
-        float4 e = E[sg_at(i, c, sub)];
-        int jt = J[sg_at(i, c, sub)], j = jt >> 2, tj = jt & 3;
-        float r = e.w, fc = 0.5f * (__cosf(kpi * r) + 1.0f), dfc = -0.5f * kpi * __sinf(kpi * r);
-        const float* gi = dXn + (size_t)i * din + tj * SG_NB;
-        const float* gj = dXn + (size_t)j * din + ti * SG_NB;
-        const float* si = xstd + tj * SG_NB;
-        const float* sj = xstd + ti * SG_NB;
-        float dEdr = 0.0f;
-#pragma unroll 8
-        for (int b = 0; b < SG_NB; b++) {
-          float u = r - (sp.mu0 + b * sp.dmu), g = __expf(-sp.eta * u * u);
-          dEdr += (gi[b] / si[b] + gj[b] / sj[b]) * (g * (dfc - 2.0f * sp.eta * u * fc));
-        }
-        float k = -dEdr / r;
-        fx += (double)(k * e.x);
-        fy += (double)(k * e.y);
-        fz += (double)(k * e.z);
-        w[0] += -0.5 * (double)(dEdr * r);
-      }
-    }
-  }
-#pragma unroll
-  for (int off = TPA / 2; off > 0; off >>= 1) {
-    fx += __shfl_xor(fx, off, 64);
-    fy += __shfl_xor(fy, off, 64);
-    fz += __shfl_xor(fz, off, 64);
-  }
-  if (i < N && sub == 0) {
-    F[i] += fx;
-    F[Npad + i] += fy;
-    F[2 * Npad + i] += fz;
-  }
-  block_sum<1>(w);
-  if (threadIdx.x == 0) red[blockIdx.x] = w[0];
-}
-
 // ---- host side -------------------------------------------------------------------
 
 static void ensure_work(EmDeeSurrogate* s, long rows) {
@@ -451,31 +760,86 @@ static void ensure_work(EmDeeSurrogate* s, long rows) {
   s->workRows = pad;
 }
 
-static void build_list(EmDeeSurrogate* s, Device* d, double L, hipStream_t st) {
-  int N = d->N;
-  s->lE.alloc((size_t)N * SG_SUBS * SG_SEG);
-  s->lJ.alloc((size_t)N * SG_SUBS * SG_SEG);
-  s->lN.alloc((size_t)N * SG_SUBS);
-  s->lOver.alloc(1);
-  HIP_CHECK(hipMemsetAsync(s->lOver.p, 0, sizeof(int), st));
-  hipLaunchKernelGGL(k_sg_list, dim3(div_up((long)d->Npad * SG_SUBS, BLOCK)), dim3(BLOCK), 0, st, s->lE.p, s->lJ.p, s->lN.p,
-                     s->lOver.p, d->R.p, d->nbr.p, d->nnb.p, d->type.p, s->sp.rc, L, N, d->Npad);
-  launch_check("surrogate list");
-  int over = 0;
-  HIP_CHECK(hipMemcpyAsync(&over, s->lOver.p, sizeof(int), hipMemcpyDeviceToHost, st));
-  HIP_CHECK(hipStreamSynchronize(st));
-  if (over) fatal("surrogate", "more neighbours within the descriptor cutoff than the compact list holds");
+static void check_context(const EmDeeSurrogate* s, const Device* d, const SurrogateContext& c, const char* task) {
+  if (s->sp.nt != d->ntypes) fatal(task, "the model and the system have different numbers of types");
+  if ((double)s->sp.rc > c.Rc) fatal(task, "the descriptor cutoff exceeds the system's cutoff");
+  if (!d->built) fatal(task, "the system has no neighbor list yet");
 }
 
-static void launch_desc(EmDeeSurrogate* s, Device* d, float* D, double L, hipStream_t st, bool normalised) {
-  build_list(s, d, L, st);
-  int g = div_up((long)d->N * SG_NB, BLOCK);
+double surrogate_cutoff(const EmDeeSurrogate* s) { return (double)s->sp.rc; }
+
+// The compact list of the system's current Verlet list: rebuilt when the
+// engine has rebuilt its list since (or the model moved to another system).
+static void ensure_list(EmDeeSurrogate* s, Device* d, const SurrogateContext& c) {
+  if (s->listOwner == d && s->listVersion == d->listVersion) return;
+  int N = d->N, Np = d->Npad;
+  double rl = (double)s->sp.rc + c.skin, rl2 = rl * rl;
+  if (s->cap == 0) {
+    double vol = 4.18879020478639 * rl * rl * rl * ((double)N / (c.L * c.L * c.L));
+    s->cap = ((int)(1.3 * vol + 32) + 7) & ~7;
+  }
+  s->cN.alloc(Np);
+  s->cOver.alloc(1);
+  for (;;) {
+    s->cJ.alloc((size_t)s->cap * Np);
+    HIP_CHECK(hipMemsetAsync(s->cOver.p, 0, sizeof(int), d->stream));
+    hipLaunchKernelGGL(k_sg_list, dim3(div_up((long)Np * SG_TPA, BLOCK)), dim3(BLOCK), 0, d->stream, s->cJ.p, s->cN.p,
+                       s->cOver.p, d->R0.p, d->nbr.p, d->nnb.p, rl2, c.L, N, Np, s->cap);
+    launch_check("surrogate list");
+    int over = 0;
+    HIP_CHECK(hipMemcpyAsync(&over, s->cOver.p, sizeof(int), hipMemcpyDeviceToHost, d->stream));
+    HIP_CHECK(hipStreamSynchronize(d->stream));
+    if (over == 0) break;
+    s->cap = ((int)(over * 1.2) + 15) & ~7;
+  }
+  // network rows by species, in the new slot order
+  s->ngroups = d->Npad / 64 + s->sp.nt;
+  s->rowSlot.alloc((size_t)64 * s->ngroups);
+  s->grpSpecies.alloc(s->ngroups);
+  hipLaunchKernelGGL(k_sg_rows, dim3(1), dim3(1024), 0, d->stream, s->rowSlot.p, s->grpSpecies.p, d->type.p, s->sp.nt,
+                     N, s->ngroups);
+  launch_check("surrogate rows");
+  s->listOwner = d;
+  s->listVersion = d->listVersion;
+}
+
+// Descriptors of the system's current positions into D ([N][din], rows in
+// slot order), normalised or raw.
+static void launch_desc(EmDeeSurrogate* s, Device* d, const SurrogateContext& c, float* D, bool normalised) {
+  ensure_list(s, d, c);
+  int Np = d->Npad;
+  s->Rf.alloc(Np);
+  if (s->nearRows < Np) {
+    s->nearE.alloc((size_t)Np * SG_NEAR);
+    s->nearG.alloc((size_t)Np * SG_NEAR);
+    s->nearN.alloc(Np);
+    s->nearOver.alloc(1);
+    s->nearRows = Np;
+  }
+  HIP_CHECK(hipMemsetAsync(s->nearOver.p, 0, sizeof(int), d->stream));
+  hipLaunchKernelGGL(k_sg_pos, dim3(div_up(d->N, BLOCK)), dim3(BLOCK), 0, d->stream, s->Rf.p, d->R.p, d->type.p, d->N,
+                     Np);
+  int g = div_up(d->N, BLOCK / 64);
   const float* m = normalised ? s->xmean.p : nullptr;
   const float* sd = normalised ? s->xstd.p : nullptr;
-#define SG_DESC(NT) hipLaunchKernelGGL(k_sg_desc2<NT>, dim3(g), dim3(BLOCK), 0, st, D, s->lE.p, s->lJ.p, s->lN.p, d->type.p, m, sd, s->sp, d->N)
+#define SG_DESC(NT)                                                                                                 \
+  hipLaunchKernelGGL(k_sg_desc<NT>, dim3(g), dim3(BLOCK), 0, d->stream, D, s->nearE.p, s->nearG.p, s->nearN.p,      \
+                     s->nearOver.p, s->cJ.p, s->cN.p, s->Rf.p, m, sd, s->sp, (float)c.L, d->N, s->cap)
   switch (s->sp.nt) { case 1: SG_DESC(1); break; case 2: SG_DESC(2); break; case 3: SG_DESC(3); break; default: SG_DESC(4); }
 #undef SG_DESC
   launch_check("surrogate descriptors");
+}
+
+// The low-k state for this system and box (rebuilt when either changes).
+static EwaldState* ensure_lowk(EmDeeSurrogate* s, Device* d, const SurrogateContext& c, const char* task) {
+  if (s->kcut <= 0.0) return nullptr;
+  if (s->lowk && s->lowkOwner == d && s->lowkL == c.L) return s->lowk;
+  if (!c.ewald) fatal(task, "the long-range channel needs the system's kspace model");
+  dev_lowk_release(s->lowk);
+  s->lowk = dev_lowk_create(lowk_setup(*c.ewald, c.lambdaLayer, c.alpha, c.L, s->kcut));
+  s->lowkOwner = d;
+  s->lowkL = c.L;
+  return s->lowk;
 }
 
 static void gemm(EmDeeSurrogate* s, int K, int NC, int epi, const float* A, const unsigned short* Bt, const float* bias,
@@ -571,54 +935,98 @@ static void repack(EmDeeSurrogate* s) {
   launch_check("surrogate pack");
 }
 
-// Inference on a system's device state: adds the model forces to F (slot
-// order) and returns the model energy and pair virial; eatom (slot order, may
-// be null) receives the atom energies.
 __global__ void k_sg_add_atom(double* __restrict__ dst, const float* __restrict__ src, int n) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) dst[i] += (double)src[i];
 }
 
-double dev_surrogate_apply(Device* d, EmDeeSurrogate* s, int layer, double L, double* W) {
-  if (!s->normalised) fatal("surrogate", "the model has not been trained");
-  if (s->sp.nt != d->ntypes) fatal("surrogate", "the model and the system have different numbers of types");
-  int N = d->N;
-  double* F = d->F.p + 3 * (size_t)d->Npad * layer;
-  if (N == 0) { *W = 0; return 0; }
+// Inference on a system's device state: adds the model forces to F(layer)
+// (slot order) and its atom energies to the atom-energy array when that is
+// on; returns the model energy (low-k sum plus network) and its virial in *W.
+double dev_surrogate_apply(Device* d, EmDeeSurrogate* s, int layer, const SurrogateContext& c, double* W) {
+  const char* task = "surrogate";
+  if (!s->normalised) fatal(task, "the model has not been trained");
+  int N = d->N, Np = d->Npad, din = s->sp.din;
+  *W = 0.0;
+  if (N == 0) return 0.0;
+  check_context(s, d, c, task);
   s->stream = d->stream;
-  ensure_work(s, N);
-  long pad = s->workRows;
-  int din = s->sp.din;
-  // normalised descriptors straight into X (rows past N stay zero)
-  launch_desc(s, d, s->X.p, L, s->stream, true);
-  double E = 0;
-  float* eatom = s->e.p;
-  for (int t = 0; t < s->sp.nt; t++) E += mlp_pass(s, t, N, d->type.p, nullptr, 0.0f, 0, eatom);
-  int g = div_up((long)N * 16, BLOCK);
-#define SG_FORCE(NT)                                                                                                \
-  hipLaunchKernelGGL(k_sg_force2<NT>, dim3(g), dim3(BLOCK), 0, s->stream, F, s->red.p, s->dXsum.p, s->xstd.p, s->lE.p, \
-                     s->lJ.p, s->lN.p, d->type.p, s->sp, N, d->Npad)
-  switch (s->sp.nt) { case 1: SG_FORCE(1); break; case 2: SG_FORCE(2); break; case 3: SG_FORCE(3); break; default: SG_FORCE(4); }
-#undef SG_FORCE
-  launch_check("surrogate forces");
-  hipLaunchKernelGGL(k_reduce_partials<1>, dim3(1), dim3(BLOCK), 0, s->stream, s->Fw.p + 1, s->red.p, g);
-  HIP_CHECK(hipMemcpyAsync(W, s->Fw.p + 1, sizeof(double), hipMemcpyDeviceToHost, s->stream));
+  double* F = d->F.p + 3 * (size_t)Np * layer;
+  if (s->inferRows < Np) {
+    s->Xi.alloc((size_t)Np * din);
+    s->dEdD.alloc((size_t)Np * din);
+    s->eat.alloc(Np);
+    s->wpart.alloc(div_up(Np, BLOCK / 64));
+    s->eres.alloc(2);
+    s->inferRows = Np;
+  }
+  launch_desc(s, d, c, s->Xi.p, true);
+  s->epart.alloc(s->ngroups);
+  SgNorm nrm;
+  for (int t = 0; t < SG_MAXT; t++) {
+    nrm.ymean[t] = t < s->sp.nt ? s->ymean[t] : 0.0f;
+    nrm.ystd[t] = t < s->sp.nt ? s->ystd[t] : 0.0f;
+  }
+  int gf = div_up(N, BLOCK / 64);
+#define SG_RUN(NT)                                                                                                     \
+  do {                                                                                                                 \
+    hipLaunchKernelGGL(k_sg_mlp<NT>, dim3(s->ngroups), dim3(256), 0, d->stream, s->eat.p, s->dEdD.p, s->epart.p,       \
+                       s->Xi.p, s->rowSlot.p, s->grpSpecies.p, s->w1t.p, s->w2t.p, s->w2.p, s->w1.p, s->params.p, s->P, \
+                       s->xstd.p, nrm);                                                                                \
+    hipLaunchKernelGGL(k_sg_force<NT>, dim3(gf), dim3(BLOCK), 0, d->stream, F, s->wpart.p, s->dEdD.p, s->nearE.p,    \
+                       s->nearG.p, s->nearN.p, d->type.p, s->sp, N, Np);                                               \
+  } while (0)
+  switch (s->sp.nt) { case 1: SG_RUN(1); break; case 2: SG_RUN(2); break; case 3: SG_RUN(3); break; default: SG_RUN(4); }
+#undef SG_RUN
+  launch_check("surrogate network and forces");
+  hipLaunchKernelGGL(k_reduce_partials<1>, dim3(1), dim3(BLOCK), 0, d->stream, s->eres.p, s->epart.p, s->ngroups);
+  hipLaunchKernelGGL(k_reduce_partials<1>, dim3(1), dim3(BLOCK), 0, d->stream, s->eres.p + 1, s->wpart.p, gf);
   if (d->wantAtomE)
-    hipLaunchKernelGGL(k_sg_add_atom, dim3(div_up(N, BLOCK)), dim3(BLOCK), 0, s->stream, d->eatom.p, eatom, N);
-  HIP_CHECK(hipStreamSynchronize(s->stream));
-  return E;
+    hipLaunchKernelGGL(k_sg_add_atom, dim3(div_up(N, BLOCK)), dim3(BLOCK), 0, d->stream, d->eatom.p, s->eat.p, N);
+  launch_check("surrogate sums");
+  // results into the device's pinned block (slots 48..50), read after the
+  // one wait below (the low-k evaluation ends with it when it runs)
+  double* hr = d->hresult + 48;
+  HIP_CHECK(hipMemcpyAsync(hr, s->eres.p, 2 * sizeof(double), hipMemcpyDeviceToHost, d->stream));
+  HIP_CHECK(hipMemcpyAsync(reinterpret_cast<int*>(hr + 2), s->nearOver.p, sizeof(int), hipMemcpyDeviceToHost, d->stream));
+  double El = 0.0, Wl = 0.0;
+  if (EwaldState* lk = ensure_lowk(s, d, c, task))
+    El = dev_lowk_compute(d, lk, F, d->wantAtomE ? d->eatom.p : nullptr, c.L, &Wl);
+  else
+    HIP_CHECK(hipStreamSynchronize(d->stream));
+  if (*reinterpret_cast<int*>(hr + 2)) fatal(task, "more neighbours within the descriptor cutoff than the model holds (256)");
+  *W = hr[1] + Wl;
+  return hr[0] + El;
 }
 
 // Stores the raw descriptors of a system's current configuration as training
-// rows, with per-atom labels given in slot order.
-void dev_surrogate_add_sample(Device* d, EmDeeSurrogate* s, double L, const double* labels) {
+// rows, with per-atom labels given in original order; with the long-range
+// channel the rows' labels are the residuals after its atom energies.
+void dev_surrogate_add_sample(Device* d, EmDeeSurrogate* s, const SurrogateContext& c, const double* labels) {
+  const char* task = "surrogate sampling";
   int N = d->N, din = s->sp.din;
+  check_context(s, d, c, task);
   s->stream = d->stream;
   std::vector<int> iperm(N);
   HIP_CHECK(hipMemcpyAsync(iperm.data(), d->iperm.p, sizeof(int) * N, hipMemcpyDeviceToHost, d->stream));
   HIP_CHECK(hipStreamSynchronize(d->stream));
   std::vector<double> labelsSlot(N);
   for (int a = 0; a < N; a++) labelsSlot[iperm[a]] = labels[a];
+  if (EwaldState* lk = ensure_lowk(s, d, c, task)) {
+    // low-k atom energies at this configuration; its forces go to a scratch layer copy
+    s->lowkAtom.alloc(d->Npad);
+    HIP_CHECK(hipMemsetAsync(s->lowkAtom.p, 0, sizeof(double) * d->Npad, d->stream));
+    DBuf<double> Fs;
+    Fs.alloc(3 * (size_t)d->Npad);
+    HIP_CHECK(hipMemsetAsync(Fs.p, 0, sizeof(double) * 3 * d->Npad, d->stream));
+    double Wl = 0.0;
+    dev_lowk_compute(d, lk, Fs.p, s->lowkAtom.p, c.L, &Wl);
+    std::vector<double> el(N);
+    HIP_CHECK(hipMemcpyAsync(el.data(), s->lowkAtom.p, sizeof(double) * N, hipMemcpyDeviceToHost, d->stream));
+    HIP_CHECK(hipStreamSynchronize(d->stream));
+    Fs.release();
+    for (int i = 0; i < N; i++) labelsSlot[i] -= el[i];
+  }
   if (s->nrows + N > s->caprows) {
     long cap = std::max(2 * s->caprows, s->nrows + N);
     DBuf<float> nx, ny;
@@ -633,14 +1041,17 @@ void dev_surrogate_add_sample(Device* d, EmDeeSurrogate* s, double L, const doub
     s->Xs = nx; s->Ys = ny; s->Ss = ns;
     s->caprows = cap;
   }
-  launch_desc(s, d, s->Xs.p + s->nrows * din, L, d->stream, false);
+  launch_desc(s, d, c, s->Xs.p + s->nrows * din, false);
   std::vector<float> y(N);
   for (int i = 0; i < N; i++) y[i] = (float)labelsSlot[i];
   HIP_CHECK(hipMemcpyAsync(s->Ys.p + s->nrows, y.data(), sizeof(float) * N, hipMemcpyHostToDevice, d->stream));
   HIP_CHECK(hipMemcpyAsync(s->Ss.p + s->nrows, d->type.p, sizeof(int) * N, hipMemcpyDeviceToDevice, d->stream));
   std::vector<int> sp(N);
+  int nover = 0;
   HIP_CHECK(hipMemcpyAsync(sp.data(), d->type.p, sizeof(int) * N, hipMemcpyDeviceToHost, d->stream));
+  HIP_CHECK(hipMemcpyAsync(&nover, s->nearOver.p, sizeof(int), hipMemcpyDeviceToHost, d->stream));
   HIP_CHECK(hipStreamSynchronize(d->stream));
+  if (nover) fatal(task, "more neighbours within the descriptor cutoff than the model holds (256)");
   s->hostY.insert(s->hostY.end(), y.begin(), y.end());
   s->hostS.insert(s->hostS.end(), sp.begin(), sp.end());
   s->nrows += N;
@@ -698,6 +1109,21 @@ EmDeeSurrogate* EmDeeSurrogate_create(int ntypes, double rcut, unsigned long lon
   HIP_CHECK(hipStreamSynchronize(s->stream));
   return s;
 }
+
+
+// The exact long-range channel: the reciprocal Ewald sum over the k-vectors
+// with |k| <= kcut (0 disables it).  Set before the first sample: the rows'
+// labels are the residuals after it.
+void EmDeeSurrogate_set_long_range(EmDeeSurrogate* s, double kcut) {
+  if (kcut < 0.0) fatal("surrogate", "the long-range cutoff must not be negative");
+  if (s->nrows > 0) fatal("surrogate", "the long-range channel must be set before the first sample");
+  s->kcut = kcut;
+  dev_lowk_release(s->lowk);
+  s->lowk = nullptr;
+  s->lowkOwner = nullptr;
+}
+
+int EmDeeSurrogate_long_range_vectors(EmDeeSurrogate* s) { return dev_lowk_vectors(s->lowk); }
 
 int EmDeeSurrogate_rows(EmDeeSurrogate* s) { return (int)s->nrows; }
 
@@ -806,7 +1232,11 @@ void EmDeeSurrogate_destroy(EmDeeSurrogate* s) {
                           &s->H1, &s->Z2, &s->H2, &s->G2, &s->G1, &s->dX, &s->e, &s->dXsum, &s->part })
     b->release();
   s->w1t.release(); s->w1.release(); s->w2t.release(); s->w2.release(); s->Ss.release();
-  s->lE.release(); s->lJ.release(); s->lN.release(); s->lOver.release();
+  s->cJ.release(); s->cN.release(); s->cOver.release(); s->rowSlot.release(); s->grpSpecies.release(); s->Rf.release();
+  s->nearE.release(); s->nearG.release(); s->nearN.release(); s->nearOver.release();
+  s->Xi.release(); s->dEdD.release(); s->eat.release(); s->epart.release(); s->wpart.release(); s->eres.release();
+  s->lowkAtom.release();
+  dev_lowk_release(s->lowk);
   s->Fw.release(); s->red.release();
   (void)hipStreamDestroy(s->own);
   delete s;

@@ -132,6 +132,8 @@ GPU_SIGNATURES = {
     "EmDeeSurrogate_destroy": (None, [c_void_p]),
     "EmDeeGPU_attach_surrogate": (None, [tEmDee, c_void_p, c_int]),
     "EmDeeGPU_rdf_noalloc": (None, [tEmDee, c_int, c_double, c_int, _ip, _ip, _dp]),
+    "EmDeeSurrogate_set_long_range": (None, [c_void_p, c_double]),
+    "EmDeeSurrogate_long_range_vectors": (c_int, [c_void_p]),
 }
 
 

@@ -74,10 +74,17 @@ void EmDeeGPU_atom_energies( tEmDee* md, double* e );
    neighbour species within rcut, a 2 x 128 SiLU MLP per species on BF16 MFMA
    with FP32 accumulation), trained on per-atom energy labels.  Attached to a
    model layer it adds its energy, forces and virial to that layer, standing
-   in for a costly term (e.g. the k-space electrostatics).  No counterpart in
-   the reference. */
+   in for a costly term (e.g. the k-space electrostatics).  rcut must not
+   exceed the system's cutoff.  No counterpart in the reference. */
 typedef struct EmDeeSurrogate EmDeeSurrogate;
 EmDeeSurrogate* EmDeeSurrogate_create( int ntypes, double rcut, unsigned long long seed );
+/* Optional exact long-range channel: the model adds the reciprocal Ewald sum
+   over the k-vectors with |k| <= kcut (1/distance; the system's kspace model
+   gives the splitting parameter and charges), and the network learns the
+   labels minus that sum's atom shares.  Set before the first sample; 0
+   disables it.  _long_range_vectors reports the number of k-vectors in use. */
+void   EmDeeSurrogate_set_long_range( EmDeeSurrogate* s, double kcut );
+int    EmDeeSurrogate_long_range_vectors( EmDeeSurrogate* s );
 /* Stores the descriptors of md's current configuration (its neighbor list
    must be current, e.g. after a force evaluation) with one label per atom. */
 void   EmDeeGPU_surrogate_sample( tEmDee md, EmDeeSurrogate* s, const double* labels );

@@ -150,3 +150,39 @@ def loss_grad(params, norm, D, types, y, nt):
         g[o:o + H] = H2.T @ s; o += H
         g[o] = s.sum()
     return g
+
+
+# ---- the exact long-range channel: reciprocal Ewald sum over |k| <= kcut ----
+
+def lowk_vectors(L, kcut):
+    """Half-space integer vectors n (first nonzero component positive) with
+    |2 pi n / L| <= kcut, n3 fastest (the enumeration of kspace_ewald.f90:115-138)."""
+    unit = 2.0 * np.pi / L
+    nmax = max(1, int(np.ceil(kcut / unit)))
+    out = []
+    for n1 in range(0, nmax + 1):
+        for n2 in range(-nmax, nmax + 1):
+            for n3 in range(-nmax, nmax + 1):
+                upper = n1 > 0 or (n1 == 0 and (n2 > 0 or (n2 == 0 and n3 > 0)))
+                if upper and unit * unit * n1 * n1 + unit * unit * n2 * n2 + unit * unit * n3 * n3 <= kcut * kcut:
+                    out.append((n1, n2, n3))
+    return np.array(out, dtype=np.int64).reshape(-1, 3)
+
+
+def lowk_energy_forces(R, q, L, alpha, kcut, kcoul):
+    """E = kcoul sum_k prefac |S(k)|^2 over the half space, prefac =
+    4 pi/V exp(-k^2/(4 alpha^2))/k^2 (kspace_ewald.f90:174-180, 287), forces,
+    atom shares Re(q_j e^{ik.r_j} conj(kcoul S)) prefac, and the virial
+    sum_k E_k (1 - k^2/(2 alpha^2))."""
+    n = lowk_vectors(L, kcut)
+    k = 2.0 * np.pi * n / L
+    k2 = (k * k).sum(1)
+    pref = 4.0 * np.pi / L ** 3 * np.exp(-k2 / (4.0 * alpha ** 2)) / k2
+    ph = R @ k.T
+    c, s = np.cos(ph), np.sin(ph)
+    Sc, Ss = q @ c, q @ s
+    Ek = kcoul * pref * (Sc * Sc + Ss * Ss)
+    F = 2.0 * kcoul * q[:, None] * ((s * Sc - c * Ss) * pref) @ k
+    e = kcoul * q * ((c * Sc + s * Ss) * pref).sum(1)
+    W = np.sum(Ek * (1.0 - k2 / (2.0 * alpha ** 2)))
+    return Ek.sum(), F, e, W

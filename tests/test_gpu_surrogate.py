@@ -1,7 +1,8 @@
 """The learned energy model on the GPU (surrogate.hip) against its FP64
-restatement (tests/surrogate_ref.py): descriptors, BF16-MFMA forward pass,
-forces from the backward pass, the first Adam step of training, and what the
-trained model gains as an HMC proposal force for rigid water."""
+restatement (tests/surrogate_ref.py): descriptors, the BF16-MFMA network and
+its backward pass, forces, the exact long-range channel (reciprocal sum over
+|k| <= kcut), the first Adam step of training and the fit of held-out labels.
+What the trained model gains as a proposal force is in test_gpu_water_hmc.py."""
 import numpy as np
 import pytest
 
@@ -20,7 +21,13 @@ def layer2_state(gpu, md, N):
     return gpu.download(md, "forces", (N, 3)), md.Energy.Coulomb
 
 
-def test_forward_and_forces_match_fp64(gpu):
+def lowk_reference(cfg, R, kcut):
+    q = sg.CHARGE[cfg["types"] - 1]
+    return sr.lowk_energy_forces(R, q, cfg["L"], 0.3, kcut, sg.KCOUL)
+
+
+@pytest.mark.parametrize("kcut", [0.0, 0.9])
+def test_forward_and_forces_match_fp64(gpu, kcut):
     md, cfg = sg.two_layer_water(gpu, NMOL, seed=3)
     N = cfg["N"]
     F0, E0 = layer2_state(gpu, md, N)
@@ -29,7 +36,7 @@ def test_forward_and_forces_match_fp64(gpu):
     D = sr.descriptors(R, types, L, RC, mol, 2)
     norm = np.concatenate([D.mean(0), D.std(0) + 1e-3, [0.02, -0.01], [0.05, 0.03]]).astype(np.float32)
     p = sr.random_params(2, seed=7)
-    s = sg.Surrogate(gpu, 2, RC, seed=1)
+    s = sg.Surrogate(gpu, 2, RC, seed=1, kcut=kcut)
     s.set_params(p, norm)
     s.attach(md, 2)
     F1, E1 = layer2_state(gpu, md, N)
@@ -37,13 +44,78 @@ def test_forward_and_forces_match_fp64(gpu):
     s.detach(md)
     e_base = gpu.atom_energies(md, N)
     E_ref, F_ref, e_ref, _ = sr.energy_forces(p.astype(np.float64), norm.astype(np.float64), R, types, L, RC, mol, 2)
+    if kcut > 0.0:
+        El, Fl, el, _ = lowk_reference(cfg, R, kcut)
+        E_ref, F_ref, e_ref = E_ref + El, F_ref + Fl, e_ref + el
     dF = F1 - F0
     ferr = np.linalg.norm(dF - F_ref) / np.linalg.norm(F_ref)
-    print("surrogate: energy", E1 - E0, E_ref, "force rel rms", ferr)
+    print("surrogate: energy", E1 - E0, E_ref, "force rel rms", ferr, flush=True)
     assert E1 - E0 == pytest.approx(E_ref, rel=2e-2, abs=1e-3 * np.abs(e_ref).sum())
     assert ferr < 3e-2
     assert np.max(np.abs((e_atoms - e_base) - e_ref)) < 3e-2 * np.max(np.abs(e_ref))
     s.close()
+
+
+def test_long_range_channel_is_the_partial_ewald_sum(gpu):
+    """With a network that outputs zero, the attached model is exactly the
+    reciprocal sum over |k| <= kcut: energy, forces, atom energies and virial
+    against the FP64 restatement (pinned to the oracle's Ewald sum)."""
+    kcut = 1.0
+    md, cfg = sg.two_layer_water(gpu, NMOL, seed=6)
+    N = cfg["N"]
+    F0, _ = layer2_state(gpu, md, N)
+    E0, W0 = md.Energy.Potential, md.Virial
+    e0 = gpu.atom_energies(md, N)
+    R = gpu.download(md, "coordinates", (N, 3))
+    s = sg.Surrogate(gpu, 2, RC, seed=1, kcut=kcut)
+    P = len(sr.random_params(2, seed=1)) // 2
+    din = 2 * sr.NB
+    s.set_params(np.zeros(2 * P, np.float32),
+                 np.concatenate([np.zeros(din), np.ones(din), [0.0, 0.0], [1.0, 1.0]]).astype(np.float32))
+    s.attach(md, 2)
+    F1, _ = layer2_state(gpu, md, N)
+    E1, W1 = md.Energy.Potential, md.Virial
+    e1 = gpu.atom_energies(md, N)
+    El, Fl, el, Wl = lowk_reference(cfg, R, kcut)
+    print("long-range channel: vectors", s.long_range_vectors(), "E", E1 - E0, El, "W", W1 - W0, Wl, flush=True)
+    assert s.long_range_vectors() == len(sr.lowk_vectors(cfg["L"], kcut))
+    assert E1 - E0 == pytest.approx(El, rel=1e-9, abs=1e-12 * abs(E0))
+    assert np.max(np.abs((F1 - F0) - Fl)) <= 1e-9 * np.max(np.abs(Fl))
+    assert np.max(np.abs((e1 - e0) - el)) <= 1e-9 * np.max(np.abs(el))
+    # the added forces also change the rigid-body virial, -sum_k F_k . delta_k
+    # (EmDeeData.f90:926-953), delta_k the atom's offset from its molecule's centre of mass
+    m = sg.MASS[cfg["types"] - 1]
+    delta = np.zeros_like(R)
+    for b in range(NMOL):
+        idx = np.arange(3 * b, 3 * b + 3)
+        X = R[idx].copy()
+        X[1:] -= cfg["L"] * np.round((X[1:] - X[0]) / cfg["L"])
+        delta[idx] = X - (m[idx, None] * X).sum(0) / m[idx].sum()
+    assert W1 - W0 == pytest.approx(Wl - np.sum(Fl * delta), rel=1e-8, abs=1e-12 * abs(W0))
+    s.close()
+
+
+def test_descriptor_cutoff_above_the_system_cutoff_is_rejected(gpu):
+    """The compact list follows the engine's Verlet list, which is complete
+    only to the system's cutoff: a larger descriptor cutoff is an error."""
+    import subprocess
+    import sys
+    import os
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    code = (
+        "import sys; sys.path[:0] = [{root!r}]\n"
+        "import emdee_amd\n"
+        "from emdee_amd import surrogate as sg\n"
+        "lib = emdee_amd.load()\n"
+        "md, cfg = sg.two_layer_water(lib, 216, seed=1, rc=7.0, skin=1.0)\n"
+        "s = sg.Surrogate(lib, 2, 8.0, seed=1)\n"
+        "s.attach(md, 2)\n"
+        "print('not reached')\n"
+    ).format(root=root)
+    r = subprocess.run([sys.executable, "-c", code], capture_output=True, text=True, timeout=120,
+                       env=dict(os.environ, EMDEE_QUIET="1"))
+    assert r.returncode == 1, r.stderr
+    assert "Error in surrogate attachment: the descriptor cutoff exceeds the system's cutoff." in r.stderr
 
 
 def test_first_adam_step_follows_gradient(gpu):

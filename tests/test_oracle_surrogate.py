@@ -48,3 +48,62 @@ def test_descriptor_invariances():
     # a pair at the cutoff contributes nothing
     mu, eta = sr.constants(rc)
     assert np.all(0.5 * (np.cos(np.pi * rc / rc) + 1.0) * np.exp(-eta * (rc - mu) ** 2) < 1e-15)
+
+
+def test_lowk_channel_at_the_full_cutoff_is_the_ewald_force(oracle):
+    """With every vector of the reference's set (|k| <= 2 pi nmax / L,
+    kspace_ewald.f90:111-138) the low-k restatement is the reciprocal Ewald
+    sum: its forces equal the oracle's k-space forces (layer 1 minus layer 2
+    of two-layer water; self and intrabody terms carry no force)."""
+    import math
+    from emdee_amd import surrogate as sg
+    md, cfg = sg.two_layer_water(oracle, 64, seed=2)
+    N, L = cfg["N"], cfg["L"]
+    oracle.switch_model_layer(md, 1)
+    F1 = oracle.download(md, "forces", (N, 3))
+    oracle.switch_model_layer(md, 2)
+    F2 = oracle.download(md, "forces", (N, 3))
+    R = oracle.download(md, "coordinates", (N, 3))
+    acc = math.exp(-9.0) / 9.0
+    s = np.sqrt(sr_inverse_x_plus_ln_x(-np.log(acc)))
+    alpha = s / 10.0
+    kmax = 2.0 * alpha * s
+    unit = 2.0 * np.pi / L
+    kfull = unit * np.ceil(kmax / unit)
+    q = sg.CHARGE[cfg["types"] - 1]
+    E, F, e, W = sr.lowk_energy_forces(R, q, L, alpha, kfull, sg.KCOUL)
+    Fk = F1 - F2
+    assert np.max(np.abs(F - Fk)) < 1e-9 * np.max(np.abs(Fk))
+    assert e.sum() == pytest.approx(E, rel=1e-12)
+    # a truncated set is a different sum
+    E8, F8, _, _ = sr.lowk_energy_forces(R, q, L, alpha, 0.8, sg.KCOUL)
+    assert np.max(np.abs(F8 - Fk)) > 1e-3 * np.max(np.abs(Fk))
+
+
+def test_lowk_forces_and_virial_are_derivatives():
+    R, types, mol, L = small_water()
+    q = np.where(types == 0, -0.8476, 0.4238)
+    alpha, kcut, kc = 0.3, 1.0, 0.139
+    E, F, e, W = sr.lowk_energy_forces(R, q, L, alpha, kcut, kc)
+    h = 1e-6
+    for a in (0, 7, 40):
+        for x in range(3):
+            Rp, Rm = R.copy(), R.copy()
+            Rp[a, x] += h
+            Rm[a, x] -= h
+            d = (sr.lowk_energy_forces(Rp, q, L, alpha, kcut, kc)[0] - sr.lowk_energy_forces(Rm, q, L, alpha, kcut, kc)[0])
+            assert F[a, x] == pytest.approx(-d / (2 * h), rel=1e-6, abs=1e-10)
+    # virial: W = -dE/dln(lambda) with positions and box scaled by lambda (same integer vectors)
+    lam = 1e-6
+    Ep = sr.lowk_energy_forces(R * (1 + lam), q, L * (1 + lam), alpha, kcut * (1 - 1e-3), kc)[0]
+    Em = sr.lowk_energy_forces(R * (1 - lam), q, L * (1 - lam), alpha, kcut * (1 - 1e-3), kc)[0]
+    E0, _, _, W0 = sr.lowk_energy_forces(R, q, L, alpha, kcut * (1 - 1e-3), kc)
+    assert W0 == pytest.approx(-(Ep - Em) / (2 * lam), rel=1e-6)
+
+
+def sr_inverse_x_plus_ln_x(y):
+    """Solution x of x + ln x = y (math.f90:642-657), by Newton."""
+    x = y - np.log(y) if y > 0.5671432904097839 else np.exp(y)
+    for _ in range(100):
+        x = x * (y + 1.0 - np.log(x)) / (x + 1.0)
+    return x
