@@ -121,6 +121,53 @@ def force_error(lib, md, N):
     return float(np.linalg.norm(F32 - F64) / np.linalg.norm(F64))
 
 
+def host_cpu():
+    """CPU model, physical cores and the thread count the CPU baselines use:
+    one OpenMP thread per physical core, capped by OMP_NUM_THREADS when the
+    environment sets it (the GPU host's share of cores)."""
+    import subprocess
+    model, cores, logical = "unknown", 0, os.cpu_count() or 1
+    try:
+        out = subprocess.run(["lscpu"], capture_output=True, text=True).stdout
+        for line in out.splitlines():
+            if line.startswith("Model name:"):
+                model = line.split(":", 1)[1].strip()
+        out = subprocess.run(["lscpu", "-p=CPU,CORE,SOCKET"], capture_output=True, text=True).stdout
+        cores = len({tuple(l.split(",")[1:3]) for l in out.splitlines() if l and not l.startswith("#")})
+    except OSError:
+        pass
+    cores = cores or logical
+    cap = int(os.environ.get("OMP_NUM_THREADS", cores))
+    return {"model": model, "physical_cores": cores, "logical_cpus": logical, "threads": max(1, min(cores, cap)),
+            "omp_places": "cores", "omp_proc_bind": "close"}
+
+
+def cpu_child(kind, threads, **kw):
+    """Runs a CPU baseline in a child process whose OpenMP runtime starts
+    with OMP_NUM_THREADS = threads, OMP_PLACES=cores, OMP_PROC_BIND=close."""
+    import subprocess
+    env = dict(os.environ, OMP_NUM_THREADS=str(threads), OMP_PLACES="cores", OMP_PROC_BIND="close")
+    code = ("import json, sys; sys.path.insert(0, %r); import bench; "
+            "print(json.dumps(bench.%s(**json.loads(sys.argv[1]))))" % (ROOT, kind))
+    r = subprocess.run([sys.executable, "-c", code, json.dumps(dict(kw, threads=threads))], capture_output=True,
+                       text=True, env=env)
+    if r.returncode != 0:
+        raise RuntimeError(f"cpu baseline {kind} failed: {r.stderr[-2000:]}")
+    return json.loads(r.stdout.strip().splitlines()[-1])
+
+
+def cpu_lj(natoms, steps, dt, threads):
+    R, L = fcc_lattice(natoms, 0.8442)
+    v, t = cpu_baseline(R, L, steps, dt, threads)
+    return [v, t]
+
+
+def cpu_hmc(nrep, threads, seed=1):
+    from emdee_amd.hmc import replica_set
+    v, t = hmc_cpu_baseline(replica_set(nrep, seed=seed), nrep, threads, seed)
+    return [v, t]
+
+
 def cpu_baseline(R, L, steps, dt, threads):
     from emdee_amd.emdee import EmDeeLibrary
     so = os.path.join(ROOT, "oracle", "liboracle_emdee_fast.so")
@@ -185,7 +232,8 @@ def hmc_cpu_baseline(replicas, sample, threads, seed=1):
 def water_bench(lib, nmol, steps, warm, mode, dt=2.0, seed=1):
     """Rigid SPC/E water (32,768 molecules, 98,304 atoms, L = 99.3 A):
     coul_damped_smoothed(0.2, 1 A), rc 10 A, skin 2 A, exact rotations,
-    velocity Verlet through the C ABI with energies every step."""
+    velocity Verlet through the C ABI with energies every step.  Also the
+    mixed form's force RMS error against FP64 on the final coordinates."""
     from emdee_amd.water_box import KB, water_system
     md, cfg = water_system(lib, nmol, seed, coul="damped_smoothed", alpha=0.2)
     lib.EmDeeGPU_set_precision(md, mode)
@@ -196,8 +244,28 @@ def water_bench(lib, nmol, steps, warm, mode, dt=2.0, seed=1):
     run_steps(lib, md, steps, dt)
     lib.EmDeeGPU_synchronize(md)
     t = time.perf_counter() - t0
+    err = force_error(lib, md, cfg["N"])
     lib.EmDeeGPU_destroy(ctypes.byref(md))
-    return cfg["N"] * steps / t, 1e3 * t / steps
+    return cfg["N"] * steps / t, 1e3 * t / steps, err
+
+
+def lj_4m_bench(lib, steps, warm, dt=0.005):
+    """LJ at 4,194,304 atoms (fcc lattice of 102^3 cells truncated to N,
+    spanning the box), both forms, with the mixed form's force error."""
+    N = 4194304
+    R, L = fcc_lattice(N, 0.8442)
+    out = {"natoms": N}
+    for mode, name in ((0, "fp64"), (1, "mixed")):
+        md, r = gpu_run(lib, R, L, mode, steps, warm, dt)
+        out[name + "_atom_steps_per_s"] = N * steps / r["seconds"]
+        out[name + "_pair_kernel_ms"] = r["pair_ms"]
+        out[name + "_list_builds"] = r["builds"]
+        if mode == 1:
+            out["force_rms_error"] = force_error(lib, md, N)
+        lib.EmDeeGPU_destroy(ctypes.byref(md))
+    out["mixed_kernel_over_fp64_kernel"] = out["fp64_pair_kernel_ms"] / out["mixed_pair_kernel_ms"]
+    out["steps"] = steps
+    return out
 
 
 def layers_bench(lib, nmol, outer, seed=1):
@@ -242,6 +310,8 @@ def main():
     ap.add_argument("--no-hmc", action="store_true")
     ap.add_argument("--no-water", action="store_true")
     ap.add_argument("--no-water-hmc", action="store_true")
+    ap.add_argument("--no-4m", action="store_true")
+    ap.add_argument("--steps-4m", type=int, default=100)
     ap.add_argument("--water-hmc-cycles", type=int, default=120)
     args = ap.parse_args()
     if args.gpus != 1:
@@ -272,13 +342,20 @@ def main():
     bytes_per_launch = N * (4 * n_half + 24 + 24)
     achieved = bytes_per_launch / (best["pair_ms"] * 1e-3) / 1e9
 
+    lj4m = None
+    if not args.no_4m:
+        print("# LJ 4,194,304 atoms, both forms", file=sys.stderr, flush=True)
+        lj4m = lj_4m_bench(lib, args.steps_4m, 10, dt)
+        print(f"# lj_4m: {lj4m}", file=sys.stderr, flush=True)
+
     cpu = None
+    host = host_cpu()
     if not args.no_cpu:
-        threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+        threads = host["threads"]
         print(f"# cpu baseline: oracle, {threads} threads, {args.cpu_steps} steps", file=sys.stderr, flush=True)
-        cv, ct = cpu_baseline(R, L, args.cpu_steps, dt, threads)
+        cv, ct = cpu_child("cpu_lj", threads, natoms=N, steps=args.cpu_steps, dt=dt)
         cpu = {"value": cv, "unit": "atom-steps/s", "cores": threads, "kind": "port",
-               "build": "oracle/liboracle_emdee_fast.so (-Ofast)",
+               "build": "oracle/liboracle_emdee_fast.so (-Ofast)", "host": host,
                "sample": f"{N}-atom LJ, {args.cpu_steps} steps after 1 warm-up step ({ct:.1f} s), same set-up"}
 
     hmc = None
@@ -287,19 +364,21 @@ def main():
         hmc, reps = hmc_bench(lib, args.hmc_replicas, args.hmc_cycles, warm=20)
         print(f"# hmc: {hmc}", file=sys.stderr, flush=True)
         if not args.no_cpu:
-            threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
-            hv, ht = hmc_cpu_baseline(reps, len(reps[0]), threads)
+            threads = host["threads"]
+            hv, ht = cpu_child("cpu_hmc", threads, nrep=len(reps[0]))
             hmc["cpu_baseline"] = {"value": hv, "unit": "atom-steps/s", "cores": threads, "kind": "port",
+                                   "host": host,
                                    "sample": f"reference-API replay of all {len(reps[0])} replicas, one 10-step "
                                              f"cycle each ({ht:.1f} s)"}
 
     water = None
     if not args.no_water:
         print("# rigid water: 32768 molecules; two-layer water: 4096 molecules", file=sys.stderr, flush=True)
-        w64, m64 = water_bench(lib, 32768, 100, 10, 0)
-        wmx, mmx = water_bench(lib, 32768, 100, 10, 1)
+        w64, m64, _ = water_bench(lib, 32768, 100, 10, 0)
+        wmx, mmx, werr = water_bench(lib, 32768, 100, 10, 1)
         water = {"rigid_spce_98304": {"fp64_atom_steps_per_s": w64, "fp64_ms_per_step": m64,
                                       "mixed_atom_steps_per_s": wmx, "mixed_ms_per_step": mmx,
+                                      "mixed_force_rms_error": werr,
                                       "coul": "damped_smoothed(0.2, 1.0)", "rc": 10.0, "skin": 2.0, "dt_fs": 2.0},
                  "two_layer_spce_12288": layers_bench(lib, 4096, 10)}
         print(f"# water: {water}", file=sys.stderr, flush=True)
@@ -323,7 +402,8 @@ def main():
                    "fp64_pair_kernel_ms": r64["pair_ms"], "mixed_pair_kernel_ms": rmx["pair_ms"],
                    "mixed_kernel_over_fp64_kernel": r64["pair_ms"] / rmx["pair_ms"],
                    "list_build_ms": best["list_ms"],
-                   "force_rms_error": err, "list_builds": best["builds"], "half_list_pairs_per_atom": n_half},
+                   "force_rms_error": err, "list_builds": best["builds"], "half_list_pairs_per_atom": n_half,
+                   "lj_4194304": lj4m},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS,
                      "traffic": measured_traffic("mixed" if use_mixed else "fp64"),

@@ -80,8 +80,9 @@ struct System {
   std::vector<EnergyRec> layerEnergy;
   std::vector<VirialRec> layerVirial;
   bool multilayer_coulomb = false;
-  int nbonds = 0, nangles = 0;
+  int nbonds = 0, nangles = 0, ndihedrals = 0;
   std::vector<DevStruct> bonds, angles;     // structs.f90 lists, 0-based atoms
+  std::vector<DevDihedral> dihedrals;
   EwaldHost ewald;
   EmDeeSurrogate* surrogate = nullptr;       // learned model added to one layer
   int surrogateLayer = 0;
@@ -281,7 +282,7 @@ void perform_initialization(System* me, tEmDee* md) {
     for (int l = 1; l <= me->nlayers; l++)
       if (me->coul[l - 1].requires_kspace) coul_kspace_setup(me->coul[l - 1], me->ewald.alpha);
   }
-  if (me->nbonds + me->nangles > 0) dev_structs_setup(me->dev, me->bonds, me->angles);
+  if (me->nbonds + me->nangles + me->ndihedrals > 0) dev_structs_setup(me->dev, me->bonds, me->angles, me->dihedrals);
   upload_models(me);
   me->initialized = true;
 }
@@ -342,7 +343,7 @@ static bool forces_begin(tEmDee* md, bool defer, ForceSums& fs) {
   c.Rc2 = (me->useInRc[l] ? me->InRcSq : me->RcSq) * c.invL2;
   c.scaled = 1;
   fs = ForceSums{ 0, 0, 0, 0 };
-  bool pairsOnly = me->nbonds + me->nangles == 0 && !me->coul[l - 1].requires_kspace &&
+  bool pairsOnly = me->nbonds + me->nangles + me->ndihedrals == 0 && !me->coul[l - 1].requires_kspace &&
                    !(me->surrogate && me->surrogateLayer == l) && me->nbodies == 0;
   c.defer = defer && pairsOnly && me->pairs_exist[l];
   // handle_neighbor_lists: neighbor_lists.f90:175-195.  When the recent
@@ -377,18 +378,20 @@ static bool forces_begin(tEmDee* md, bool defer, ForceSums& fs) {
 
 static void forces_end(tEmDee* md, const ForceSums& fs) {
   System* me = S(*md);
-  double E[5] = { 0, 0, 0, 0, 0 }, W[5] = { 0, 0, 0, 0, 0 };
-  enum { PAIR_ = 0, COUL_ = 1, LONG_ = 2, BOND_ = 3, ANGLE_ = 4 };
+  double E[6] = { 0, 0, 0, 0, 0, 0 }, W[6] = { 0, 0, 0, 0, 0, 0 };
+  enum { PAIR_ = 0, COUL_ = 1, LONG_ = 2, BOND_ = 3, ANGLE_ = 4, DIHEDRAL_ = 5 };
   bool compute = md->Options.Compute || me->atomEnergies;
   int l = me->layer;
   E[PAIR_] = fs.Epair; E[COUL_] = fs.Ecoul; W[PAIR_] = fs.Wpair; W[COUL_] = fs.Wcoul;
-  // compute_bonds / compute_angles, EmDeeData.f90:443-550
+  // compute_bonds / compute_angles, EmDeeData.f90:443-550, and the dihedrals
+  // (EmDeeData.f90:554-640, which the reference does not call)
   bool kspace = me->coul[l - 1].requires_kspace;
-  if (me->nbonds + me->nangles > 0 && (me->bonded[l] || kspace)) {
-    double o[5];
+  if (me->nbonds + me->nangles + me->ndihedrals > 0 && (me->bonded[l] || kspace)) {
+    double o[7];
     dev_structs_compute(me->dev, l - 1, me->L, me->ntypes, me->bonded[l], kspace, me->ewald.alpha, me->ewald.beta, o);
     E[BOND_] = o[0]; W[BOND_] = o[1]; E[ANGLE_] = o[2]; W[ANGLE_] = o[3];
     E[COUL_] += o[4];
+    E[DIHEDRAL_] = o[5]; W[DIHEDRAL_] = o[6];
   }
   // compute_kspace, EmDeeData.f90:689-700; virial EmDeeCode.f90:1248-1252
   if (kspace) {
@@ -401,7 +404,7 @@ static void forces_end(tEmDee* md, const ForceSums& fs) {
     E[COUL_] += Es;
     W[COUL_] += Ws;
   }
-  md->Virial = W[0] + W[1] + W[2] + W[3] + W[4];
+  md->Virial = W[0] + W[1] + W[2] + W[3] + W[4] + W[5];
   if (me->nbodies != 0) {
     md->BodyVirial = dev_bodies_virial(me->dev, l - 1);
     md->Virial = md->Virial + md->BodyVirial;
@@ -411,7 +414,8 @@ static void forces_end(tEmDee* md, const ForceSums& fs) {
     md->Energy.Coulomb = E[COUL_] + E[LONG_];
     md->Energy.Bond = E[BOND_];
     md->Energy.Angle = E[ANGLE_];
-    md->Energy.Potential = E[0] + E[1] + E[2] + E[3] + E[4];
+    md->Energy.Dihedral = E[DIHEDRAL_];
+    md->Energy.Potential = E[0] + E[1] + E[2] + E[3] + E[4] + E[5];
     md->Energy.ShadowPotential = md->Energy.Potential;
   }
   md->Energy.UpToDate = compute;
@@ -789,13 +793,16 @@ void EmDee_add_angle(tEmDee md, int i, int j, int k, void* model) {
   EmDee_ignore_pair(md, j, k);
 }
 
-// EmDeeCode.f90:628-655 (no dihedral model exists; only exclusions apply)
+// EmDeeCode.f90:628-655
 void EmDee_add_dihedral(tEmDee md, int i, int j, int k, int l, void* model) {
   System* me = S(md);
   if (!ranged({ i, j, k, l }, me->natoms)) fatal("add_dihedral", "atom index out of range");
   if (!model) fatal("add_dihedral", "a valid model must be provided");
   const Model& m = *static_cast<Model*>(model);
-  if (m.kind != DIHEDRAL_NONE) fatal("add_dihedral", "the provided model must be a dihedral model");
+  if (m.kind != DIHEDRAL_NONE && m.kind != DIHEDRAL_HARMONIC)
+    fatal("add_dihedral", "the provided model must be a dihedral model");
+  me->ndihedrals++;
+  me->dihedrals.push_back(DevDihedral{ i - 1, j - 1, k - 1, l - 1, m.kind, 0, m.half_k, m.minus_k, m.x0, 0.0 });
   EmDee_ignore_pair(md, i, j); EmDee_ignore_pair(md, i, k); EmDee_ignore_pair(md, i, l);
   EmDee_ignore_pair(md, j, k); EmDee_ignore_pair(md, j, l); EmDee_ignore_pair(md, k, l);
 }

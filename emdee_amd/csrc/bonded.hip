@@ -1,7 +1,7 @@
 // bonded.hip -- harmonic bonds and angles (EmDeeData.f90:443-550,
-// bond_harmonic.f90:69-81, angle_harmonic.f90:69-79) and, when an Ewald
-// solver is active, the erf discount of the excluded 1-2 and 1-3 pairs
-// (modelClass_kspace.f90:321-334).
+// bond_harmonic.f90:69-81, angle_harmonic.f90:69-79), harmonic dihedrals
+// (EmDeeData.f90:554-640) and, when an Ewald solver is active, the erf
+// discount of the excluded 1-2 and 1-3 pairs (modelClass_kspace.f90:321-334).
 //
 // One thread per bond or angle computes its force contributions into a small
 // per-structure array; a gather kernel then adds, for every atom, the
@@ -18,18 +18,21 @@
 namespace emdee {
 
 struct StructState {
-  int nb = 0, na = 0;
+  int nb = 0, na = 0, nd = 0;
   DBuf<DevStruct> bonds, angles;
-  DBuf<double> bF, aF, part, result;
-  DBuf<int> incOff, incItem;   // incidences of each original atom: (index << 3) | role
+  DBuf<DevDihedral> dihedrals;
+  DBuf<double> bF, aF, dF, part, result;
+  DBuf<int> incOff, incItem;   // incidences of each original atom: (index << 4) | role
 };
 
-enum { ROLE_BOND_I = 0, ROLE_BOND_J = 1, ROLE_ANGLE_I = 2, ROLE_ANGLE_K = 3, ROLE_ANGLE_J = 4 };
+enum { ROLE_BOND_I = 0, ROLE_BOND_J = 1, ROLE_ANGLE_I = 2, ROLE_ANGLE_K = 3, ROLE_ANGLE_J = 4, ROLE_DIH_I = 5,
+       ROLE_DIH_J = 6, ROLE_DIH_K = 7, ROLE_DIH_L = 8 };
 
 void dev_structs_release(Device* d) {
   StructState* s = d->structs;
   if (!s) return;
-  s->bonds.release(); s->angles.release(); s->bF.release(); s->aF.release(); s->part.release();
+  s->bonds.release(); s->angles.release(); s->dihedrals.release(); s->bF.release(); s->aF.release();
+  s->dF.release(); s->part.release();
   s->result.release(); s->incOff.release(); s->incItem.release();
   delete s;
   d->structs = nullptr;
@@ -144,10 +147,91 @@ __global__ void __launch_bounds__(BLOCK) k_angles(double* __restrict__ aF, doubl
     for (int k = 0; k < 3; k++) part[3 * blockIdx.x + k] = acc[k];
 }
 
+// EmDeeData.f90:554-640 (declared in the reference, not called by its force
+// driver), in real units: z along r_kj, x the part of r_ij normal to it, y =
+// z x x, phi = atan2(r_lk . y, r_lk . x); Fd = -dE/dphi.  dF per dihedral:
+// force on i, j, k, l, with j's force -(Fi + Fk + Fl) so that the four sum to
+// zero (the reference's line :604 adds +(Fi + Fk + Fl)).  Out per block:
+// {Edihedral, Wdihedral}.
+__device__ __forceinline__ void cross3(const double* a, const double* b, double* c) {
+  c[0] = a[1] * b[2] - a[2] * b[1];
+  c[1] = a[2] * b[0] - a[0] * b[2];
+  c[2] = a[0] * b[1] - a[1] * b[0];
+}
+__device__ __forceinline__ double dot3(const double* a, const double* b) { return a[0] * b[0] + a[1] * b[1] + a[2] * b[2]; }
+
+__global__ void __launch_bounds__(BLOCK) k_dihedrals(double* __restrict__ dF, double* __restrict__ part,
+                                                     const DevDihedral* __restrict__ dih, const double* __restrict__ R,
+                                                     const int* __restrict__ iperm, double L, int nd, int Npad) {
+  int m = blockIdx.x * blockDim.x + threadIdx.x;
+  double acc[2] = { 0.0, 0.0 };
+  if (m < nd) {
+    DevDihedral s = dih[m];
+    int si = iperm[s.i], sj = iperm[s.j], sk = iperm[s.k], sl = iperm[s.l];
+    double rij[3], rkj[3], rlk[3], x[3], y[3], z[3], u[3], v[3], w[3], c[3], Fi[3], Fk[3], Fl[3];
+    for (int q = 0; q < 3; q++) {
+      double Rj = R[q * (size_t)Npad + sj] / L, Rk = R[q * (size_t)Npad + sk] / L;
+      rij[q] = R[q * (size_t)Npad + si] / L - Rj;
+      rkj[q] = Rk - Rj;
+      rlk[q] = R[q * (size_t)Npad + sl] / L - Rk;
+      rij[q] = L * (rij[q] - round(rij[q]));
+      rkj[q] = L * (rkj[q] - round(rkj[q]));
+      rlk[q] = L * (rlk[q] - round(rlk[q]));
+    }
+    double normRkj = sqrt(dot3(rkj, rkj));
+    for (int q = 0; q < 3; q++) z[q] = rkj[q] / normRkj;
+    double rz = dot3(rij, z);
+    for (int q = 0; q < 3; q++) x[q] = rij[q] - rz * z[q];
+    double normX = sqrt(dot3(x, x));
+    for (int q = 0; q < 3; q++) x[q] = x[q] / normX;
+    cross3(z, x, y);
+    double a = dot3(x, rlk), b = dot3(y, rlk);
+    double phi = atan2(b, a), Ed = 0.0, Fd = 0.0;
+    if (s.kind == DIHEDRAL_HARMONIC) {
+      const double twoPi = 6.28318530717958648;
+      double delta = phi - s.x0;
+      delta = delta - twoPi * round(delta / twoPi);
+      Ed = s.half_k * delta * delta;
+      Fd = s.minus_k * delta;
+    }
+    Fd = Fd / (a * a + b * b);
+    cross3(rlk, z, c);
+    for (int q = 0; q < 3; q++) u[q] = (a * c[q] - b * rlk[q]) / normX;
+    cross3(rlk, x, c);
+    double zu = dot3(z, u);
+    for (int q = 0; q < 3; q++) v[q] = (a * c[q] + zu * rij[q]) / normRkj;
+    double zr = dot3(z, rij);
+    for (int q = 0; q < 3; q++) w[q] = v[q] + zr * u[q] / normRkj;
+    double uy = dot3(u, y), vx = dot3(v, x), wy = dot3(w, y);
+    for (int q = 0; q < 3; q++) {
+      Fi[q] = Fd * uy * y[q];
+      Fl[q] = Fd * (a * y[q] - b * x[q]);
+      Fk[q] = -(Fd * (vx * x[q] + wy * y[q]) + Fl[q]);
+    }
+    double* o = dF + 12 * (size_t)m;
+    for (int q = 0; q < 3; q++) {
+      o[q] = Fi[q];
+      o[3 + q] = -(Fi[q] + Fk[q] + Fl[q]);
+      o[6 + q] = Fk[q];
+      o[9 + q] = Fl[q];
+    }
+    acc[0] = Ed;
+    acc[1] = (Fi[0] * rij[0] + Fk[0] * rkj[0] + Fl[0] * (rlk[0] + rkj[0])) +
+             (Fi[1] * rij[1] + Fk[1] * rkj[1] + Fl[1] * (rlk[1] + rkj[1])) +
+             (Fi[2] * rij[2] + Fk[2] * rkj[2] + Fl[2] * (rlk[2] + rkj[2]));
+  }
+  block_sum<2>(acc);
+  if (threadIdx.x == 0) {
+    part[2 * blockIdx.x] = acc[0];
+    part[2 * blockIdx.x + 1] = acc[1];
+  }
+}
+
 // F(slot) += contributions of the structures of atom perm[slot], in the
 // fixed order of the incidence list.
 __global__ void k_structs_gather(double* __restrict__ F, const double* __restrict__ bF, const double* __restrict__ aF,
-                                 const int* __restrict__ incOff, const int* __restrict__ incItem,
+                                 const double* __restrict__ dF, const int* __restrict__ incOff,
+                                 const int* __restrict__ incItem,
                                  const int* __restrict__ perm, int N, int Npad) {
   int s = blockIdx.x * blockDim.x + threadIdx.x;
   if (s >= N) return;
@@ -156,7 +240,7 @@ __global__ void k_structs_gather(double* __restrict__ F, const double* __restric
   if (e0 == e1) return;
   double f[3] = { 0.0, 0.0, 0.0 };
   for (int e = e0; e < e1; e++) {
-    int it = incItem[e], idx = it >> 3, role = it & 7;
+    int it = incItem[e], idx = it >> 4, role = it & 15;
     const double* src;
     double sign = 1.0;
     switch (role) {
@@ -164,34 +248,47 @@ __global__ void k_structs_gather(double* __restrict__ F, const double* __restric
       case ROLE_BOND_J: src = bF + 3 * (size_t)idx; sign = -1.0; break;
       case ROLE_ANGLE_I: src = aF + 9 * (size_t)idx; break;
       case ROLE_ANGLE_K: src = aF + 9 * (size_t)idx + 3; break;
-      default: src = aF + 9 * (size_t)idx + 6; break;
+      case ROLE_ANGLE_J: src = aF + 9 * (size_t)idx + 6; break;
+      default: src = dF + 12 * (size_t)idx + 3 * (role - ROLE_DIH_I); break;
     }
     for (int x = 0; x < 3; x++) f[x] += sign * src[x];
   }
   for (int x = 0; x < 3; x++) F[x * (size_t)Npad + s] += f[x];
 }
 
-void dev_structs_setup(Device* d, const std::vector<DevStruct>& bonds, const std::vector<DevStruct>& angles) {
+void dev_structs_setup(Device* d, const std::vector<DevStruct>& bonds, const std::vector<DevStruct>& angles,
+                       const std::vector<DevDihedral>& dihedrals) {
   if (!d->structs) d->structs = new StructState();
   StructState* st = d->structs;
   int N = d->N;
   st->nb = (int)bonds.size();
   st->na = (int)angles.size();
+  st->nd = (int)dihedrals.size();
   for (const DevStruct& b : bonds)
     if (b.i < 0 || b.i >= N || b.j < 0 || b.j >= N) fatal("bonded set-up", "atom index out of range");
   for (const DevStruct& a : angles)
     if (a.i < 0 || a.i >= N || a.j < 0 || a.j >= N || a.k < 0 || a.k >= N) fatal("bonded set-up", "atom index out of range");
-  if (st->nb >= (1 << 27) || st->na >= (1 << 27)) fatal("bonded set-up", "too many bonds or angles");
-  // incidences in structure order: bonds first, then angles
+  for (const DevDihedral& h : dihedrals)
+    if (h.i < 0 || h.i >= N || h.j < 0 || h.j >= N || h.k < 0 || h.k >= N || h.l < 0 || h.l >= N)
+      fatal("bonded set-up", "atom index out of range");
+  if (st->nb >= (1 << 26) || st->na >= (1 << 26) || st->nd >= (1 << 26))
+    fatal("bonded set-up", "too many bonds, angles or dihedrals");
+  // incidences in structure order: bonds, then angles, then dihedrals
   std::vector<std::vector<int>> inc(N);
   for (int b = 0; b < st->nb; b++) {
-    inc[bonds[b].i].push_back((b << 3) | ROLE_BOND_I);
-    inc[bonds[b].j].push_back((b << 3) | ROLE_BOND_J);
+    inc[bonds[b].i].push_back((b << 4) | ROLE_BOND_I);
+    inc[bonds[b].j].push_back((b << 4) | ROLE_BOND_J);
   }
   for (int m = 0; m < st->na; m++) {
-    inc[angles[m].i].push_back((m << 3) | ROLE_ANGLE_I);
-    inc[angles[m].k].push_back((m << 3) | ROLE_ANGLE_K);
-    inc[angles[m].j].push_back((m << 3) | ROLE_ANGLE_J);
+    inc[angles[m].i].push_back((m << 4) | ROLE_ANGLE_I);
+    inc[angles[m].k].push_back((m << 4) | ROLE_ANGLE_K);
+    inc[angles[m].j].push_back((m << 4) | ROLE_ANGLE_J);
+  }
+  for (int m = 0; m < st->nd; m++) {
+    inc[dihedrals[m].i].push_back((m << 4) | ROLE_DIH_I);
+    inc[dihedrals[m].j].push_back((m << 4) | ROLE_DIH_J);
+    inc[dihedrals[m].k].push_back((m << 4) | ROLE_DIH_K);
+    inc[dihedrals[m].l].push_back((m << 4) | ROLE_DIH_L);
   }
   std::vector<int> off(N + 1, 0), items;
   for (int a = 0; a < N; a++) { items.insert(items.end(), inc[a].begin(), inc[a].end()); off[a + 1] = (int)items.size(); }
@@ -201,23 +298,25 @@ void dev_structs_setup(Device* d, const std::vector<DevStruct>& bonds, const std
   };
   put(st->bonds, bonds);
   put(st->angles, angles);
+  put(st->dihedrals, dihedrals);
   put(st->incOff, off);
   put(st->incItem, items);
   st->bF.alloc(3 * (size_t)st->nb);
   st->aF.alloc(9 * (size_t)st->na);
-  st->part.alloc(3 * (size_t)std::max(div_up(st->nb, BLOCK), div_up(st->na, BLOCK)));
+  st->dF.alloc(12 * (size_t)st->nd);
+  st->part.alloc(3 * (size_t)std::max(std::max(div_up(st->nb, BLOCK), div_up(st->na, BLOCK)), div_up(st->nd, BLOCK)));
   st->result.alloc(8);
 }
 
 void dev_structs_compute(Device* d, int layer, double L, int ntypes, bool bonded, bool kspace, double alpha,
                          double beta, double* out) {
-  for (int k = 0; k < 5; k++) out[k] = 0.0;
+  for (int k = 0; k < 7; k++) out[k] = 0.0;
   StructState* st = d->structs;
-  if (!st || (st->nb == 0 && st->na == 0) || d->N == 0) return;
+  if (!st || (st->nb == 0 && st->na == 0 && st->nd == 0) || d->N == 0) return;
   int Np = d->Npad;
   double* F = d->F.p + 3 * (size_t)Np * layer;
   const DevPair* pt = d->pairs.p + (size_t)layer * ntypes * ntypes;
-  int gb = div_up(st->nb, BLOCK), ga = div_up(st->na, BLOCK);
+  int gb = div_up(st->nb, BLOCK), ga = div_up(st->na, BLOCK), gd = div_up(st->nd, BLOCK);
   if (st->nb > 0) {
     hipLaunchKernelGGL(k_bonds, dim3(gb), dim3(BLOCK), 0, d->stream, st->bF.p, st->part.p, st->bonds.p, d->R.p,
                        d->iperm.p, d->type.p, d->charge.p, pt, ntypes, L, (int)bonded, (int)kspace, alpha, beta,
@@ -233,19 +332,31 @@ void dev_structs_compute(Device* d, int layer, double L, int ntypes, bool bonded
     launch_check("angles");
     hipLaunchKernelGGL(k_reduce_partials<3>, dim3(1), dim3(BLOCK), 0, d->stream, st->result.p + 3, st->part.p, ga);
   }
+  // dihedrals are bonded terms only (no k-space discount of their 1-4 pairs)
+  if (st->nd > 0 && bonded) {
+    hipLaunchKernelGGL(k_dihedrals, dim3(gd), dim3(BLOCK), 0, d->stream, st->dF.p, st->part.p, st->dihedrals.p, d->R.p,
+                       d->iperm.p, L, st->nd, Np);
+    launch_check("dihedrals");
+    hipLaunchKernelGGL(k_reduce_partials<2>, dim3(1), dim3(BLOCK), 0, d->stream, st->result.p + 6, st->part.p, gd);
+  } else if (st->nd > 0) {
+    HIP_CHECK(hipMemsetAsync(st->dF.p, 0, sizeof(double) * 12 * (size_t)st->nd, d->stream));
+  }
   hipLaunchKernelGGL(k_structs_gather, dim3(div_up(d->N, BLOCK)), dim3(BLOCK), 0, d->stream, F, st->bF.p, st->aF.p,
-                     st->incOff.p, st->incItem.p, d->perm.p, d->N, Np);
+                     st->dF.p, st->incOff.p, st->incItem.p, d->perm.p, d->N, Np);
   launch_check("bonded gather");
-  HIP_CHECK(hipMemcpyAsync(d->hresult, st->result.p, 6 * sizeof(double), hipMemcpyDeviceToHost, d->stream));
+  HIP_CHECK(hipMemcpyAsync(d->hresult, st->result.p, 8 * sizeof(double), hipMemcpyDeviceToHost, d->stream));
   HIP_CHECK(hipStreamSynchronize(d->stream));
   double* h = d->hresult;
   if (st->nb == 0) h[0] = h[1] = h[2] = 0.0;
   if (st->na == 0) h[3] = h[4] = h[5] = 0.0;
+  if (st->nd == 0 || !bonded) h[6] = h[7] = 0.0;
   out[0] = h[0];
   out[1] = h[1];
   out[2] = h[3];
   out[3] = h[4];
   out[4] = h[2] + h[5];
+  out[5] = h[6];
+  out[6] = h[7];
 }
 
 }  // namespace emdee

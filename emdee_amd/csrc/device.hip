@@ -1003,8 +1003,9 @@ k_pairs_mx(double* __restrict__ F, double* __restrict__ partials, const int4* __
       // Branch-free groups of four: every entry is evaluated and masked, and
       // only a group holding an entry in the cutoff band takes the FP64 test.
       const float c0 = float(pu.c0), c1 = float(pu.c1), c2 = float(pu.c2), es = float(pu.eshift);
-      // lists are padded to a multiple of 4 with the atom itself; its zero
-      // separation marks the padding
+      // lists are padded to a multiple of 4 with the atom itself, masked by
+      // index (a real neighbour at zero separation is evaluated, and gives a
+      // non-finite force as in the FP64 path)
       const int n4 = (n + 3) & ~3;
       auto group4 = [&](const int* j, const int4* q) {
         MixedAcc a = { 0, 0, 0, 0, 0, 0, 0 };
@@ -1014,7 +1015,7 @@ k_pairs_mx(double* __restrict__ F, double* __restrict__ partials, const int4* __
           float dy = (float)(int)((unsigned int)qi.y - (unsigned int)q[u].y);
           float dz = (float)(int)((unsigned int)qi.z - (unsigned int)q[u].z);
           float r2 = dx * dx + dy * dy + dz * dz;
-          if (r2 == 0.0f) r2 = mc.hi;
+          if (j[u] == i) r2 = mc.hi;
           bool in = r2 < mc.lo;
           if (r2 >= mc.lo && r2 < mc.hi) in = exact_inside_R(R, Npad, mc.L, i, j[u], mc.Rc2s);
           float invR2 = mc.K * __builtin_amdgcn_rcpf(r2);

@@ -29,6 +29,7 @@ enum ModelKind : int {
   BOND_NONE, BOND_HARMONIC,      // modelClass_bond.f90, bond_harmonic.f90
   ANGLE_NONE, ANGLE_HARMONIC,    // modelClass_angle.f90, angle_harmonic.f90
   DIHEDRAL_NONE,                 // modelClass_dihedral.f90
+  DIHEDRAL_HARMONIC,             // harmonic dihedral (no reference model file; see models.cpp)
   KSPACE_EWALD                   // kspace_ewald.f90
 };
 

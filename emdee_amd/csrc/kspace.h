@@ -69,10 +69,15 @@ struct DevStruct {
   int i, j, k, kind;          // 0-based original atoms; kind of the model
   double half_k, minus_k, x0, pad;
 };
-void dev_structs_setup(Device* d, const std::vector<DevStruct>& bonds, const std::vector<DevStruct>& angles);
+struct DevDihedral {
+  int i, j, k, l, kind, pad;  // 0-based original atoms; kind of the model
+  double half_k, minus_k, x0, pad2;
+};
+void dev_structs_setup(Device* d, const std::vector<DevStruct>& bonds, const std::vector<DevStruct>& angles,
+                       const std::vector<DevDihedral>& dihedrals);
 // Adds the bonded forces (and, when kspace is on, the erf discounts of the
 // excluded 1-2 and 1-3 pairs) to F(layer); out = {Ebond, Wbond, Eangle,
-// Wangle, Ecoul discount}.
+// Wangle, Ecoul discount, Edihedral, Wdihedral}.
 void dev_structs_compute(Device* d, int layer, double L, int ntypes, bool bonded, bool kspace, double alpha,
                          double beta, double* out);
 

@@ -303,6 +303,14 @@ void* EmDee_bond_harmonic(double k, double r0) {
 void* EmDee_angle_harmonic(double k, double theta0) {
   Model* m = make(ANGLE_HARMONIC); m->k = k; m->x0 = theta0; m->minus_k = -m->k; m->half_k = 0.5 * m->k; return m;
 }
+// A harmonic dihedral, E = k/2 (phi - phi0)^2 with the difference taken in
+// (-pi, pi]: the reference defines the dihedral model class
+// (modelClass_dihedral.f90:28-45: compute(Ed, Fd, phi), factor14) but ships
+// no concrete model.  Laid out after bond_harmonic and angle_harmonic; no
+// scaled 1-4 pair term (factor14 = 0).
+void* EmDee_dihedral_harmonic(double k, double phi0) {
+  Model* m = make(DIHEDRAL_HARMONIC); m->k = k; m->x0 = phi0; m->minus_k = -m->k; m->half_k = 0.5 * m->k; return m;
+}
 void* EmDee_kspace_ewald(double accuracy) { Model* m = make(KSPACE_EWALD); m->accuracy = accuracy; return m; }
 
 }  // extern "C"

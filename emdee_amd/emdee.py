@@ -99,6 +99,7 @@ SIGNATURES = {
     "EmDee_coul_shifted_square_smoothed": (c_void_p, [c_double]),
     "EmDee_bond_harmonic": (c_void_p, [c_double, c_double]),
     "EmDee_angle_harmonic": (c_void_p, [c_double, c_double]),
+    "EmDee_dihedral_harmonic": (c_void_p, [c_double, c_double]),
     "EmDee_kspace_ewald": (c_void_p, [c_double]),
 }
 

@@ -436,6 +436,15 @@ interface
     type(c_ptr) :: EmDee_angle_harmonic
   end function EmDee_angle_harmonic
 
+  ! E = k/2 (phi - phi0)^2: no reference model file (modelClass_dihedral.f90:28-45 defines the class)
+  function EmDee_dihedral_harmonic( k, phi0 ) &
+    bind(C,name="EmDee_dihedral_harmonic")
+    import
+    real(c_double), value :: k
+    real(c_double), value :: phi0
+    type(c_ptr) :: EmDee_dihedral_harmonic
+  end function EmDee_dihedral_harmonic
+
   function EmDee_kspace_ewald( accuracy ) &
     bind(C,name="EmDee_kspace_ewald")
     import

@@ -179,6 +179,8 @@ void* EmDee_coul_square_smoothed( double skinWidth );                          /
 void* EmDee_coul_shifted_square_smoothed( double skinWidth );                  /* coul_shifted_square_smoothed.f90:35 */
 void* EmDee_bond_harmonic( double k, double r0 );                              /* bond_harmonic.f90:36-37 */
 void* EmDee_angle_harmonic( double k, double theta0 );                         /* angle_harmonic.f90:36-37 */
+/* E = k/2 (phi - phi0)^2: no reference model file (modelClass_dihedral.f90:28-45 defines the class) */
+void* EmDee_dihedral_harmonic( double k, double phi0 );
 void* EmDee_kspace_ewald( double accuracy );                                   /* kspace_ewald.f90:38 */
 
 #ifdef __cplusplus

@@ -592,6 +592,15 @@ void *EmDee_angle_harmonic(double k, double theta0) {
   omodel_t *m = new_model(K_ANGLE_HARMONIC);
   m->k = k; m->r0 = theta0; m->minus_k = -m->k; m->half_k = HALF * m->k; return m;
 }
+/* A harmonic dihedral, E = k/2 (phi - phi0)^2 with the difference taken in
+   (-pi, pi]; the reference defines the dihedral model class
+   (modelClass_dihedral.f90:28-45: compute(Ed, Fd, phi), factor14) but ships
+   no concrete model.  Named and laid out after bond_harmonic and
+   angle_harmonic; factor14 = 0 (no scaled 1-4 pair term). */
+void *EmDee_dihedral_harmonic(double k, double phi0) {
+  omodel_t *m = new_model(K_DIHEDRAL_HARMONIC);
+  m->k = k; m->r0 = phi0; m->minus_k = -m->k; m->half_k = HALF * m->k; m->factor14 = ZERO; return m;
+}
 void *EmDee_kspace_ewald(double accuracy) {                                                          /* kspace_ewald.f90:63-78 */
   omodel_t *m = new_model(K_KSPACE_EWALD); m->accuracy = accuracy; return m;
 }
@@ -1540,16 +1549,16 @@ void EmDee_add_angle(tEmDee md, int i, int j, int k, void *model) {
   EmDee_ignore_pair(md, i, k);
   EmDee_ignore_pair(md, j, k);
 }
-/* EmDeeCode.f90:628-655.  No concrete dihedral model exists in the
-   reference and compute_dihedrals is never called from the force driver;
-   only the exclusions take effect. */
+/* EmDeeCode.f90:628-655 */
 void EmDee_add_dihedral(tEmDee md, int i, int j, int k, int l, void *model) {
   odata_t *me = ME(md);
   int ijkl[4] = { i, j, k, l };
   if (!ranged(ijkl, 4, me->natoms)) oerror("add_dihedral", "atom index out of range");
   if (!model) oerror("add_dihedral", "a valid model must be provided");
   omodel_t *m = model;
-  if (m->kind != K_DIHEDRAL_NONE) oerror("add_dihedral", "the provided model must be a dihedral model");
+  if (m->kind != K_DIHEDRAL_NONE && m->kind != K_DIHEDRAL_HARMONIC)
+    oerror("add_dihedral", "the provided model must be a dihedral model");
+  struct_add(&me->dihedrals, &me->ndihedrals, &me->capdihedrals, i, j, k, l, m);
   EmDee_ignore_pair(md, i, j); EmDee_ignore_pair(md, i, k); EmDee_ignore_pair(md, i, l);
   EmDee_ignore_pair(md, j, k); EmDee_ignore_pair(md, j, l); EmDee_ignore_pair(md, k, l);
 }
@@ -1932,6 +1941,73 @@ static void compute_bonds(odata_t *me, int thread, const double *R, double *F, d
   }
 }
 
+static double dot3(const double *a, const double *b) { return a[0] * b[0] + a[1] * b[1] + a[2] * b[2]; }
+
+/* EmDeeData.f90:554-640, which the reference declares but never calls from
+   its force driver; here it is called after the angles, for layers with
+   bonded terms.  R is scaled (R/L).  One deliberate difference: atom j
+   receives -(Fi + Fk + Fl), so that the four forces sum to zero (the
+   reference line :604 adds +(Fi + Fk + Fl)).  The scaled 1-4 pair term of
+   :607-630 is skipped when factor14 = 0, the only value a model here has. */
+static void compute_dihedrals(odata_t *me, int thread, const double *R, double *F, double *Potential, double *Virial) {
+  if (!(me->ndihedrals > 0 && me->bonded[me->layer])) return;
+  double L = *me->Lbox;
+  int nd = (me->ndihedrals + me->nthreads - 1) / me->nthreads;
+  int mlast = me->ndihedrals < thread * nd ? me->ndihedrals : thread * nd;
+  for (int m = (thread - 1) * nd + 1; m <= mlast; m++) {
+    const ostruct_t *dh = &me->dihedrals[m - 1];
+    double rij[3], rkj[3], rlk[3], x[3], y[3], z[3], u[3], v[3], w[3], c[3], Fi[3], Fk[3], Fl[3];
+    for (int q = 0; q < 3; q++) {
+      double Rj = R[3 * (dh->j - 1) + q], Rk = R[3 * (dh->k - 1) + q];
+      rij[q] = R[3 * (dh->i - 1) + q] - Rj;
+      rkj[q] = Rk - Rj;
+      rlk[q] = R[3 * (dh->l - 1) + q] - Rk;
+      rij[q] = rij[q] - anint(rij[q]);
+      rkj[q] = rkj[q] - anint(rkj[q]);
+      rlk[q] = rlk[q] - anint(rlk[q]);
+    }
+    double normRkj = sqrt(dot3(rkj, rkj));
+    for (int q = 0; q < 3; q++) z[q] = rkj[q] / normRkj;
+    double rz = dot3(rij, z);
+    for (int q = 0; q < 3; q++) x[q] = rij[q] - rz * z[q];
+    double normX = sqrt(dot3(x, x));
+    for (int q = 0; q < 3; q++) x[q] = x[q] / normX;
+    cross3(z, x, y);
+    double a = dot3(x, rlk), b = dot3(y, rlk);
+    double phi = atan2(b, a), Ed = ZERO, Fd = ZERO;
+    if (dh->model.kind == K_DIHEDRAL_HARMONIC) {
+      double delta = phi - dh->model.r0;
+      delta = delta - TWO * PI_ * anint(delta / (TWO * PI_));
+      Ed = dh->model.half_k * delta * delta;
+      Fd = dh->model.minus_k * delta;
+    }
+    Fd = Fd / (L * (a * a + b * b));
+    cross3(rlk, z, c);
+    for (int q = 0; q < 3; q++) u[q] = (a * c[q] - b * rlk[q]) / normX;
+    cross3(rlk, x, c);
+    double zu = dot3(z, u);
+    for (int q = 0; q < 3; q++) v[q] = (a * c[q] + zu * rij[q]) / normRkj;
+    double zr = dot3(z, rij);
+    for (int q = 0; q < 3; q++) w[q] = v[q] + zr * u[q] / normRkj;
+    double uy = dot3(u, y), vx = dot3(v, x), wy = dot3(w, y);
+    for (int q = 0; q < 3; q++) {
+      Fi[q] = Fd * uy * y[q];
+      Fl[q] = Fd * (a * y[q] - b * x[q]);
+      Fk[q] = -(Fd * (vx * x[q] + wy * y[q]) + Fl[q]);
+    }
+    for (int q = 0; q < 3; q++) {
+      F[3 * (dh->i - 1) + q] += Fi[q];
+      F[3 * (dh->k - 1) + q] += Fk[q];
+      F[3 * (dh->l - 1) + q] += Fl[q];
+      F[3 * (dh->j - 1) + q] -= (Fi[q] + Fk[q] + Fl[q]);
+    }
+    *Potential += Ed;
+    *Virial += L * ((Fi[0] * rij[0] + Fk[0] * rkj[0] + Fl[0] * (rlk[0] + rkj[0])) +
+                    (Fi[1] * rij[1] + Fk[1] * rkj[1] + Fl[1] * (rlk[1] + rkj[1])) +
+                    (Fi[2] * rij[2] + Fk[2] * rkj[2] + Fl[2] * (rlk[2] + rkj[2])));
+  }
+}
+
 /* EmDeeData.f90:491-550 */
 static void compute_angles(odata_t *me, int thread, const double *R, double *F, double *Potential, double *Virial, double *Ecoul) {
   int bonded = me->bonded[me->layer];
@@ -2251,8 +2327,8 @@ static double rigid_body_virial(odata_t *me) {
 void EmDee_compute_forces(tEmDee *md) {
   odata_t *me = ME(*md);
   int N = me->natoms, T = me->nthreads;
-  double E[5] = { 0 }, W[5] = { 0 };
-  enum { PAIR_ = 0, COUL_ = 1, LONG_ = 2, BOND_ = 3, ANGLE_ = 4 };
+  double E[6] = { 0 }, W[6] = { 0 };
+  enum { PAIR_ = 0, COUL_ = 1, LONG_ = 2, BOND_ = 3, ANGLE_ = 4, DIHEDRAL_ = 5 };
   double *Rs = xcalloc(3 * (size_t)N, sizeof(double));
   double *Fs = xcalloc(3 * (size_t)N * T, sizeof(double));
   double L = *me->Lbox;
@@ -2260,17 +2336,18 @@ void EmDee_compute_forces(tEmDee *md) {
   handle_neighbor_lists(me, &md->Builds, &md->Time.Neighbor, Rs);
   md->Time.Pair -= wtime();
   int compute = md->Options.Compute;
-  double Et[5 * 64] = { 0 }, Wt[5 * 64] = { 0 };
+  double Et[6 * 64] = { 0 }, Wt[6 * 64] = { 0 };
   if (T > 64) oerror("oracle", "at most 64 threads");
 #pragma omp parallel num_threads(T)
   {
     int t = thread_num() + 1;
-    double *F = Fs + 3 * (size_t)N * (t - 1), *e = Et + 5 * (t - 1), *w = Wt + 5 * (t - 1);
+    double *F = Fs + 3 * (size_t)N * (t - 1), *e = Et + 6 * (t - 1), *w = Wt + 6 * (t - 1);
     compute_pairs(me, t, compute, Rs, F, &e[PAIR_], &e[COUL_], &w[PAIR_], &w[COUL_]);
     compute_bonds(me, t, Rs, F, &e[BOND_], &w[BOND_], &e[COUL_]);
     compute_angles(me, t, Rs, F, &e[ANGLE_], &w[ANGLE_], &e[COUL_]);
+    compute_dihedrals(me, t, Rs, F, &e[DIHEDRAL_], &w[DIHEDRAL_]);
   }
-  for (int t = 0; t < T; t++) for (int k = 0; k < 5; k++) { E[k] += Et[5 * t + k]; W[k] += Wt[5 * t + k]; }
+  for (int t = 0; t < T; t++) for (int k = 0; k < 6; k++) { E[k] += Et[6 * t + k]; W[k] += Wt[6 * t + k]; }
   for (int k = 0; k < 3 * N; k++) {
     double s = 0;
     for (int t = 0; t < T; t++) s += Fs[3 * (size_t)N * t + k];
@@ -2280,7 +2357,7 @@ void EmDee_compute_forces(tEmDee *md) {
     compute_kspace(me, Rs, &E[LONG_], me->F);
     W[LONG_] = E[COUL_] + E[LONG_] - W[COUL_];
   }
-  md->Virial = W[0] + W[1] + W[2] + W[3] + W[4];
+  md->Virial = W[0] + W[1] + W[2] + W[3] + W[4] + W[5];
   if (me->nbodies != 0) {
     md->BodyVirial = rigid_body_virial(me);
     md->Virial = md->Virial + md->BodyVirial;
@@ -2290,7 +2367,8 @@ void EmDee_compute_forces(tEmDee *md) {
     md->Energy.Coulomb = E[COUL_] + E[LONG_];
     md->Energy.Bond = E[BOND_];
     md->Energy.Angle = E[ANGLE_];
-    md->Energy.Potential = E[0] + E[1] + E[2] + E[3] + E[4];
+    md->Energy.Dihedral = E[DIHEDRAL_];
+    md->Energy.Potential = E[0] + E[1] + E[2] + E[3] + E[4] + E[5];
     md->Energy.ShadowPotential = md->Energy.Potential;
   }
   md->Energy.UpToDate = compute;

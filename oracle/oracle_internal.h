@@ -22,7 +22,7 @@ enum {
   K_COUL_DAMPED_SMOOTHED, K_COUL_DAMPED_SQUARE_SMOOTHED,
   K_COUL_SQUARE_SMOOTHED, K_COUL_SHIFTED_SQUARE_SMOOTHED,
   K_BOND_NONE, K_BOND_HARMONIC, K_ANGLE_NONE, K_ANGLE_HARMONIC,
-  K_DIHEDRAL_NONE, K_KSPACE_EWALD
+  K_DIHEDRAL_NONE, K_DIHEDRAL_HARMONIC, K_KSPACE_EWALD
 };
 
 /* Modifier codes: modelClass_nonbonded.f90:27-33 */
@@ -44,8 +44,10 @@ typedef struct {
   /* cCoulModel */
   int shifted, shifted_force, requires_kspace;
   double alpha, beta, damp, skinWidth, Rm2, invRm;
-  /* bond/angle harmonic */
+  /* bond/angle/dihedral harmonic (r0: equilibrium distance or angle) */
   double k, r0, minus_k, half_k;
+  /* cDihedralModel (modelClass_dihedral.f90:28-31) */
+  double factor14;
   /* kspace_ewald */
   double accuracy;
 } omodel_t;
@@ -126,6 +128,7 @@ typedef struct {
   int *exCount, *exCap, **exItem;
   ostruct_t *bonds; int nbonds, capbonds;
   ostruct_t *angles; int nangles, capangles;
+  ostruct_t *dihedrals; int ndihedrals, capdihedrals;
   int *atomType, *atomCell, *atomBody, *free_, *atomsInCell;
   double *R, *P;             /* R(3,N) */
   obody_t *body;

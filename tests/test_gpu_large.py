@@ -1,0 +1,70 @@
+"""Large Lennard-Jones systems against the oracle on identical inputs.
+
+* 1,048,576 atoms (fcc 64^3 cells at density 0.8442, the large-LJ workload's
+  size), plain lj_cut, in both arithmetic forms: forces, energy, virial
+  before and after ten steps with list rebuilds, kinetic energy and rebuild
+  count.  The oracle runs with all host cores (up to 16 threads).
+* 131,072 atoms with shifted-force LJ, which no dedicated LJ kernel serves:
+  the pair loop takes the generic kernels with one thread per atom (short
+  lists at this size), in both forms.
+"""
+import os
+
+import numpy as np
+import pytest
+
+from test_gpu_mixed import FORCE_RMS, errors, fcc_box
+
+pytestmark = pytest.mark.gpu
+
+THREADS = min(16, os.cpu_count() or 1)
+
+
+def make(lib, R, L, model, gpu_mixed, threads=1):
+    N = len(R)
+    md = lib.system(threads, 1, 2.5, 0.3, N, None, None, None)
+    lib.EmDee_set_pair_model(md, 1, 1, model(lib), 0.0)
+    if gpu_mixed:
+        lib.EmDeeGPU_set_precision(md, 1)
+    lib.upload(md, "box", np.array([L]))
+    lib.upload(md, "coordinates", R)
+    lib.random_momenta(md, 1.0, True, 5)
+    return md
+
+
+def compare(gpu, oracle, nc, model, mixed, steps, label, jitter):
+    R, L = fcc_box(nc, 0.8442, jitter, seed=23)
+    N = len(R)
+    mg = make(gpu, R, L, model, mixed)
+    mo = make(oracle, R, L, model, False, THREADS)
+    tol = FORCE_RMS if mixed else 1e-9
+    for stage in range(2):
+        if stage:
+            for _ in range(steps):
+                for lib, md in ((gpu, mg), (oracle, mo)):
+                    lib.boost(md, 1.0, 0.0, 0.0025)
+                    lib.displace(md, 1.0, 0.0, 0.005)
+                    lib.boost(md, 1.0, 0.0, 0.0025)
+        gpu.compute_forces(mg)
+        oracle.compute_forces(mo)
+        ferr, eerr, werr = errors(gpu, oracle, mg, mo, N)
+        dW = abs(mg.Virial - mo.Virial) / N
+        kerr = abs(mg.Kinetic.Total - mo.Kinetic.Total) / mo.Kinetic.Total
+        print(f"{label} N={N} mixed={mixed} stage {stage}: force rms {ferr:.2e} energy {eerr:.2e} "
+              f"virial per atom {dW:.1e} kinetic {kerr:.1e} builds {mg.Builds}/{mo.Builds}", flush=True)
+        assert ferr < tol and eerr < 1e-6 and dW < 2e-5 and kerr < 1e-6
+        assert mg.Builds == mo.Builds
+    gpu.EmDeeGPU_destroy(mg)
+
+
+@pytest.mark.parametrize("mixed", [False, True])
+def test_million_atom_lj(gpu, oracle, mixed):
+    # a Gaussian jitter of 0.08 sigma would bring some of the 6.3 million
+    # nearest-neighbour pairs within 0.6 sigma (forces ~2e4) at this size
+    compare(gpu, oracle, 64, lambda lib: lib.EmDee_pair_lj_cut(1.0, 1.0), mixed, 10, "lj_cut", 0.02)
+
+
+@pytest.mark.parametrize("mixed", [False, True])
+def test_generic_one_thread_kernels(gpu, oracle, mixed):
+    compare(gpu, oracle, 32, lambda lib: lib.EmDee_shifted_force(lib.EmDee_pair_lj_cut(1.0, 1.0)), mixed, 20,
+            "lj_sf", 0.08)
