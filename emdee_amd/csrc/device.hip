@@ -367,13 +367,20 @@ struct MixedCut {
   double L;
 };
 
+// P16 (optional): the top 16 bits of each fixed-point coordinate, x and y
+// packed in one word, z with bit 31 set when the position is not finite.
 __global__ void k_scale4q(double4* __restrict__ Rs4, int4* __restrict__ Q4, float* __restrict__ qf,
                           const double* __restrict__ R, const int* __restrict__ type, const double* __restrict__ charge,
-                          const int* __restrict__ charged, double L, int N, int Npad) {
+                          const int* __restrict__ charged, double L, int N, int Npad, uint2* __restrict__ P16) {
   int s = blockIdx.x * blockDim.x + threadIdx.x;
   if (s >= N) return;
   Scaled sc{ Rs4, Q4, qf, type, charge, charged, L };
   scale_slot(sc, R, s, Npad);
+  if (P16) {
+    int4 q = Q4[s];
+    P16[s] = make_uint2(((unsigned int)q.x >> 16) | ((unsigned int)q.y & 0xffff0000u),
+                        ((unsigned int)q.z >> 16) | ((q.w & Q4_NONFINITE) ? 0x80000000u : 0u));
+  }
 }
 
 // relative half-width of the FP32 band around a cutoff Rc2s (scaled units^2)
@@ -630,6 +637,231 @@ k_build_list(int* __restrict__ nbr, int* __restrict__ nnb, int* __restrict__ fla
   double v[1] = { sub == 0 ? (double)cnt : 0.0 };
   block_sum<1>(v);
   if (threadIdx.x == 0) part[blk] = v[0];
+}
+
+// The same pair set for free atoms with every type pair interacting
+// (SIMPLE), one lane per atom, built for latency rather than by rows:
+// * A wave that walks the 7x7 rows one by one pays, in every row, for its
+//   lane with the most candidates there.  Here each lane loads the slot
+//   bounds of the seven rows of a z plane at once, keeps the non-empty
+//   segments in its own slice of LDS and streams its candidates four at a
+//   time, so a wave pays for its busiest lane once per plane.
+// * Prefilter: the 16-bit fixed-point coordinates of P16 give the squared
+//   distance r2 (units 2^-32 box^2) in four instructions, wrapped
+//   differences being the minimum image.  With e = 2^-16 + 2^-32 the largest
+//   error of a coordinate difference, |r2 - r^2| <= 2 sqrt(3) e r + 3 e^2, so
+//   r2 < lo (r2 >= hi) decides "inside" ("outside") for every candidate whose
+//   exact r^2 is not within the relative band b of the cutoff.  Cells have
+//   edge >= 1/12 of the box (M >= 12), so |differences| <= 4 cells < 2^15
+//   and r2 < 2^31.
+// * The candidates in the band (about 0.5 per atom) are queued in four
+//   registers and decided after the stream by the FP64 test on the scaled positions, the
+//   test of k_build_list, so the pair set is the same; they are appended
+//   after the others.  The stream itself then loads only P16 and keeps two
+//   groups' loads in flight, which its stores do not hold up (a wait for a
+//   load also waits for every store issued before it).
+constexpr int SEG_MAX = 2 * (2 * CELL_REACH + 1) + 1;   // segments of one plane (the own row split)
+constexpr int BANDQ = 4;                             // queued band candidates per lane
+typedef short short2v __attribute__((ext_vector_type(2)));
+__global__ void __launch_bounds__(BLOCK)
+k_build_list_seg(int* __restrict__ nbr, int* __restrict__ nnb, int* __restrict__ flags, double* __restrict__ part,
+                 const double4* __restrict__ Rs4, const int* __restrict__ key, const int* __restrict__ start, int M,
+                 double xRc2, int cap, int N, int Npad, const uint2* __restrict__ P16, int lo, int hi) {
+  __shared__ int2 seg[SEG_MAX * BLOCK];
+  int blk = xcd_block();
+  int i = blk * BLOCK + threadIdx.x;
+  int cnt = 0;
+  if (i < N) {
+    const double4 ri = Rs4[i];
+    const int c = key[i];
+    const int cx = c % M, cy = (c / M) % M, cz = c / (M * M);
+    const uint2 pi = P16[i];
+    const bool oddi = (int)pi.y < 0;
+    auto frac = [&](double v, int cc) {
+      double f = M * (v - floor(v)) - cc;
+      if (f > 1.5) f -= M;                  // the coordinate wrapped to cell 0
+      return fmin(fmax(f, 0.0), 1.0);
+    };
+    // conservative culling in FP32 with margins, as in k_build_list
+    const float fx = (float)frac(ri.x, cx), fy = (float)frac(ri.y, cy), fz = (float)frac(ri.z, cz);
+    const float rcell2 = (float)(xRc2 * (double)M * (double)M * (1.0 + 1e-5) + 1e-5);
+    // gap, in cells, between the atom and the cells at offset d
+    auto gap = [](int d, float f) { return fmaxf(fmaxf((float)d - f, f - (float)d - 1.0f), 0.0f); };
+    constexpr int W = 2 * CELL_REACH + 1;
+    float gy2[W];
+    int yrow[W];
+#pragma unroll
+    for (int t = 0; t < W; t++) {
+      float g = gap(t - CELL_REACH, fy);
+      gy2[t] = g * g;
+      int y = cy + t - CELL_REACH;
+      yrow[t] = M * (y < 0 ? y + M : (y >= M ? y - M : y));
+    }
+    int2* __restrict__ my = seg + threadIdx.x;
+    // queued band candidates, in four named registers (an array would be
+    // indexed by nq and kept in scratch memory)
+    int bq0 = 0, bq1 = 0, bq2 = 0, bq3 = 0, nq = 0;
+    unsigned int off = (unsigned int)i;     // element offset of entry cnt of slot i
+    auto append = [&](int j) __attribute__((always_inline)) {
+      if (cnt < cap) nbr[off] = j;
+      off += (unsigned int)Npad;
+      cnt++;
+    };
+    auto exact = [&](int j) __attribute__((always_inline)) {
+      double4 rj = Rs4[j];
+      double r2;
+      {
+#pragma clang fp contract(off)
+        double d0 = ri.x - rj.x, d1 = ri.y - rj.y, d2 = ri.z - rj.z;
+        d0 = d0 - rint(d0); d1 = d1 - rint(d1); d2 = d2 - rint(d2);
+        r2 = d0 * d0 + d1 * d1 + d2 * d2;
+      }
+      return r2 < xRc2;
+    };
+    for (int dz = -CELL_REACH; dz <= CELL_REACH; dz++) {
+      float gz = gap(dz, fz);
+      float remz = rcell2 - gz * gz;
+      // slot bounds of the plane's rows, all loaded before any is used:
+      // row t gives cells c0..e1 and, where it wraps, 0..d1 after them (the
+      // order of k_build_list); a culled row gives empty ranges.  xa >= -3
+      // and xa <= cx < M; xb >= cx >= 0 and xb <= cx + 3.
+      int z = cz + dz; z = z < 0 ? z + M : (z >= M ? z - M : z);
+      const unsigned int plane = (unsigned int)(M * M * z);
+      int sa[SEG_MAX], sb[SEG_MAX];
+#pragma unroll
+      for (int t = 0; t < W; t++) {
+        float rem = remz - gy2[t];
+        bool on = rem > 0.0f;
+        float e = __builtin_amdgcn_sqrtf(fmaxf(rem, 0.0f)) + 1e-4f;
+        int xa = cx + max((int)floorf(fx - e), -CELL_REACH), xb = cx + min((int)floorf(fx + e), CELL_REACH);
+        bool wlo = xa < 0, two = on && (wlo || xb >= M);
+        unsigned int row = plane + (unsigned int)yrow[t];
+        unsigned int i0 = row + (unsigned int)(on ? (wlo ? xa + M : xa) : 0);
+        unsigned int i1 = row + (unsigned int)(on ? (wlo ? M : min(xb + 1, M)) : 0);
+        unsigned int i3 = row + (unsigned int)(two ? (wlo ? xb + 1 : xb - M + 1) : 0);
+        // slots of the row, in order
+        int s0 = start[i0], s1 = start[i1], w0 = start[row], w1 = start[i3];
+        if (t < CELL_REACH) {
+          sa[2 * t] = s0; sb[2 * t] = s1; sa[2 * t + 1] = w0; sb[2 * t + 1] = w1;
+        } else if (t > CELL_REACH) {
+          sa[2 * t + 1] = s0; sb[2 * t + 1] = s1; sa[2 * t + 2] = w0; sb[2 * t + 2] = w1;
+        } else {
+          // the atom's own row, split at the atom: its cell is in the wrapped
+          // part when the row wraps below 0, else in the first.  A slot
+          // outside [p0, p1) leaves [p0, min(i, p1)) whole and the rest empty,
+          // or the reverse, so the planes dz != 0 are split harmlessly.
+          int p0 = wlo ? w0 : s0, p1 = wlo ? w1 : s1;
+          int h0 = p0, h1 = min(i, p1), g0 = max(i + 1, p0), g1 = p1;
+          sa[2 * t] = wlo ? s0 : h0;     sb[2 * t] = wlo ? s1 : h1;
+          sa[2 * t + 1] = wlo ? h0 : g0; sb[2 * t + 1] = wlo ? h1 : g1;
+          sa[2 * t + 2] = wlo ? g0 : w0; sb[2 * t + 2] = wlo ? g1 : w1;
+        }
+      }
+      // the non-empty ones, in order, into the lane's LDS slice, so that the
+      // stream below steps to the next segment with a single test
+      int nseg = 0;
+#pragma unroll
+      for (int t = 0; t < SEG_MAX; t++) {
+        my[BLOCK * nseg] = make_int2(sa[t], sb[t]);
+        nseg += sa[t] < sb[t] ? 1 : 0;
+      }
+      if (oddi) nseg = 0;                   // a non-finite atom has no neighbours
+      // stream the plane's segments (a lane-private LDS slice: no barrier);
+      // group g is tested after the loads of groups g+1 and g+2 are issued
+      int k = 0, a = 0, b = 0;
+      if (nseg > 0) { int2 t = my[0]; a = t.x; b = t.y; }
+      auto advance = [&]() __attribute__((always_inline)) {
+        a += 4;
+        if (a >= b && ++k < nseg) { int2 t = my[BLOCK * k]; a = t.x; b = t.y; }
+      };
+      // loads of the group at the current position; P16 has 16 readable
+      // slots past the last atom, so a group (or the three past a plane's
+      // end) never reads out of bounds
+      auto fetch = [&](uint2 (&q)[4], int& qa, int& qb) __attribute__((always_inline)) {
+        const bool more = k < nseg;
+        qa = more ? a : 0;
+        qb = more ? b : 0;
+        const uint2* __restrict__ p = P16 + (unsigned int)a;
+#pragma unroll
+        for (int u = 0; u < 4; u++) q[u] = p[u];
+        advance();
+      };
+      auto test = [&](const uint2 (&q)[4], int qa, int qb) __attribute__((always_inline)) {
+        bool in[4], chk[4];
+#pragma unroll
+        for (int u = 0; u < 4; u++) {
+          short2v dxy = __builtin_bit_cast(short2v, pi.x) - __builtin_bit_cast(short2v, q[u].x);
+          int dzs = (int)(short)(pi.y - q[u].y);
+          int r2 = __builtin_amdgcn_sdot2(dxy, dxy, dzs * dzs, true);
+          bool odd = (int)q[u].y < 0;
+          bool ok = qa + u < qb;
+          in[u] = ok && r2 < lo && !odd;
+          chk[u] = ok && !in[u] && (r2 < hi || odd);
+        }
+        // one append site: increments of the count and of the queue length
+        // on exclusive paths would otherwise be merged into a store through
+        // a pointer, which keeps both in scratch memory
+#pragma unroll
+        for (int u = 0; u < 4; u++) {
+          const int j = qa + u;
+          bool take = in[u];
+          if (chk[u]) {
+            if (nq < BANDQ) {
+              bq0 = nq == 0 ? j : bq0;
+              bq1 = nq == 1 ? j : bq1;
+              bq2 = nq == 2 ? j : bq2;
+              bq3 = nq == 3 ? j : bq3;
+            } else {
+              take = exact(j);                // queue full (rare): decided now
+            }
+            nq = min(nq + 1, BANDQ + 1);
+          }
+          if (take) append(j);
+        }
+      };
+      uint2 q0[4], q1[4], q2[4];
+      int a0, b0, a1, b1, a2, b2;
+      fetch(q0, a0, b0);
+      fetch(q1, a1, b1);
+      // three register sets in turn: no copy waits for a load
+      while (a0 < b0) {
+        fetch(q2, a2, b2);
+        test(q0, a0, b0);
+        if (a1 >= b1) break;
+        fetch(q0, a0, b0);
+        test(q1, a1, b1);
+        if (a2 >= b2) break;
+        fetch(q1, a1, b1);
+        test(q2, a2, b2);
+      }
+    }
+    // the band candidates, by the FP64 test
+    bool t0 = nq > 0 && exact(bq0), t1 = nq > 1 && exact(bq1), t2 = nq > 2 && exact(bq2), t3 = nq > 3 && exact(bq3);
+    if (t0) append(bq0);
+    if (t1) append(bq1);
+    if (t2) append(bq2);
+    if (t3) append(bq3);
+    nnb[i] = cnt;
+    // pad the list to a multiple of 4 with the atom itself (zero separation):
+    // the mixed-form kernel reads whole groups of four without a bound test
+    if (cnt <= cap)
+      for (int k = cnt; k < ((cnt + 3) & ~3); k++) nbr[(size_t)k * Npad + i] = i;
+    if (cnt > cap) atomicMax(&flags[0], cnt);
+  }
+  double v[1] = { (double)cnt };
+  block_sum<1>(v);
+  if (threadIdx.x == 0) part[blk] = v[0];
+}
+
+// integer band of k_build_list_seg for the squared list cutoff xRc2 (scaled
+// units^2): {lo, hi} in units of 2^-32 box^2
+static void band16(double xRc2, int* lo, int* hi) {
+  const double e = 1.0 / 65536.0 + 1.0 / 4294967296.0;
+  double r = std::sqrt(xRc2);
+  double b = 2.0 * std::sqrt(3.0) * e / r + 3.0 * e * e / xRc2 + 1e-6;
+  double c = xRc2 * 4294967296.0;
+  *lo = (int)std::floor(c * (1.0 - b));
+  *hi = (int)std::ceil(c * (1.0 + b));
 }
 
 // ---- pair forces ----------------------------------------------------------
@@ -1697,8 +1929,9 @@ void dev_rebuild(Device* d, double L, double xRc, double xInRc) {
   // R, type and charge)
   d->Q4.alloc(Np);
   d->qf.alloc(Np);
+  d->P16.alloc(Np + 16);               // the segment list build reads up to 16 slots past N
   hipLaunchKernelGGL(k_scale4q, dim3(g), dim3(BLOCK), 0, d->stream, d->Rs4.p, d->Q4.p, d->qf.p, d->R.p, d->type.p,
-                     d->charge.p, d->chargedFlag.p, L, N, Np);
+                     d->charge.p, d->chargedFlag.p, L, N, Np, d->P16.p);
   hipLaunchKernelGGL(k_cell_key, dim3(g), dim3(BLOCK), 0, d->stream, d->key.p, d->cellCount.p, d->Rs4.p, M, N);
   // R0 = R
   HIP_CHECK(hipMemcpyAsync(d->R0.p, d->R.p, 3 * sizeof(double) * Np, hipMemcpyDeviceToDevice, d->stream));
@@ -1721,16 +1954,26 @@ void dev_rebuild(Device* d, double L, double xRc, double xInRc) {
     double cand = 2.6 * vol;
     long n = N;
     int S = (cand >= 16 * 4 && n * 16 <= (1L << 21)) ? 16 : ((cand >= 4 * 4 && n * 4 <= (1L << 21)) ? 4 : 1);
-    auto kern = simple ? (S == 16 ? k_build_list<true, 16> : S == 4 ? k_build_list<true, 4> : k_build_list<true, 1>)
-                       : (S == 16 ? k_build_list<false, 16> : S == 4 ? k_build_list<false, 4> : k_build_list<false, 1>);
     int gl = div_up((long)N * S, BLOCK);
     hipEvent_t e0 = d->profiling ? d->ev0 : nullptr, e1 = d->profiling ? d->ev1 : nullptr;
-    hipExtLaunchKernelGGL(kern, dim3(gl), dim3(BLOCK), 0, d->stream, e0, e1, 0, d->nbr.p, d->nnb.p, d->flags.p,
-                          d->partials.p, (const double4*)d->Rs4.p, (const int*)d->key.p, (const int*)d->cellStart.p,
-                          (const int*)d->body.p, (const int*)d->perm.p,
-                          d->allInteract ? nullptr : (const uint8_t*)d->interact.p,
-                          (const int*)d->exOff.p, (const int*)d->exItem.p, d->nexcl, d->ntypes, M, xRc2, d->cap, N, Np,
-                          (const int4*)d->Q4.p, lo, hi);
+    // the segment-streaming form addresses list entries with 32-bit element
+    // offsets (slot + k * Npad, k <= cap) and needs cells of edge <= 1/12 box
+    if (simple && S == 1 && M >= 12 && (double)(d->cap + 1) * Np < 2147483648.0) {
+      int lo16, hi16;
+      band16(xRc2, &lo16, &hi16);
+      hipExtLaunchKernelGGL(k_build_list_seg, dim3(gl), dim3(BLOCK), 0, d->stream, e0, e1, 0, d->nbr.p, d->nnb.p,
+                            d->flags.p, d->partials.p, (const double4*)d->Rs4.p, (const int*)d->key.p,
+                            (const int*)d->cellStart.p, M, xRc2, d->cap, N, Np, (const uint2*)d->P16.p, lo16, hi16);
+    } else {
+      auto kern = simple ? (S == 16 ? k_build_list<true, 16> : S == 4 ? k_build_list<true, 4> : k_build_list<true, 1>)
+                         : (S == 16 ? k_build_list<false, 16> : S == 4 ? k_build_list<false, 4> : k_build_list<false, 1>);
+      hipExtLaunchKernelGGL(kern, dim3(gl), dim3(BLOCK), 0, d->stream, e0, e1, 0, d->nbr.p, d->nnb.p, d->flags.p,
+                            d->partials.p, (const double4*)d->Rs4.p, (const int*)d->key.p, (const int*)d->cellStart.p,
+                            (const int*)d->body.p, (const int*)d->perm.p,
+                            d->allInteract ? nullptr : (const uint8_t*)d->interact.p,
+                            (const int*)d->exOff.p, (const int*)d->exItem.p, d->nexcl, d->ntypes, M, xRc2, d->cap, N, Np,
+                            (const int4*)d->Q4.p, lo, hi);
+    }
     launch_check("build_list");
     hipLaunchKernelGGL(k_reduce_partials<1>, dim3(1), dim3(BLOCK), 0, d->stream, d->result.p + 8, d->partials.p, gl);
     HIP_CHECK(hipMemcpyAsync(d->hresult + 8, d->result.p + 8, sizeof(double), hipMemcpyDeviceToHost, d->stream));
@@ -1780,7 +2023,7 @@ void dev_compute_pairs(Device* d, const PairCall& c, ForceSums* out) {
     d->Q4.alloc(Np);
     d->qf.alloc(Np);
     hipLaunchKernelGGL(k_scale4q, dim3(g), dim3(BLOCK), 0, d->stream, d->Rs4.p, d->Q4.p, d->qf.p, d->R.p, d->type.p,
-                       d->charge.p, d->chargedFlag.p, c.L, N, Np);
+                       d->charge.p, d->chargedFlag.p, c.L, N, Np, (uint2*)nullptr);
   } else {
     hipLaunchKernelGGL(k_scale4, dim3(g), dim3(BLOCK), 0, d->stream, d->Rs4.p, d->R.p, d->type.p, c.L, N, Np);
   }

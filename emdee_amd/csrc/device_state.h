@@ -146,6 +146,7 @@ struct Device {
   DBuf<double> R, P, R0, invMass, charge, F;   // F holds nlayers * 3 * Npad
   DBuf<double4> Rs4;      // {x/L, y/L, z/L, type}: one 32-byte gather per neighbor
   DBuf<int4> Q4;          // mixed form: 32-bit fixed-point scaled coordinates and type
+  DBuf<uint2> P16;        // list build: 16-bit fixed-point scaled coordinates {x | y << 16, z | nonfinite << 31}
   DBuf<float> qf;         // mixed form: FP32 charge (0 when uncharged)
   DBuf<int> type, body, perm, freeFlag, chargedFlag;
   DBuf<double> tmp3, tmpF, tmp1;   // permutation scratch (swapped with the arrays)

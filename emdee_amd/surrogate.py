@@ -312,15 +312,42 @@ def water_hmc_workload(lib, nmol=4096, kcut=1.2, samples=60, stride=5, epochs=30
     return out
 
 
+def evaluation_costs(lib, nmol=4096, kcut=1.2, evals=20, seed=1):
+    """Force-evaluation costs on the two-layer water box, for kernel traces:
+    `evals` evaluations each of the exact layer 1, of the real-space layer 2,
+    and of layer 2 with a model attached (long-range channel plus network; the
+    network's weights are zero, which changes no kernel's work).  Returns the
+    wall milliseconds per evaluation of each."""
+    md, cfg = two_layer_water(lib, nmol, seed)
+    out = {"molecules": nmol, "atoms": cfg["N"], "kcut": kcut,
+           "exact_eval_ms": eval_ms(lib, md, 1, evals), "short_eval_ms": eval_ms(lib, md, 2, evals)}
+    m = zero_network(lib, kcut)
+    m.attach(md, 2)
+    out["model_eval_ms"] = eval_ms(lib, md, 2, evals)
+    out["long_range_vectors"] = m.long_range_vectors()
+    m.detach(md)
+    m.close()
+    lib.EmDeeGPU_destroy(ctypes.byref(md))
+    return out
+
+
 if __name__ == "__main__":
     # Workload: python -m emdee_amd.surrogate [molecules] [kcut]
+    # Evaluation costs only: python -m emdee_amd.surrogate --evals [molecules] [kcut] [evaluations]
     import json
     import os
     import sys
 
     from . import load
-    nmol = int(sys.argv[1]) if len(sys.argv) > 1 else 4096
-    kc = float(sys.argv[2]) if len(sys.argv) > 2 else 1.2
+    args = sys.argv[1:]
+    costs = bool(args) and args[0] == "--evals"
+    if costs:
+        args = args[1:]
+    nmol = int(args[0]) if len(args) > 0 else 4096
+    kc = float(args[1]) if len(args) > 1 else 1.2
     os.environ.setdefault("EMDEE_QUIET", "1")
-    res = water_hmc_workload(load(), nmol, kc, log=lambda *a: print(*a, flush=True))
-    print(json.dumps(res))
+    if costs:
+        res = evaluation_costs(load(), nmol, kc, int(args[2]) if len(args) > 2 else 20)
+    else:
+        res = water_hmc_workload(load(), nmol, kc, log=lambda *a: print(*a, flush=True))
+    print(json.dumps(res), flush=True)

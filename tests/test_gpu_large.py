@@ -3,11 +3,14 @@
 * 1,048,576 atoms (fcc 64^3 cells at density 0.8442, the large-LJ workload's
   size), plain lj_cut, in both arithmetic forms: forces, energy, virial
   before and after ten steps with list rebuilds, kinetic energy and rebuild
-  count.  The oracle runs with all host cores (up to 16 threads).
+  count, and the Verlet-list pair set entry for entry (systems this large
+  take the one-lane list build).  The oracle runs with all host cores (up to
+  16 threads).
 * 131,072 atoms with shifted-force LJ, which no dedicated LJ kernel serves:
   the pair loop takes the generic kernels with one thread per atom (short
   lists at this size), in both forms.
 """
+import ctypes
 import os
 
 import numpy as np
@@ -32,7 +35,23 @@ def make(lib, R, L, model, gpu_mixed, threads=1):
     return md
 
 
-def compare(gpu, oracle, nc, model, mixed, steps, label, jitter):
+def pair_keys(pairs, N):
+    """Sorted int64 keys (i - 1) * N + (j - 1) of 1-based pairs i < j."""
+    p = np.asarray(pairs, dtype=np.int64)
+    return np.sort((p[:, 0] - 1) * N + (p[:, 1] - 1))
+
+
+def oracle_pair_array(oracle, md):
+    f = oracle.lib.oracle_neighbor_pairs
+    f.restype = ctypes.c_int
+    f.argtypes = [type(md), ctypes.c_int, ctypes.POINTER(ctypes.c_int)]
+    n = f(md, 0, None)
+    out = np.zeros((max(n, 1), 2), dtype=np.int32)
+    f(md, n, out.ctypes.data_as(ctypes.POINTER(ctypes.c_int)))
+    return out[:n]
+
+
+def compare(gpu, oracle, nc, model, mixed, steps, label, jitter, pairs=False):
     R, L = fcc_box(nc, 0.8442, jitter, seed=23)
     N = len(R)
     mg = make(gpu, R, L, model, mixed)
@@ -54,6 +73,10 @@ def compare(gpu, oracle, nc, model, mixed, steps, label, jitter):
               f"virial per atom {dW:.1e} kinetic {kerr:.1e} builds {mg.Builds}/{mo.Builds}", flush=True)
         assert ferr < tol and eerr < 1e-6 and dW < 2e-5 and kerr < 1e-6
         assert mg.Builds == mo.Builds
+        if pairs:
+            kg, ko = pair_keys(gpu.neighbor_pairs(mg), N), pair_keys(oracle_pair_array(oracle, mo), N)
+            print(f"{label} stage {stage}: {len(kg)} pairs on the GPU, {len(ko)} in the oracle", flush=True)
+            assert len(kg) == len(ko) and np.array_equal(kg, ko)
     gpu.EmDeeGPU_destroy(mg)
 
 
@@ -61,7 +84,7 @@ def compare(gpu, oracle, nc, model, mixed, steps, label, jitter):
 def test_million_atom_lj(gpu, oracle, mixed):
     # a Gaussian jitter of 0.08 sigma would bring some of the 6.3 million
     # nearest-neighbour pairs within 0.6 sigma (forces ~2e4) at this size
-    compare(gpu, oracle, 64, lambda lib: lib.EmDee_pair_lj_cut(1.0, 1.0), mixed, 10, "lj_cut", 0.02)
+    compare(gpu, oracle, 64, lambda lib: lib.EmDee_pair_lj_cut(1.0, 1.0), mixed, 10, "lj_cut", 0.02, pairs=True)
 
 
 @pytest.mark.parametrize("mixed", [False, True])
