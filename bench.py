@@ -306,7 +306,7 @@ def main():
     ap.add_argument("--cpu-steps", type=int, default=20)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--hmc-replicas", type=int, default=512)
-    ap.add_argument("--hmc-cycles", type=int, default=60)
+    ap.add_argument("--hmc-cycles", type=int, default=400)
     ap.add_argument("--no-hmc", action="store_true")
     ap.add_argument("--no-water", action="store_true")
     ap.add_argument("--no-water-hmc", action="store_true")

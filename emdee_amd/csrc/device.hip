@@ -63,11 +63,6 @@ __global__ void k_iota(int* p, int n) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) p[i] = i;
 }
-// iperm[perm[s]] = s
-__global__ void k_invert_perm(int* __restrict__ iperm, const int* __restrict__ perm, int n) {
-  int s = blockIdx.x * blockDim.x + threadIdx.x;
-  if (s < n) iperm[perm[s]] = s;
-}
 
 // AoS (original order) -> SoA (sorted order)
 __global__ void k_gather_aos(double* __restrict__ dst, const double* __restrict__ aos,
@@ -137,8 +132,8 @@ struct Scaled {
   const int* charged;
   double L;
 };
-__device__ __forceinline__ void scale_slot(const Scaled& sc, const double* __restrict__ R, int s, int Npad) {
-  double v[3] = { R[s] / sc.L, R[Npad + s] / sc.L, R[2 * Npad + s] / sc.L };
+__device__ __forceinline__ void scale_vals(const Scaled& sc, double x, double y, double z, int s) {
+  double v[3] = { x / sc.L, y / sc.L, z / sc.L };
   int t = sc.type[s];
   if (sc.Rs4) sc.Rs4[s] = make_double4(v[0], v[1], v[2], (double)t);
   if (sc.Q4) {
@@ -151,6 +146,9 @@ __device__ __forceinline__ void scale_slot(const Scaled& sc, const double* __res
     sc.Q4[s] = make_int4((int)u[0], (int)u[1], (int)u[2], fin ? t : (t | Q4_NONFINITE));
     if (sc.qf) sc.qf[s] = sc.charged[s] ? (float)sc.charge[s] : 0.0f;
   }
+}
+__device__ __forceinline__ void scale_slot(const Scaled& sc, const double* __restrict__ R, int s, int Npad) {
+  scale_vals(sc, R[s], R[Npad + s], R[2 * Npad + s], s);
 }
 
 // d_k of slot s, |R - R0|^2 summed in this order with no contraction (pass
@@ -258,12 +256,13 @@ __device__ __forceinline__ int cell_coord(double s, int M) {
   int c = (int)(M * (s - floor(s)));
   return c >= M ? c - M : c;
 }
-__global__ void k_cell_key(int* __restrict__ key, int* __restrict__ count, const double4* __restrict__ Rs4,
-                           int M, int N) {
+// Cell of each slot from Rs = R / L (the division of k_scale4), counted per
+// cell for the counting sort.
+__global__ void k_cell_key(int* __restrict__ key, int* __restrict__ count, const double* __restrict__ R, double L,
+                           int M, int N, int Npad) {
   int s = blockIdx.x * blockDim.x + threadIdx.x;
   if (s >= N) return;
-  double4 r = Rs4[s];
-  int ix = cell_coord(r.x, M), iy = cell_coord(r.y, M), iz = cell_coord(r.z, M);
+  int ix = cell_coord(R[s] / L, M), iy = cell_coord(R[Npad + s] / L, M), iz = cell_coord(R[2 * Npad + s] / L, M);
   int c = ix + M * (iy + M * iz);
   key[s] = c;
   atomicAdd(&count[c], 1);
@@ -284,27 +283,31 @@ __global__ void k_scan_block(int* __restrict__ out, int* __restrict__ bsum, cons
   if (i < n) out[i] = sh[threadIdx.x] - v;
   if (threadIdx.x == BLOCK - 1) bsum[blockIdx.x] = sh[BLOCK - 1];
 }
-__global__ void k_scan_sums(int* bsum, int nb, int* total) {
-  // one block; serial carry across chunks of BLOCK
-  __shared__ int sh[BLOCK];
-  int carry = 0;
-  for (int base = 0; base < nb; base += BLOCK) {
-    int i = base + threadIdx.x;
-    int v = i < nb ? bsum[i] : 0;
-    sh[threadIdx.x] = v;
-    __syncthreads();
-    for (int off = 1; off < BLOCK; off <<= 1) {
-      int t = threadIdx.x >= off ? sh[threadIdx.x - off] : 0;
-      __syncthreads();
-      sh[threadIdx.x] += t;
-      __syncthreads();
-    }
-    if (i < nb) bsum[i] = sh[threadIdx.x] - v + carry;
-    int add = sh[BLOCK - 1];
-    __syncthreads();
-    carry += add;
+// Exclusive scan of the nb block sums in one block of SCAN_T threads: each
+// thread sums a contiguous run, the runs are scanned across the block, and
+// each thread writes its run's prefixes.
+constexpr int SCAN_T = 1024;
+__global__ void __launch_bounds__(SCAN_T) k_scan_sums(int* bsum, int nb, int* total) {
+  __shared__ int wsum[SCAN_T / 64];
+  const int t = threadIdx.x, lane = t & 63, w = t >> 6;
+  const int per = (nb + SCAN_T - 1) / SCAN_T, a = min(t * per, nb), b = min(a + per, nb);
+  int sum = 0;
+  for (int k = a; k < b; k++) sum += bsum[k];
+  int incl = sum;
+  for (int off = 1; off < 64; off <<= 1) {
+    int y = __shfl_up(incl, off, 64);
+    if (lane >= off) incl += y;
   }
-  if (threadIdx.x == 0 && total) *total = carry;
+  if (lane == 63) wsum[w] = incl;
+  __syncthreads();
+  if (t == 0) {
+    int run = 0;
+    for (int k = 0; k < SCAN_T / 64; k++) { int v = wsum[k]; wsum[k] = run; run += v; }
+  }
+  __syncthreads();
+  int run = wsum[w] + incl - sum;
+  for (int k = a; k < b; k++) { int v = bsum[k]; bsum[k] = run; run += v; }
+  if (t == SCAN_T - 1 && total) *total = run;
 }
 __global__ void k_scan_add(int* __restrict__ out, const int* __restrict__ bsum, int n) {
   int i = blockIdx.x * BLOCK + threadIdx.x;
@@ -329,18 +332,41 @@ __global__ void k_cell_sort(int* __restrict__ order, const int* __restrict__ sta
     order[y + 1] = v;
   }
 }
-template <typename T>
-__global__ void k_permute(T* __restrict__ dst, const T* __restrict__ src, const int* __restrict__ order, int N) {
-  int s = blockIdx.x * blockDim.x + threadIdx.x;
-  if (s < N) dst[s] = src[order[s]];
-}
 // ncomp consecutive [Npad] components at once
-__global__ void k_permute_n(double* __restrict__ dst, const double* __restrict__ src, const int* __restrict__ order,
-                            int ncomp, int N, int Npad) {
+// Every per-slot array gathered into the new slot order in one pass: slot s
+// takes old slot order[s].  R goes to both R (via scratch) and R0 (the list's
+// reference positions); the inverse permutation is written as perm is.
+// F is permuted only when given (other layers' forces; the current layer's
+// are recomputed after every list build).
+struct PermuteAll {
+  const int* order;
+  const double *R, *P, *F, *invMass, *charge;
+  double *Rn, *R0, *Pn, *Fn, *invMassn, *chargen;
+  const int *type, *body, *perm, *freeFlag, *chargedFlag;
+  int *typen, *bodyn, *permn, *freeFlagn, *chargedFlagn, *iperm;
+  int nF;
+};
+__global__ void k_permute_all(PermuteAll a, int N, int Npad) {
   int s = blockIdx.x * blockDim.x + threadIdx.x;
   if (s >= N) return;
-  int o = order[s];
-  for (int c = 0; c < ncomp; c++) dst[(size_t)c * Npad + s] = src[(size_t)c * Npad + o];
+  const int o = a.order[s];
+  for (int c = 0; c < 3; c++) {
+    double r = a.R[(size_t)c * Npad + o];
+    a.Rn[(size_t)c * Npad + s] = r;
+    a.R0[(size_t)c * Npad + s] = r;
+    a.Pn[(size_t)c * Npad + s] = a.P[(size_t)c * Npad + o];
+  }
+  if (a.F)
+    for (int c = 0; c < a.nF; c++) a.Fn[(size_t)c * Npad + s] = a.F[(size_t)c * Npad + o];
+  a.invMassn[s] = a.invMass[o];
+  a.chargen[s] = a.charge[o];
+  a.typen[s] = a.type[o];
+  a.bodyn[s] = a.body[o];
+  a.freeFlagn[s] = a.freeFlag[o];
+  a.chargedFlagn[s] = a.chargedFlag[o];
+  const int p = a.perm[o];
+  a.permn[s] = p;
+  a.iperm[p] = s;
 }
 
 // ---- mixed form: FP32 pair arithmetic, FP64 accumulation -----------------
@@ -369,13 +395,19 @@ struct MixedCut {
 
 // P16 (optional): the top 16 bits of each fixed-point coordinate, x and y
 // packed in one word, z with bit 31 set when the position is not finite.
+// key (optional): the cell of each slot, as k_cell_key computes it.
 __global__ void k_scale4q(double4* __restrict__ Rs4, int4* __restrict__ Q4, float* __restrict__ qf,
                           const double* __restrict__ R, const int* __restrict__ type, const double* __restrict__ charge,
-                          const int* __restrict__ charged, double L, int N, int Npad, uint2* __restrict__ P16) {
+                          const int* __restrict__ charged, double L, int N, int Npad, uint2* __restrict__ P16,
+                          int* __restrict__ key, int M) {
   int s = blockIdx.x * blockDim.x + threadIdx.x;
   if (s >= N) return;
   Scaled sc{ Rs4, Q4, qf, type, charge, charged, L };
   scale_slot(sc, R, s, Npad);
+  if (key) {
+    double4 r = Rs4[s];
+    key[s] = cell_coord(r.x, M) + M * (cell_coord(r.y, M) + M * cell_coord(r.z, M));
+  }
   if (P16) {
     int4 q = Q4[s];
     P16[s] = make_uint2(((unsigned int)q.x >> 16) | ((unsigned int)q.y & 0xffff0000u),
@@ -1433,6 +1465,62 @@ __global__ void k_displace_u2(double* __restrict__ R, const double* __restrict__
     *rx = r;
   }
 }
+// k_displace_u2 that also forms pass A of the rebuild trigger for the new
+// positions (k_disp_a: per-block maximum of |R - R0|^2 with the smallest
+// original index holding it) and their scaled copies, so that the force
+// evaluation that follows does not read R and R0 again.  Two slots per
+// thread: part holds one (max, index) pair per block of 2 BLOCK slots.
+__global__ void k_displace_u2_trig(double* __restrict__ R, const double* __restrict__ P, double CR, double CP,
+                                   int N, int Npad, double im, const double* __restrict__ R0,
+                                   const int* __restrict__ perm, double* __restrict__ part, Scaled sc) {
+  __shared__ double smax[BLOCK / 64];
+  __shared__ int sidx[BLOCK / 64];
+  int s = 2 * (blockIdx.x * BLOCK + threadIdx.x);
+  double v = -1.0;
+  int idx = INT_MAX;
+  if (s < N) {
+    bool two = s + 1 < N;
+    double2 rn[3], r0[3];
+#pragma unroll
+    for (int x = 0; x < 3; x++) {
+      double2* rx = reinterpret_cast<double2*>(R + (size_t)x * Npad + s);
+      double2 r = *rx, p = *reinterpret_cast<const double2*>(P + (size_t)x * Npad + s);
+      r.x = CR * r.x + CP * p.x * im;
+      if (two) r.y = CR * r.y + CP * p.y * im;
+      *rx = r;
+      rn[x] = r;
+      r0[x] = *reinterpret_cast<const double2*>(R0 + (size_t)x * Npad + s);
+    }
+    {
+#pragma clang fp contract(off)
+      double dx = rn[0].x - r0[0].x, dy = rn[1].x - r0[1].x, dz = rn[2].x - r0[2].x;
+      v = dx * dx + dy * dy + dz * dz;
+      idx = perm[s];
+      if (two) {
+        double ex = rn[0].y - r0[0].y, ey = rn[1].y - r0[1].y, ez = rn[2].y - r0[2].y;
+        double w = ex * ex + ey * ey + ez * ez;
+        int j = perm[s + 1];
+        if (w > v || (w == v && j < idx)) { v = w; idx = j; }
+      }
+    }
+    scale_vals(sc, rn[0].x, rn[1].x, rn[2].x, s);
+    if (two) scale_vals(sc, rn[0].y, rn[1].y, rn[2].y, s + 1);
+  }
+  for (int off = 32; off > 0; off >>= 1) {
+    double ov = __shfl_down(v, off, 64);
+    int oi = __shfl_down(idx, off, 64);
+    if (ov > v || (ov == v && oi < idx)) { v = ov; idx = oi; }
+  }
+  int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  if (lane == 0) { smax[wid] = v; sidx[wid] = idx; }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    for (int w = 1; w < BLOCK / 64; w++)
+      if (smax[w] > v || (smax[w] == v && sidx[w] < idx)) { v = smax[w]; idx = sidx[w]; }
+    part[2 * blockIdx.x] = v;
+    part[2 * blockIdx.x + 1] = (double)idx;
+  }
+}
 
 // ---- shadow terms of EmDee_verlet_step, free atoms (EmDeeCode.f90:1167-1204)
 // s0 = 2.5 R + dt/2 invMass (P - dt/2 F), stored by ORIGINAL atom index so
@@ -1612,6 +1700,8 @@ Device* dev_create(const DeviceSetup& s) {
   d->invMass.alloc(Np); d->charge.alloc(Np);
   d->type.alloc(Np); d->body.alloc(Np); d->perm.alloc(Np); d->freeFlag.alloc(Np); d->chargedFlag.alloc(Np);
   d->tmp3.alloc(3 * (size_t)Np); d->tmpF.alloc(3 * (size_t)Np * s.nlayers); d->tmp1.alloc(Np); d->tmpI.alloc(Np);
+  d->tmpP.alloc(3 * (size_t)Np); d->tmpQ.alloc(Np);
+  d->tmpBody.alloc(Np); d->tmpPerm.alloc(Np); d->tmpFree.alloc(Np); d->tmpCharged.alloc(Np);
   d->stage.alloc(3 * (size_t)std::max(s.N, 1));
   HIP_CHECK(hipHostMalloc(&d->hstage, 3 * sizeof(double) * std::max(s.N, 1), hipHostMallocDefault));
   HIP_CHECK(hipHostMalloc(&d->hresult, 64 * sizeof(double), hipHostMallocDefault));
@@ -1622,6 +1712,7 @@ Device* dev_create(const DeviceSetup& s) {
   *d->hseq = 0;
   d->result.alloc(64);
   d->partials.alloc(64 * (size_t)(Np / BLOCK + 1));   // up to 16 threads per atom x 4 sums
+  d->trigPart.alloc(2 * (size_t)(Np / BLOCK + 1));
   d->trig.alloc(4);
   HIP_CHECK(hipMemsetAsync(d->Rs4.p, 0, sizeof(double4) * Np, d->stream));
   d->key.alloc(Np); d->order.alloc(Np); d->nnb.alloc(Np); d->flags.alloc(4);
@@ -1670,9 +1761,9 @@ Device* dev_create(const DeviceSetup& s) {
 void dev_destroy(Device* d) {
   if (!d) return;
   (void)hipStreamSynchronize(d->stream);
-  for (DBuf<double>* b : { &d->R, &d->P, &d->R0, &d->invMass, &d->charge, &d->F, &d->tmp3, &d->tmpF, &d->tmp1, &d->stage, &d->partials, &d->result, &d->trig, &d->shadow }) b->release();
-  d->Rs4.release(); d->Q4.release(); d->qf.release(); d->eatom.release();
-  for (DBuf<int>* b : { &d->type, &d->body, &d->perm, &d->freeFlag, &d->chargedFlag, &d->tmpI, &d->key, &d->cellCount, &d->cellStart, &d->order, &d->blockSums, &d->nbr, &d->nnb, &d->flags, &d->exOff, &d->exItem, &d->iperm }) b->release();
+  for (DBuf<double>* b : { &d->R, &d->P, &d->R0, &d->invMass, &d->charge, &d->F, &d->tmp3, &d->tmpF, &d->tmp1, &d->tmpP, &d->tmpQ, &d->stage, &d->partials, &d->trigPart, &d->result, &d->trig, &d->shadow }) b->release();
+  d->Rs4.release(); d->Q4.release(); d->P16.release(); d->qf.release(); d->eatom.release();
+  for (DBuf<int>* b : { &d->type, &d->body, &d->perm, &d->freeFlag, &d->chargedFlag, &d->tmpI, &d->tmpBody, &d->tmpPerm, &d->tmpFree, &d->tmpCharged, &d->key, &d->cellCount, &d->cellStart, &d->order, &d->blockSums, &d->nbr, &d->nnb, &d->flags, &d->exOff, &d->exItem, &d->iperm }) b->release();
   d->pairs.release(); d->couls.release(); d->interact.release(); d->ljt.release(); d->ljk.release(); d->ljt64.release();
   d->ljk64.release();
   dev_ewald_release(d);
@@ -1771,7 +1862,10 @@ static void download_aos(Device* d, double* host, const double* src) {
   memcpy(host, d->hstage, 3 * sizeof(double) * N);
 }
 
-void dev_upload_positions(Device* d, const double* R) { if (d->N) upload_aos(d, d->R.p, R, nullptr); }
+void dev_upload_positions(Device* d, const double* R) {
+  d->posWrites++;
+  if (d->N) upload_aos(d, d->R.p, R, nullptr);
+}
 void dev_download_positions(Device* d, double* R) { if (d->N) download_aos(d, R, d->R.p); }
 void dev_upload_momenta(Device* d, const double* P) { if (d->N) upload_aos(d, d->P.p, P, d->freeFlag.p); }
 void dev_download_momenta(Device* d, double* P) {
@@ -1820,12 +1914,21 @@ void dev_trigger_launch(Device* d, double L, bool mixed, int layer) {
   // test divides R by L itself)
   bool needRs4 = !mixed || !lj_path(d, layer);
   // (the FP32 charges change only with the slot order: the list build writes them)
-  Scaled sc{ needRs4 ? d->Rs4.p : nullptr, mixed ? d->Q4.p : nullptr, nullptr, d->type.p, d->charge.p,
-             d->chargedFlag.p, L };
-  hipLaunchKernelGGL(k_disp_a, dim3(g), dim3(BLOCK), 0, d->stream, d->partials.p, d->R.p, d->R0.p, d->perm.p, N,
-                     d->Npad, sc);
+  int nb = g;
+  if (d->trigPreAt == d->posWrites && d->trigPreL == L && d->trigPreMixed == mixed && d->trigPreRs4 == needRs4) {
+    nb = d->trigPreBlocks;               // the drift formed pass A for these positions
+  } else {
+    Scaled sc{ needRs4 ? d->Rs4.p : nullptr, mixed ? d->Q4.p : nullptr, nullptr, d->type.p, d->charge.p,
+               d->chargedFlag.p, L };
+    hipLaunchKernelGGL(k_disp_a, dim3(g), dim3(BLOCK), 0, d->stream, d->trigPart.p, d->R.p, d->R0.p, d->perm.p, N,
+                       d->Npad, sc);
+  }
+  d->trigKnown = true;
+  d->trigMixed = mixed;
+  d->trigRs4 = needRs4;
+  d->trigL = L;
   d->trigSeq = ++d->seq;
-  hipLaunchKernelGGL(k_disp_a_reduce, dim3(1), dim3(BLOCK), 0, d->stream, d->trig.p, d->partials.p, g, d->hpostDev,
+  hipLaunchKernelGGL(k_disp_a_reduce, dim3(1), dim3(BLOCK), 0, d->stream, d->trig.p, d->trigPart.p, nb, d->hpostDev,
                      d->hseqDev, d->trigSeq);
   launch_check("trigger");
 }
@@ -1889,41 +1992,48 @@ void dev_rebuild(Device* d, double L, double xRc, double xInRc) {
   (void)xInRc;
   int N = d->N, Np = d->Npad;
   if (N == 0) return;
+  d->posWrites++;                      // R and R0 in a new slot order
   int g = div_up(N, BLOCK);
   int M = (int)std::floor(CELL_REACH * L / xRc);
   if (M < 2 * CELL_REACH + 1) M = 2 * CELL_REACH + 1;   // every cell of a row at most once
   if ((long)M * M * M > 64l * 1024 * 1024) M = 400;   // bound the cell table
   ensure_cells(d, M);
   int nc = d->ncells, nc1 = nc + 1, gc = div_up(nc1, BLOCK);
-  // current scaled positions
-  hipLaunchKernelGGL(k_scale4, dim3(g), dim3(BLOCK), 0, d->stream, d->Rs4.p, d->R.p, d->type.p, L, N, Np);
   // counting sort by cell: count, exclusive scan over ncells+1 entries
   // (start[ncells] = N), scatter, order inside cells
   HIP_CHECK(hipMemsetAsync(d->cellCount.p, 0, sizeof(int) * nc1, d->stream));
-  hipLaunchKernelGGL(k_cell_key, dim3(g), dim3(BLOCK), 0, d->stream, d->key.p, d->cellCount.p, d->Rs4.p, M, N);
+  hipLaunchKernelGGL(k_cell_key, dim3(g), dim3(BLOCK), 0, d->stream, d->key.p, d->cellCount.p, d->R.p, L, M, N, Np);
   hipLaunchKernelGGL(k_scan_block, dim3(gc), dim3(BLOCK), 0, d->stream, d->cellStart.p, d->blockSums.p, d->cellCount.p, nc1);
-  hipLaunchKernelGGL(k_scan_sums, dim3(1), dim3(BLOCK), 0, d->stream, d->blockSums.p, gc, (int*)nullptr);
+  hipLaunchKernelGGL(k_scan_sums, dim3(1), dim3(SCAN_T), 0, d->stream, d->blockSums.p, gc, (int*)nullptr);
   hipLaunchKernelGGL(k_scan_add, dim3(gc), dim3(BLOCK), 0, d->stream, d->cellStart.p, d->blockSums.p, nc1);
   HIP_CHECK(hipMemsetAsync(d->cellCount.p, 0, sizeof(int) * nc1, d->stream));
   hipLaunchKernelGGL(k_cell_scatter, dim3(g), dim3(BLOCK), 0, d->stream, d->order.p, d->cellCount.p, d->key.p, d->cellStart.p, N);
   hipLaunchKernelGGL(k_cell_sort, dim3(div_up(nc, BLOCK)), dim3(BLOCK), 0, d->stream, d->order.p, d->cellStart.p, d->perm.p, nc, N);
   launch_check("cell sort");
-  // permute every per-atom array into the new order (write to scratch, swap)
-  auto permN = [&](DBuf<double>& a, DBuf<double>& tmp, int ncomp) {
-    hipLaunchKernelGGL(k_permute_n, dim3(g), dim3(BLOCK), 0, d->stream, tmp.p, a.p, d->order.p, ncomp, N, Np);
-    std::swap(a.p, tmp.p); std::swap(a.n, tmp.n);
-  };
-  auto permI = [&](DBuf<int>& a) {
-    hipLaunchKernelGGL(k_permute<int>, dim3(g), dim3(BLOCK), 0, d->stream, d->tmpI.p, a.p, d->order.p, N);
-    std::swap(a.p, d->tmpI.p); std::swap(a.n, d->tmpI.n);
-  };
-  permN(d->R, d->tmp3, 3);
-  permN(d->P, d->tmp3, 3);
-  permN(d->F, d->tmpF, 3 * d->nlayers);
-  permN(d->invMass, d->tmp1, 1);
-  permN(d->charge, d->tmp1, 1);
-  permI(d->type); permI(d->body); permI(d->perm); permI(d->freeFlag); permI(d->chargedFlag);
-  hipLaunchKernelGGL(k_invert_perm, dim3(g), dim3(BLOCK), 0, d->stream, d->iperm.p, d->perm.p, N);
+  // every per-slot array into the new order (written to scratch, swapped),
+  // R0 = R, and the inverse permutation
+  {
+    PermuteAll pa;
+    pa.order = d->order.p;
+    pa.R = d->R.p; pa.Rn = d->tmp3.p; pa.R0 = d->R0.p;
+    pa.P = d->P.p; pa.Pn = d->tmpP.p;
+    pa.F = d->nlayers > 1 ? d->F.p : nullptr; pa.Fn = d->tmpF.p; pa.nF = 3 * d->nlayers;
+    pa.invMass = d->invMass.p; pa.invMassn = d->tmp1.p;
+    pa.charge = d->charge.p; pa.chargen = d->tmpQ.p;
+    pa.type = d->type.p; pa.typen = d->tmpI.p;
+    pa.body = d->body.p; pa.bodyn = d->tmpBody.p;
+    pa.perm = d->perm.p; pa.permn = d->tmpPerm.p;
+    pa.freeFlag = d->freeFlag.p; pa.freeFlagn = d->tmpFree.p;
+    pa.chargedFlag = d->chargedFlag.p; pa.chargedFlagn = d->tmpCharged.p;
+    pa.iperm = d->iperm.p;
+    hipLaunchKernelGGL(k_permute_all, dim3(g), dim3(BLOCK), 0, d->stream, pa, N, Np);
+    auto swapD = [](DBuf<double>& x, DBuf<double>& y) { std::swap(x.p, y.p); std::swap(x.n, y.n); };
+    auto swapI = [](DBuf<int>& x, DBuf<int>& y) { std::swap(x.p, y.p); std::swap(x.n, y.n); };
+    swapD(d->R, d->tmp3); swapD(d->P, d->tmpP); swapD(d->invMass, d->tmp1); swapD(d->charge, d->tmpQ);
+    if (d->nlayers > 1) swapD(d->F, d->tmpF);
+    swapI(d->type, d->tmpI); swapI(d->body, d->tmpBody); swapI(d->perm, d->tmpPerm);
+    swapI(d->freeFlag, d->tmpFree); swapI(d->chargedFlag, d->tmpCharged);
+  }
   launch_check("permute");
   // scaled and fixed-point positions and keys in the new order (functions of
   // R, type and charge)
@@ -1931,10 +2041,8 @@ void dev_rebuild(Device* d, double L, double xRc, double xInRc) {
   d->qf.alloc(Np);
   d->P16.alloc(Np + 16);               // the segment list build reads up to 16 slots past N
   hipLaunchKernelGGL(k_scale4q, dim3(g), dim3(BLOCK), 0, d->stream, d->Rs4.p, d->Q4.p, d->qf.p, d->R.p, d->type.p,
-                     d->charge.p, d->chargedFlag.p, L, N, Np, d->P16.p);
-  hipLaunchKernelGGL(k_cell_key, dim3(g), dim3(BLOCK), 0, d->stream, d->key.p, d->cellCount.p, d->Rs4.p, M, N);
-  // R0 = R
-  HIP_CHECK(hipMemcpyAsync(d->R0.p, d->R.p, 3 * sizeof(double) * Np, hipMemcpyDeviceToDevice, d->stream));
+                     d->charge.p, d->chargedFlag.p, L, N, Np, d->P16.p, d->key.p, M);
+
   // Verlet list, growing the ELL capacity on overflow
   double xRc2 = (xRc * xRc) / (L * L);
   const double two32 = 4294967296.0, rc2u = xRc2 * two32 * two32, band = band_rel(xRc2);
@@ -2023,7 +2131,7 @@ void dev_compute_pairs(Device* d, const PairCall& c, ForceSums* out) {
     d->Q4.alloc(Np);
     d->qf.alloc(Np);
     hipLaunchKernelGGL(k_scale4q, dim3(g), dim3(BLOCK), 0, d->stream, d->Rs4.p, d->Q4.p, d->qf.p, d->R.p, d->type.p,
-                       d->charge.p, d->chargedFlag.p, c.L, N, Np, (uint2*)nullptr);
+                       d->charge.p, d->chargedFlag.p, c.L, N, Np, (uint2*)nullptr, (int*)nullptr, 0);
   } else {
     hipLaunchKernelGGL(k_scale4, dim3(g), dim3(BLOCK), 0, d->stream, d->Rs4.p, d->R.p, d->type.p, c.L, N, Np);
   }
@@ -2176,7 +2284,20 @@ void dev_kinetic(Device* d, double* twoKE) { dev_boost(d, 0, 1.0, 0.0, false, tw
 void dev_displace(Device* d, double CR, double CP) {
   int N = d->N;
   if (N == 0) return;
-  if (d->uniformFree)
+  d->posWrites++;
+  if (d->uniformFree && d->trigKnown) {
+    // the drift with the next trigger's pass A, in the last trigger's set-up
+    int g = div_up(div_up(N, 2), BLOCK);
+    Scaled sc{ d->trigRs4 ? d->Rs4.p : nullptr, d->trigMixed ? d->Q4.p : nullptr, nullptr, d->type.p, d->charge.p,
+               d->chargedFlag.p, d->trigL };
+    hipLaunchKernelGGL(k_displace_u2_trig, dim3(g), dim3(BLOCK), 0, d->stream, d->R.p, d->P.p, CR, CP, N, d->Npad,
+                       d->invMass0, d->R0.p, d->perm.p, d->trigPart.p, sc);
+    d->trigPreAt = d->posWrites;
+    d->trigPreBlocks = g;
+    d->trigPreMixed = d->trigMixed;
+    d->trigPreRs4 = d->trigRs4;
+    d->trigPreL = d->trigL;
+  } else if (d->uniformFree)
     hipLaunchKernelGGL(k_displace_u2, dim3(div_up(div_up(N, 2), BLOCK)), dim3(BLOCK), 0, d->stream, d->R.p, d->P.p, CR,
                        CP, N, d->Npad, d->invMass0);
   else
@@ -2315,6 +2436,7 @@ __global__ void k_copy_phase(double* __restrict__ dR, double* __restrict__ dP, c
 
 void dev_copy_phase(Device* dst, Device* src) {
   if (dst->N != src->N) fatal("phase space sharing", "different numbers of atoms");
+  dst->posWrites++;
   HIP_CHECK(hipStreamSynchronize(src->stream));
   if (dst->N > 0)
     hipLaunchKernelGGL(k_copy_phase, dim3(div_up(dst->N, BLOCK)), dim3(BLOCK), 0, dst->stream, dst->R.p, dst->P.p,

@@ -149,8 +149,9 @@ struct Device {
   DBuf<uint2> P16;        // list build: 16-bit fixed-point scaled coordinates {x | y << 16, z | nonfinite << 31}
   DBuf<float> qf;         // mixed form: FP32 charge (0 when uncharged)
   DBuf<int> type, body, perm, freeFlag, chargedFlag;
-  DBuf<double> tmp3, tmpF, tmp1;   // permutation scratch (swapped with the arrays)
-  DBuf<int> tmpI;
+  // permutation scratch, one per permuted array (swapped with the arrays)
+  DBuf<double> tmp3, tmpF, tmp1, tmpP, tmpQ;
+  DBuf<int> tmpI, tmpBody, tmpPerm, tmpFree, tmpCharged;
   // transfers
   DBuf<double> stage;     // 3N AoS, original order
   double* hstage = nullptr;  // pinned host mirror
@@ -171,9 +172,9 @@ struct Device {
   DBuf<uint8_t> interact;
   DBuf<int> exOff, exItem;
   int nexcl = 0;
-  bool allInteract = false;
+  bool allInteract = false;   // every type pair has a model (interaction table all ones)
   bool uniformFree = false;   // every atom free and of one mass: the integrator skips the per-atom reads
-  double invMass0 = 1.0;   // every type pair has a model (interaction table all ones)
+  double invMass0 = 1.0;      // that mass's inverse
   std::vector<int> hostType;
   // reductions
   DBuf<double> partials, result;
@@ -194,6 +195,18 @@ struct Device {
   DBuf<double> ljk64;
   std::vector<char> ljcOk;
   unsigned long long trigSeq = 0;   // post of the last rebuild trigger (slot 0)
+  // Every write of R on the device counts in posWrites.  The drift of
+  // uniform free atoms forms the next trigger's pass A (partials, scaled
+  // positions) on the way, with the set-up of the last trigger (trigMixed,
+  // trigRs4, trigL); the trigger uses them while no other write intervened.
+  unsigned long long posWrites = 0;
+  bool trigKnown = false, trigMixed = false, trigRs4 = false;
+  double trigL = 0.0;
+  unsigned long long trigPreAt = ~0ull;   // posWrites when the drift formed pass A
+  DBuf<double> trigPart;                  // pass A partials (no other reduction uses them)
+  int trigPreBlocks = 0;
+  bool trigPreMixed = false, trigPreRs4 = false;
+  double trigPreL = 0.0;
   double trigLast = -1.0, trigPrev = -1.0;   // trigger values of the last two steps (-1: none)
   DBuf<double> trig;      // [0] max, [1] first original index holding it, [2] max before it
   // rigid bodies; iperm: original atom -> slot

@@ -720,6 +720,7 @@ void dev_bodies_setup(Device* d, const std::vector<int>& start, const std::vecto
 }
 
 void dev_bodies_update(Device* d, double L) {
+  d->posWrites++;
   BodyState& B = d->bodies;
   if (B.nb == 0) return;
   hipLaunchKernelGGL(k_body_update, dim3(div_up(B.nb, BLOCK)), dim3(BLOCK), 0, d->stream, B, d->R.p, d->iperm.p, L, d->Npad);
@@ -744,6 +745,7 @@ void dev_bodies_boost(Device* d, int layer, double CP, double CF, bool translate
 }
 
 void dev_bodies_move(Device* d, double CR, double CP, double dt, bool translate, bool rotate, int mode) {
+  d->posWrites++;
   BodyState& B = d->bodies;
   if (B.nb == 0) return;
   hipLaunchKernelGGL(k_body_move, dim3(div_up(B.nb, BLOCK)), dim3(BLOCK), 0, d->stream, B, d->R.p, d->iperm.p, CR, CP, dt,
@@ -783,6 +785,7 @@ void dev_bodies_particle_momenta(Device* d) {
 }
 
 void dev_bodies_place(Device* d) {
+  d->posWrites++;
   BodyState& B = d->bodies;
   if (B.nb == 0) return;
   hipLaunchKernelGGL(k_body_place, dim3(div_up(B.nb, BLOCK)), dim3(BLOCK), 0, d->stream, B, d->R.p, d->iperm.p, d->Npad);
