@@ -354,15 +354,27 @@ __global__ void __launch_bounds__(256) k_sg_mlp(float* __restrict__ eat, float* 
     if (threadIdx.x == 0) epart[blockIdx.x] = 0.0;
     return;
   }
-  // stage the species' weights: 8-byte chunks into padded rows
+  // stage the species' weights: 16-byte chunks into padded rows, eight
+  // chunks per thread loaded before any is stored, so that the loads overlap
   {
     auto stage = [&](const unsigned short* src, int rows, int cols, int dst, int stride) {
-      const int per = cols / 8;
-      for (int c = threadIdx.x; c < rows * per; c += 256) {
-        int r = c / per, k = 8 * (c % per);
-        uint4 v = *reinterpret_cast<const uint4*>(src + (size_t)r * cols + k);
-        *reinterpret_cast<uint2*>(lw + dst + r * stride + k) = make_uint2(v.x, v.y);
-        *reinterpret_cast<uint2*>(lw + dst + r * stride + k + 4) = make_uint2(v.z, v.w);
+      const int per = cols / 8, total = rows * per;
+      for (int c0 = 0; c0 < total; c0 += 8 * 256) {
+        uint4 v[8];
+#pragma unroll
+        for (int u = 0; u < 8; u++) {
+          int c = c0 + u * 256 + threadIdx.x;
+          if (c < total) v[u] = *reinterpret_cast<const uint4*>(src + (size_t)(c / per) * cols + 8 * (c % per));
+        }
+#pragma unroll
+        for (int u = 0; u < 8; u++) {
+          int c = c0 + u * 256 + threadIdx.x;
+          if (c < total) {
+            int r = c / per, k = 8 * (c % per);
+            *reinterpret_cast<uint2*>(lw + dst + r * stride + k) = make_uint2(v[u].x, v[u].y);
+            *reinterpret_cast<uint2*>(lw + dst + r * stride + k + 4) = make_uint2(v[u].z, v[u].w);
+          }
+        }
       }
     };
     stage(w1t + (size_t)s * DIN * SG_H, SG_H, DIN, Lay::W1T, Lay::S1);
