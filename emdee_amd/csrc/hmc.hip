@@ -11,12 +11,19 @@
 // captured once in a HIP graph and replayed.
 //
 // Per replica (N <= 8192 atoms) the workgroup keeps, in LDS, the 32-bit
-// fixed-point coordinates and a cell list rebuilt every step (a counting sort
-// into cells of side >= rc/DIV; stencil of (2 DIV + 1)^3 cells), so all pair
-// gathers are LDS reads.  The pair set is the reference's: r^2 < rc^2 on the
-// FP64 scaled coordinates (neighbor_lists.f90:244), with FP32 candidates in
-// the band around the cutoff re-tested in FP64.  Pair terms are FP32 (mixed
-// form) or FP64 (plain form), accumulated per atom in FP64.
+// fixed-point coordinates of the current step, so all pair gathers of the
+// mixed form are LDS reads.  Pairs come from a Verlet list per replica in
+// global memory (16-bit indices, column-major): built from a cell list
+// (counting sort into cells of side >= (rc + skin)/DIV, stencil of
+// (2 DIV + 1)^3 cells) and rebuilt before a force evaluation whenever some
+// atom has moved more than skin/2 since the last build -- the rule of
+// neighbor_lists.f90:41-59 in its simplest sufficient form -- which also
+// covers the coordinates restored after a rejected move.  A replica whose
+// list would overflow its capacity (denser than LJ close packing) takes every
+// atom as a candidate instead.  The pair set is the reference's: r^2 < rc^2 on the FP64
+// scaled coordinates (neighbor_lists.f90:244), with FP32 candidates in the
+// band around the cutoff re-tested in FP64.  Pair terms are FP32 (mixed form)
+// or FP64 (plain form), accumulated per atom in FP64.
 //
 // Random numbers come from a counter-based generator (splitmix64 finaliser,
 // Steele, Lea & Flood 2014) keyed by (seed, replica, cycle, atom, component),
@@ -44,11 +51,13 @@ constexpr int HMC_CMAX = 6144;   // cells per replica
 const double TWO_PI_D = 6.283185307179586;
 
 struct ReplicaDesc {
-  int off, N, M, pad;
+  int off, N, M, Ml;       // atom offset and count; cells per edge for the force walk and for the list build
   double L, kT;
   float K, lo, hi, fpad;   // fixed-point cutoff band (mixed form)
   double Rc2s, unit;       // FP64 scaled cutoff; L / 2^32
+  double Rl2s, trig2s;     // list radius (rc + skin)^2 and (skin/2)^2, scaled units^2
 };
+constexpr double HMC_SKIN = 0.3;   // list skin (LJ units; 0.2 and 0.45 measured slower)
 
 __host__ __device__ inline uint64_t mix64(uint64_t z) {
   z = (z ^ (z >> 30)) * 0xbf58476d1ce4e5b9ULL;
@@ -64,6 +73,11 @@ __host__ __device__ inline uint64_t hmc_stream(int replica, int cycle) {
   return ((uint64_t)(uint32_t)replica << 32) | (uint32_t)cycle;
 }
 constexpr uint64_t HMC_ACCEPT_COUNTER = 1ULL << 40;
+
+__device__ __forceinline__ double wave_max(double v) {
+  for (int off = 32; off > 0; off >>= 1) v = fmax(v, __shfl_xor(v, off, 64));
+  return v;
+}
 
 template <int BS>
 __device__ double block_total(double v, double* red) {
@@ -85,7 +99,8 @@ __global__ void __launch_bounds__(BS)
 k_hmc(const ReplicaDesc* __restrict__ desc, const int* __restrict__ order, double4* __restrict__ R,
       double4* __restrict__ P, double4* __restrict__ F, double4* __restrict__ Rsave, double4* __restrict__ Fsave,
       double* __restrict__ U, int* __restrict__ cycle, long long* __restrict__ accepted,
-      double* __restrict__ lastDH, double rc, double dt, int nsteps, uint64_t seed, int ncycles, int mode) {
+      double* __restrict__ lastDH, double rc, double dt, int nsteps, uint64_t seed, int ncycles, int mode,
+      unsigned short* __restrict__ nlist, unsigned short* __restrict__ nnb, uint4* __restrict__ qbuild, int cap) {
   __shared__ unsigned int qx[HMC_NMAX], qy[HMC_NMAX], qz[HMC_NMAX];
   __shared__ unsigned short clist[HMC_NMAX];
   __shared__ int cstart[HMC_CMAX + 1];
@@ -94,10 +109,11 @@ k_hmc(const ReplicaDesc* __restrict__ desc, const int* __restrict__ order, doubl
   __shared__ int scanTmp[BS];
   __shared__ int cycleNow;
   __shared__ double Unow;
+  __shared__ int listState;   // 0: no valid list, 1: valid list, 2: overflow (cell walk)
 
   const int r = order[blockIdx.x];
   const ReplicaDesc d = desc[r];
-  const int N = d.N, M = d.M, nc = M * M * M;
+  const int N = d.N;
   double4* Rr = R + d.off;
   double4* Pr = P + d.off;
   double4* Fr = F + d.off;
@@ -110,9 +126,10 @@ k_hmc(const ReplicaDesc* __restrict__ desc, const int* __restrict__ order, doubl
     unsigned int old = atomicAdd(&ccount2[c >> 1], 1u << ((c & 1) * 16));
     return (int)((old >> ((c & 1) * 16)) & 0xffffu);
   };
-  auto build_cells = [&]() {
-    for (int c = threadIdx.x; c < (nc + 1) / 2; c += BS) ccount2[c] = 0;
-    __syncthreads();
+  // fixed-point coordinates of the current R into LDS; returns the largest
+  // squared displacement since the last list build (scaled units^2)
+  auto refresh = [&]() -> double {
+    double m = 0.0;
     for (int i = threadIdx.x; i < N; i += BS) {
       double4 ri = Rr[i];
       double s[3] = { ri.x / L, ri.y / L, ri.z / L };
@@ -122,13 +139,32 @@ k_hmc(const ReplicaDesc* __restrict__ desc, const int* __restrict__ order, doubl
         u[x] = (unsigned int)(unsigned long long)llrint(f * 4294967296.0);
       }
       qx[i] = u[0]; qy[i] = u[1]; qz[i] = u[2];
-      int c = (int)(((unsigned long long)u[0] * M) >> 32) +
-              M * ((int)(((unsigned long long)u[1] * M) >> 32) + M * (int)(((unsigned long long)u[2] * M) >> 32));
-      bump(c);
+      uint4 b = qbuild[d.off + i];
+      double ex = (double)(int)(u[0] - b.x), ey = (double)(int)(u[1] - b.y), ez = (double)(int)(u[2] - b.z);
+      m = fmax(m, (ex * ex + ey * ey + ez * ez) * (1.0 / 18446744073709551616.0));
     }
+    m = wave_max(m);
+    int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    __syncthreads();
+    if (lane == 0) red[w] = m;
+    __syncthreads();
+    double t = 0.0;
+    for (int k = 0; k < BS / 64; k++) t = fmax(t, red[k]);
+    return t;
+  };
+  // cell list of the coordinates in LDS, cells of side L / Mc
+  auto build_cells = [&](int Mc) {
+    const int ncc = Mc * Mc * Mc;
+    for (int c = threadIdx.x; c < (ncc + 1) / 2; c += BS) ccount2[c] = 0;
+    __syncthreads();
+    auto cell_of = [&](int i) {
+      return (int)(((unsigned long long)qx[i] * Mc) >> 32) +
+             Mc * ((int)(((unsigned long long)qy[i] * Mc) >> 32) + Mc * (int)(((unsigned long long)qz[i] * Mc) >> 32));
+    };
+    for (int i = threadIdx.x; i < N; i += BS) bump(cell_of(i));
     __syncthreads();
     // exclusive scan of the counts: per-thread chunk sums, block scan, chunks
-    int chunk = (nc + BS - 1) / BS, c0i = threadIdx.x * chunk, c1i = min(nc, c0i + chunk);
+    int chunk = (ncc + BS - 1) / BS, c0i = threadIdx.x * chunk, c1i = min(ncc, c0i + chunk);
     int local = 0;
     for (int c = c0i; c < c1i; c++) local += count_of(c);
     scanTmp[threadIdx.x] = local;
@@ -141,16 +177,86 @@ k_hmc(const ReplicaDesc* __restrict__ desc, const int* __restrict__ order, doubl
     }
     int run = scanTmp[threadIdx.x] - local;
     for (int c = c0i; c < c1i; c++) { cstart[c] = run; run += count_of(c); }
-    if (threadIdx.x == 0) cstart[nc] = N;
+    if (threadIdx.x == 0) cstart[ncc] = N;
     __syncthreads();
-    for (int c = threadIdx.x; c < (nc + 1) / 2; c += BS) ccount2[c] = 0;
+    for (int c = threadIdx.x; c < (ncc + 1) / 2; c += BS) ccount2[c] = 0;
     __syncthreads();
     for (int i = threadIdx.x; i < N; i += BS) {
-      unsigned int u0 = qx[i], u1 = qy[i], u2 = qz[i];
-      int c = (int)(((unsigned long long)u0 * M) >> 32) +
-              M * ((int)(((unsigned long long)u1 * M) >> 32) + M * (int)(((unsigned long long)u2 * M) >> 32));
+      int c = cell_of(i);
       clist[cstart[c] + bump(c)] = (unsigned short)i;
     }
+    __syncthreads();
+  };
+  // visits, for atom i, every atom j of the cells within Mc-cell reach that
+  // can lie within sqrt(R2s) (scaled units) of it, culled row by row; the
+  // atom itself is among them.  Mc < 2 DIV + 1: one cell holds every atom.
+  auto walk = [&](int i, int Mc, double R2s, auto&& visit) {
+    const int w = (Mc >= 2 * DIV + 1) ? DIV : 0;
+    // cutoff^2 in cell units, with margins far above the FP32 rounding of the
+    // culling (which can then only keep extra cells)
+    const float rcell2 = (float)(R2s * (double)Mc * (double)Mc * (1.0 + 1e-5) + 1e-5);
+    unsigned int ui = qx[i], vi = qy[i], wi = qz[i];
+    unsigned long long px = (unsigned long long)ui * Mc, py = (unsigned long long)vi * Mc,
+                       pz = (unsigned long long)wi * Mc;
+    int cx = (int)(px >> 32), cy = (int)(py >> 32), cz = (int)(pz >> 32);
+    // position of atom i inside its cell, in cell units
+    const float ox = (float)(unsigned int)px * 2.3283064e-10f, oy = (float)(unsigned int)py * 2.3283064e-10f,
+                oz = (float)(unsigned int)pz * 2.3283064e-10f;
+    // cells are numbered x fastest, so the cells x-w..x+w of one (y, z) row
+    // are one contiguous slot range (two where the row wraps, visited in
+    // the order of increasing dx); w = 0 takes every atom at once
+    for (int dz = -w; dz <= w; dz++) {
+      int z = cz + dz; z = z < 0 ? z + Mc : (z >= Mc ? z - Mc : z);
+      float gz = fmaxf(dz > 0 ? dz - oz : (dz < 0 ? oz - dz - 1 : 0.0f), 0.0f);
+      for (int dy = -w; dy <= w; dy++) {
+        int y = cy + dy; y = y < 0 ? y + Mc : (y >= Mc ? y - Mc : y);
+        int base = Mc * (y + Mc * z), xa = cx - w, xb = cx + w;
+        if (w > 0) {   // rows and cells beyond the cutoff sphere hold no partner
+          float gy = fmaxf(dy > 0 ? dy - oy : (dy < 0 ? oy - dy - 1 : 0.0f), 0.0f);
+          float rem = rcell2 - gy * gy - gz * gz;
+          if (rem <= 0.0f) continue;
+          float ex = sqrtf(rem) + 1e-4f;
+          xa = cx + max((int)floorf(ox - ex), -w);
+          xb = cx + min((int)floorf(ox + ex), w);
+        }
+        int r0a, r0b, r1a = 0, r1b = -1;   // cell ranges [r0a, r0b] then [r1a, r1b]
+        if (xa < 0) { r0a = xa + Mc; r0b = Mc - 1; r1a = 0; r1b = xb; }
+        else if (xb >= Mc) { r0a = xa; r0b = Mc - 1; r1a = 0; r1b = xb - Mc; }
+        else { r0a = xa; r0b = xb; }
+        for (int part = 0; part < 2; part++) {
+          int ca = part == 0 ? r0a : r1a, cb = part == 0 ? r0b : r1b;
+          if (cb < ca) continue;
+          int t0 = (w == 0) ? 0 : cstart[base + ca], t1 = (w == 0) ? N : cstart[base + cb + 1];
+          for (int t = t0; t < t1; t++) visit((w == 0) ? t : (int)clist[t]);
+        }
+      }
+    }
+  };
+  // Verlet list at radius sqrt(d.Rl2s) from the coordinates in LDS (every
+  // atom within it, in walk order; the FP32 test keeps a margin, so the list
+  // may hold a few more); the build coordinates are kept for the trigger
+  auto build_list = [&]() {
+    build_cells(d.Ml);
+    const float rl2 = (float)(d.Rl2s * 18446744073709551616.0 * (1.0 + 1e-4));
+    int over = 0;
+    for (int i = threadIdx.x; i < N; i += BS) {
+      const unsigned int ui = qx[i], vi = qy[i], wi = qz[i];
+      int n = 0;
+      walk(i, d.Ml, d.Rl2s, [&](int j) {
+        float ddx = (float)(int)(ui - qx[j]), ddy = (float)(int)(vi - qy[j]), ddz = (float)(int)(wi - qz[j]);
+        float r2 = ddx * ddx + ddy * ddy + ddz * ddz;
+        if (r2 < rl2 && j != i) {
+          if (n < cap) nlist[(size_t)d.off * cap + (size_t)n * N + i] = (unsigned short)j;
+          n++;
+        }
+      });
+      nnb[d.off + i] = (unsigned short)min(n, cap);
+      qbuild[d.off + i] = make_uint4(ui, vi, wi, 0u);
+      over |= n > cap;
+    }
+    // any overflow: this replica walks the cells from now on
+    over = __syncthreads_or(over);
+    if (threadIdx.x == 0) listState = over ? 2 : 1;
     __syncthreads();
   };
 
@@ -158,99 +264,81 @@ k_hmc(const ReplicaDesc* __restrict__ desc, const int* __restrict__ order, doubl
   // (only the last step of a trajectory needs it)
   auto forces = [&](auto E_) -> double {
     constexpr bool WANT_E = decltype(E_)::value;
-    build_cells();
-    const int w = (M >= 2 * DIV + 1) ? DIV : 0;   // w = 0: one cell holds every atom
-    // cutoff^2 in cell units, with margins far above the FP32 rounding of the
-    // culling below (which can then only keep extra cells)
-    const float rcell2 = (float)(d.Rc2s * (double)M * (double)M * (1.0 + 1e-5) + 1e-5);
+    // the list: rebuilt when it is missing or an atom moved more than skin/2
+    const double moved = refresh();
+    if (listState != 2 && (listState == 0 || moved > d.trig2s)) build_list();
+    const bool useList = listState == 1;
     double e = 0.0;
     for (int i = threadIdx.x; i < N; i += BS) {
-      unsigned int ui = qx[i], vi = qy[i], wi = qz[i];
-      unsigned long long px = (unsigned long long)ui * M, py = (unsigned long long)vi * M,
-                         pz = (unsigned long long)wi * M;
-      int cx = (int)(px >> 32), cy = (int)(py >> 32), cz = (int)(pz >> 32);
-      // position of atom i inside its cell, in cell units
-      const float ox = (float)(unsigned int)px * 2.3283064e-10f, oy = (float)(unsigned int)py * 2.3283064e-10f,
-                  oz = (float)(unsigned int)pz * 2.3283064e-10f;
+      const unsigned int ui = qx[i], vi = qy[i], wi = qz[i];
       double4 ri = Rr[i];
       double fx = 0, fy = 0, fz = 0, ei = 0;
-      // cells are numbered x fastest, so the cells x-w..x+w of one (y, z) row
-      // are one contiguous slot range (two where the row wraps, visited in
-      // the order of increasing dx); w = 0 takes every atom at once
-      for (int dz = -w; dz <= w; dz++) {
-        int z = cz + dz; z = z < 0 ? z + M : (z >= M ? z - M : z);
-        float gz = fmaxf(dz > 0 ? dz - oz : (dz < 0 ? oz - dz - 1 : 0.0f), 0.0f);
-        for (int dy = -w; dy <= w; dy++) {
-          int y = cy + dy; y = y < 0 ? y + M : (y >= M ? y - M : y);
-          int base = M * (y + M * z), xa = cx - w, xb = cx + w;
-          if (w > 0) {   // rows and cells beyond the cutoff sphere hold no partner
-            float gy = fmaxf(dy > 0 ? dy - oy : (dy < 0 ? oy - dy - 1 : 0.0f), 0.0f);
-            float rem = rcell2 - gy * gy - gz * gz;
-            if (rem <= 0.0f) continue;
-            float ex = sqrtf(rem) + 1e-4f;
-            xa = cx + max((int)floorf(ox - ex), -w);
-            xb = cx + min((int)floorf(ox + ex), w);
-          }
-          int r0a, r0b, r1a = 0, r1b = -1;   // cell ranges [r0a, r0b] then [r1a, r1b]
-          if (xa < 0) { r0a = xa + M; r0b = M - 1; r1a = 0; r1b = xb; }
-          else if (xb >= M) { r0a = xa; r0b = M - 1; r1a = 0; r1b = xb - M; }
-          else { r0a = xa; r0b = xb; }
-          for (int part = 0; part < 2; part++) {
-            int ca = part == 0 ? r0a : r1a, cb = part == 0 ? r0b : r1b;
-            if (cb < ca) continue;
-            int t0 = (w == 0) ? 0 : cstart[base + ca], t1 = (w == 0) ? N : cstart[base + cb + 1];
-            for (int t = t0; t < t1; t++) {
-              int j = (w == 0) ? t : clist[t];
-              if (MIXED) {
-                // the atom itself (zero separation) is masked, not branched over
-                float ddx = (float)(int)(ui - qx[j]), ddy = (float)(int)(vi - qy[j]), ddz = (float)(int)(wi - qz[j]);
-                float r2 = ddx * ddx + ddy * ddy + ddz * ddz;
-                bool in = r2 < d.lo && j != i;
-                if (r2 >= d.lo && r2 < d.hi) {
-                  double4 rj = Rr[j];
-                  double ex = ri.x / L - rj.x / L, ey = ri.y / L - rj.y / L, ez = ri.z / L - rj.z / L;
-                  ex = ex - rint(ex); ey = ey - rint(ey); ez = ez - rint(ez);
-                  double q2;
-                  {
+      auto pair = [&](int j) {
+        if (MIXED) {
+          // the atom itself (zero separation) is masked, not branched over
+          float ddx = (float)(int)(ui - qx[j]), ddy = (float)(int)(vi - qy[j]), ddz = (float)(int)(wi - qz[j]);
+          float r2 = ddx * ddx + ddy * ddy + ddz * ddz;
+          bool in = r2 < d.lo && j != i;
+          if (r2 >= d.lo && r2 < d.hi) {
+            double4 rj = Rr[j];
+            double ex = ri.x / L - rj.x / L, ey = ri.y / L - rj.y / L, ez = ri.z / L - rj.z / L;
+            ex = ex - rint(ex); ey = ey - rint(ey); ez = ez - rint(ez);
+            double q2;
+            {
 #pragma clang fp contract(off)
-                    q2 = ex * ex + ey * ey + ez * ez;
-                  }
-                  in = q2 < d.Rc2s;
-                }
-                if (in) {
-                  float invR2 = d.K * __builtin_amdgcn_rcpf(r2);
-                  float sr2 = c2 * invR2, sr6 = sr2 * sr2 * sr2;
-                  float W = c1 * sr6 * (sr6 + sr6 - 1.0f);
-                  float g = W * invR2;
-                  fx += (double)(g * ddx);
-                  fy += (double)(g * ddy);
-                  fz += (double)(g * ddz);
-                  if (WANT_E) ei += (double)(c0 * sr6 * (sr6 - 1.0f));
-                }
-              } else {
-                if (j == i) continue;
-                double4 rj = Rr[j];
-                double ex = ri.x / L - rj.x / L, ey = ri.y / L - rj.y / L, ez = ri.z / L - rj.z / L;
-                ex = ex - rint(ex); ey = ey - rint(ey); ez = ez - rint(ez);
-                double q2;
-                {
-#pragma clang fp contract(off)
-                  q2 = ex * ex + ey * ey + ez * ez;
-                }
-                if (q2 < d.Rc2s) {
-                  double invR2 = 1.0 / (L * L * q2);
-                  double sr2 = invR2, sr6 = sr2 * sr2 * sr2, sr12 = sr6 * sr6;
-                  double W = 24.0 * (sr12 + sr12 - sr6);
-                  double g = W * invR2;
-                  fx += g * ex;
-                  fy += g * ey;
-                  fz += g * ez;
-                  if (WANT_E) ei += 4.0 * (sr12 - sr6);
-                }
-              }
+              q2 = ex * ex + ey * ey + ez * ez;
             }
+            in = q2 < d.Rc2s;
+          }
+          if (in) {
+            float invR2 = d.K * __builtin_amdgcn_rcpf(r2);
+            float sr2 = c2 * invR2, sr6 = sr2 * sr2 * sr2;
+            float W = c1 * sr6 * (sr6 + sr6 - 1.0f);
+            float g = W * invR2;
+            fx += (double)(g * ddx);
+            fy += (double)(g * ddy);
+            fz += (double)(g * ddz);
+            if (WANT_E) ei += (double)(c0 * sr6 * (sr6 - 1.0f));
+          }
+        } else {
+          if (j == i) return;
+          double4 rj = Rr[j];
+          double ex = ri.x / L - rj.x / L, ey = ri.y / L - rj.y / L, ez = ri.z / L - rj.z / L;
+          ex = ex - rint(ex); ey = ey - rint(ey); ez = ez - rint(ez);
+          double q2;
+          {
+#pragma clang fp contract(off)
+            q2 = ex * ex + ey * ey + ez * ez;
+          }
+          if (q2 < d.Rc2s) {
+            double invR2 = 1.0 / (L * L * q2);
+            double sr2 = invR2, sr6 = sr2 * sr2 * sr2, sr12 = sr6 * sr6;
+            double W = 24.0 * (sr12 + sr12 - sr6);
+            double g = W * invR2;
+            fx += g * ex;
+            fy += g * ey;
+            fz += g * ez;
+            if (WANT_E) ei += 4.0 * (sr12 - sr6);
           }
         }
+      };
+      if (useList) {
+        // four indices loaded before their pairs are evaluated, so that
+        // the loads overlap
+        const unsigned short* __restrict__ li = nlist + (size_t)d.off * cap + i;
+        const int n = nnb[d.off + i];
+        for (int k = 0; k < n; k += 4) {
+          int jb[4];
+#pragma unroll
+          for (int u = 0; u < 4; u++) jb[u] = k + u < n ? (int)li[(size_t)(k + u) * N] : -1;
+#pragma unroll
+          for (int u = 0; u < 4; u++)
+            if (jb[u] >= 0) pair(jb[u]);
+        }
+      } else {
+        // a list that would overflow its capacity (more neighbours within
+        // rc + skin than LJ atoms at close packing hold): every atom
+        for (int j = 0; j < N; j++) pair(j);
       }
       double sc = MIXED ? d.unit : L;
       Fr[i] = make_double4(sc * fx, sc * fy, sc * fz, 0.0);
@@ -259,6 +347,8 @@ k_hmc(const ReplicaDesc* __restrict__ desc, const int* __restrict__ order, doubl
     return WANT_E ? 0.5 * block_total<BS>(e, red) : 0.0;
   };
 
+  if (threadIdx.x == 0) listState = 0;   // every launch starts with a build
+  __syncthreads();
   if (mode == 0) {
     double u = forces(std::true_type());
     if (threadIdx.x == 0) U[r] = u;
@@ -345,6 +435,9 @@ struct EmDeeHMC {
   DBuf<double4> R, P, F, Rsave, Fsave;
   DBuf<double> U, lastDH;
   DBuf<long long> accepted;
+  int cap = 0;                   // Verlet-list entries per atom
+  DBuf<unsigned short> nlist, nnb;
+  DBuf<uint4> qbuild;
   int variant = -1;            // index into the variant table
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
@@ -353,7 +446,8 @@ struct EmDeeHMC {
 
 namespace {
 using HmcKernel = void (*)(const ReplicaDesc*, const int*, double4*, double4*, double4*, double4*, double4*, double*,
-                           int*, long long*, double*, double, double, int, uint64_t, int, int);
+                           int*, long long*, double*, double, double, int, uint64_t, int, int, unsigned short*,
+                           unsigned short*, uint4*, int);
 struct Variant { int bs, div; HmcKernel mixed, plain; };
 const Variant VARIANTS[] = {
   { 256, 1, k_hmc<256, 1, true>, k_hmc<256, 1, false> },
@@ -377,6 +471,8 @@ void set_cells(EmDeeHMC* h, int div) {
   for (ReplicaDesc& d : h->hdesc) {
     int M = std::max(1, (int)std::floor(div * d.L / h->rc));
     d.M = M >= 2 * div + 1 ? M : 1;
+    int Ml = std::max(1, (int)std::floor(div * d.L / (h->rc + HMC_SKIN)));
+    d.Ml = Ml >= 2 * div + 1 ? Ml : 1;
   }
   HIP_CHECK(hipMemcpy(h->desc.p, h->hdesc.data(), sizeof(ReplicaDesc) * h->nrep, hipMemcpyHostToDevice));
 }
@@ -386,7 +482,7 @@ void launch(EmDeeHMC* h, int v, int ncycles, int mode) {
   HmcKernel k = h->precision ? var.mixed : var.plain;
   hipLaunchKernelGGL(k, dim3(h->nrep), dim3(var.bs), 0, h->stream, h->desc.p, h->order.p, h->R.p, h->P.p, h->F.p,
                      h->Rsave.p, h->Fsave.p, h->U.p, h->cycle.p, h->accepted.p, h->lastDH.p, h->rc, h->dt, h->nsteps,
-                     h->seed, ncycles, mode);
+                     h->seed, ncycles, mode, h->nlist.p, h->nnb.p, h->qbuild.p, h->cap);
   launch_check("hmc");
 }
 }  // namespace
@@ -417,10 +513,22 @@ EmDeeHMC* EmDeeHMC_create(int nrep, const int* natoms, const double* box, const 
     d.hi = (float)(rc2u * (1.0 + band));
     d.Rc2s = rc2s;
     d.unit = box[r] / two32;
+    const double rl = rc + HMC_SKIN;
+    d.Rl2s = (rl * rl) / (box[r] * box[r]);
+    d.trig2s = (0.25 * HMC_SKIN * HMC_SKIN) / (box[r] * box[r]);
     h->offsets[r] = off;
     off += natoms[r];
   }
   h->total = off;
+  // list capacity: 1.6 times the mean count at the densest replica's
+  // density, plus room for fluctuations (a replica that overflows walks its
+  // cells instead)
+  {
+    double rho = 0.0, rl = rc + HMC_SKIN;
+    for (int r = 0; r < nrep; r++) rho = std::max(rho, natoms[r] / (box[r] * box[r] * box[r]));
+    int cap = (int)(1.6 * 4.18879020478639 * rl * rl * rl * rho) + 24;
+    h->cap = std::min(((cap + 7) / 8) * 8, HMC_NMAX);
+  }
   // largest replicas first: the dispatcher hands out workgroups in order
   std::vector<int> ord(nrep);
   std::iota(ord.begin(), ord.end(), 0);
@@ -428,6 +536,7 @@ EmDeeHMC* EmDeeHMC_create(int nrep, const int* natoms, const double* box, const 
   h->desc.alloc(nrep); h->order.alloc(nrep); h->cycle.alloc(nrep); h->U.alloc(nrep); h->lastDH.alloc(nrep);
   h->accepted.alloc(nrep);
   for (DBuf<double4>* b : { &h->R, &h->P, &h->F, &h->Rsave, &h->Fsave }) b->alloc(off);
+  h->nlist.alloc((size_t)off * h->cap); h->nnb.alloc(off); h->qbuild.alloc(off);
   std::vector<double4> R4(off);
   for (int i = 0; i < off; i++) R4[i] = make_double4(R[3 * i], R[3 * i + 1], R[3 * i + 2], 0.0);
   HIP_CHECK(hipMemcpy(h->R.p, R4.data(), sizeof(double4) * off, hipMemcpyHostToDevice));
@@ -540,6 +649,7 @@ void EmDeeHMC_destroy(EmDeeHMC* h) {
   if (h->graph) (void)hipGraphDestroy(h->graph);
   h->desc.release(); h->order.release(); h->cycle.release(); h->U.release(); h->lastDH.release(); h->accepted.release();
   for (DBuf<double4>* b : { &h->R, &h->P, &h->F, &h->Rsave, &h->Fsave }) b->release();
+  h->nlist.release(); h->nnb.release(); h->qbuild.release();
   (void)hipStreamDestroy(h->stream);
   delete h;
 }
