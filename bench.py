@@ -186,8 +186,9 @@ def cpu_baseline(R, L, steps, dt, threads):
 def hmc_bench(lib, nrep, cycles, warm, seed=1):
     """Batched HMC workload (512 LJ replicas of 256-8192 atoms, densities
     0.7-0.9, kT 0.8-2.0, rc 2.5, 10-step trajectories of dt 0.005, FP32 pair
-    arithmetic): graph replays of one cycle each, potential-energy series per
-    replica for the effective sample count."""
+    arithmetic): graph replays of one cycle each, the potential-energy series
+    of every replica (recorded on the device) for the effective sample
+    count.  At most 1024 cycles."""
     from emdee_amd.hmc import BatchedHMC, integrated_autocorrelation, replica_set
     N, L, T, Rs = replica_set(nrep, seed=seed)
     h = BatchedHMC(lib, N, L, T, Rs, dt=0.005, nsteps=10, seed=seed + 6, precision=1)
@@ -195,15 +196,17 @@ def hmc_bench(lib, nrep, cycles, warm, seed=1):
     h.run(launches=warm, cycles=1)
     h.synchronize()
     _, acc0 = h.energies()
-    series = []
+    # one graph launch per cycle, queued back to back; the potential energy
+    # of every cycle is recorded on the device and read after the timing
     t0 = time.perf_counter()
-    for _ in range(cycles):
-        h.run(launches=1, cycles=1)
-        U, acc = h.energies()
-        series.append(U)
+    h.run(launches=cycles, cycles=1)
+    h.synchronize()
     t = time.perf_counter() - t0
+    _, acc = h.energies()
+    series = h.history(cycles)
     h.close()
-    series = np.array(series)
+    if len(series) != cycles:
+        raise RuntimeError("energy history shorter than the timed cycles")
     taus = np.array([integrated_autocorrelation(series[:, r]) for r in range(nrep)])
     ess = float(np.sum(cycles / taus))
     accepted = int(np.sum(acc - acc0))

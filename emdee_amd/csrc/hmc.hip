@@ -13,7 +13,8 @@
 // Per replica (N <= 8192 atoms) the workgroup keeps, in LDS, the 32-bit
 // fixed-point coordinates of the current step, so all pair gathers of the
 // mixed form are LDS reads.  Pairs come from a Verlet list per replica in
-// global memory (16-bit indices, column-major): built from a cell list
+// global memory (16-bit indices, column-major; kept between launches with
+// its build coordinates): built from a cell list
 // (counting sort into cells of side >= (rc + skin)/DIV, stencil of
 // (2 DIV + 1)^3 cells) and rebuilt before a force evaluation whenever some
 // atom has moved more than skin/2 since the last build -- the rule of
@@ -48,6 +49,7 @@ namespace emdee {
 namespace {
 constexpr int HMC_NMAX = 8192;   // atoms per replica
 constexpr int HMC_CMAX = 6144;   // cells per replica
+constexpr int HMC_HIST = 1024;   // cycles of potential-energy history
 const double TWO_PI_D = 6.283185307179586;
 
 struct ReplicaDesc {
@@ -100,7 +102,8 @@ k_hmc(const ReplicaDesc* __restrict__ desc, const int* __restrict__ order, doubl
       double4* __restrict__ P, double4* __restrict__ F, double4* __restrict__ Rsave, double4* __restrict__ Fsave,
       double* __restrict__ U, int* __restrict__ cycle, long long* __restrict__ accepted,
       double* __restrict__ lastDH, double rc, double dt, int nsteps, uint64_t seed, int ncycles, int mode,
-      unsigned short* __restrict__ nlist, unsigned short* __restrict__ nnb, uint4* __restrict__ qbuild, int cap) {
+      unsigned short* __restrict__ nlist, unsigned short* __restrict__ nnb, uint4* __restrict__ qbuild, int cap,
+      int* __restrict__ lstate, double* __restrict__ Uhist, int histLen) {
   __shared__ unsigned int qx[HMC_NMAX], qy[HMC_NMAX], qz[HMC_NMAX];
   __shared__ unsigned short clist[HMC_NMAX];
   __shared__ int cstart[HMC_CMAX + 1];
@@ -347,11 +350,13 @@ k_hmc(const ReplicaDesc* __restrict__ desc, const int* __restrict__ order, doubl
     return WANT_E ? 0.5 * block_total<BS>(e, red) : 0.0;
   };
 
-  if (threadIdx.x == 0) listState = 0;   // every launch starts with a build
+  // the list and its build coordinates persist between launches (written
+  // together), so a list still valid for the current coordinates is kept
+  if (threadIdx.x == 0) listState = lstate[r];
   __syncthreads();
   if (mode == 0) {
     double u = forces(std::true_type());
-    if (threadIdx.x == 0) U[r] = u;
+    if (threadIdx.x == 0) { U[r] = u; lstate[r] = listState; }
     return;
   }
   const double sq = sqrt(d.kT);
@@ -412,9 +417,11 @@ k_hmc(const ReplicaDesc* __restrict__ desc, const int* __restrict__ order, doubl
       accepted[r] += accept ? 1 : 0;
       lastDH[r] = dH;
       cycle[r] = cycleNow;
+      Uhist[(size_t)(cnum % histLen) * gridDim.x + r] = Unow;   // the grid holds one workgroup per replica
     }
     __syncthreads();
   }
+  if (threadIdx.x == 0) lstate[r] = listState;
 }
 
 }  // namespace emdee
@@ -438,6 +445,8 @@ struct EmDeeHMC {
   int cap = 0;                   // Verlet-list entries per atom
   DBuf<unsigned short> nlist, nnb;
   DBuf<uint4> qbuild;
+  DBuf<int> lstate;              // per replica: 0 no list, 1 valid list, 2 overflow
+  DBuf<double> Uhist;            // potential energy after each of the last HMC_HIST cycles [cycle % HMC_HIST][replica]
   int variant = -1;            // index into the variant table
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
@@ -447,7 +456,7 @@ struct EmDeeHMC {
 namespace {
 using HmcKernel = void (*)(const ReplicaDesc*, const int*, double4*, double4*, double4*, double4*, double4*, double*,
                            int*, long long*, double*, double, double, int, uint64_t, int, int, unsigned short*,
-                           unsigned short*, uint4*, int);
+                           unsigned short*, uint4*, int, int*, double*, int);
 struct Variant { int bs, div; HmcKernel mixed, plain; };
 const Variant VARIANTS[] = {
   { 256, 1, k_hmc<256, 1, true>, k_hmc<256, 1, false> },
@@ -482,7 +491,8 @@ void launch(EmDeeHMC* h, int v, int ncycles, int mode) {
   HmcKernel k = h->precision ? var.mixed : var.plain;
   hipLaunchKernelGGL(k, dim3(h->nrep), dim3(var.bs), 0, h->stream, h->desc.p, h->order.p, h->R.p, h->P.p, h->F.p,
                      h->Rsave.p, h->Fsave.p, h->U.p, h->cycle.p, h->accepted.p, h->lastDH.p, h->rc, h->dt, h->nsteps,
-                     h->seed, ncycles, mode, h->nlist.p, h->nnb.p, h->qbuild.p, h->cap);
+                     h->seed, ncycles, mode, h->nlist.p, h->nnb.p, h->qbuild.p, h->cap, h->lstate.p, h->Uhist.p,
+                     HMC_HIST);
   launch_check("hmc");
 }
 }  // namespace
@@ -536,7 +546,8 @@ EmDeeHMC* EmDeeHMC_create(int nrep, const int* natoms, const double* box, const 
   h->desc.alloc(nrep); h->order.alloc(nrep); h->cycle.alloc(nrep); h->U.alloc(nrep); h->lastDH.alloc(nrep);
   h->accepted.alloc(nrep);
   for (DBuf<double4>* b : { &h->R, &h->P, &h->F, &h->Rsave, &h->Fsave }) b->alloc(off);
-  h->nlist.alloc((size_t)off * h->cap); h->nnb.alloc(off); h->qbuild.alloc(off);
+  h->nlist.alloc((size_t)off * h->cap); h->nnb.alloc(off); h->qbuild.alloc(off); h->lstate.alloc(nrep);
+  h->Uhist.alloc((size_t)HMC_HIST * nrep);
   std::vector<double4> R4(off);
   for (int i = 0; i < off; i++) R4[i] = make_double4(R[3 * i], R[3 * i + 1], R[3 * i + 2], 0.0);
   HIP_CHECK(hipMemcpy(h->R.p, R4.data(), sizeof(double4) * off, hipMemcpyHostToDevice));
@@ -544,6 +555,7 @@ EmDeeHMC* EmDeeHMC_create(int nrep, const int* natoms, const double* box, const 
   std::vector<int> zeroI(nrep, 0);
   std::vector<long long> zeroL(nrep, 0);
   HIP_CHECK(hipMemcpy(h->cycle.p, zeroI.data(), sizeof(int) * nrep, hipMemcpyHostToDevice));
+  HIP_CHECK(hipMemcpy(h->lstate.p, zeroI.data(), sizeof(int) * nrep, hipMemcpyHostToDevice));
   HIP_CHECK(hipMemcpy(h->accepted.p, zeroL.data(), sizeof(long long) * nrep, hipMemcpyHostToDevice));
   // default variant: the first that fits; forces of the initial coordinates
   for (int v = 0; v < NVARIANTS; v++)
@@ -635,6 +647,25 @@ void EmDeeHMC_get(EmDeeHMC* h, double* R, double* U, long long* accepted, double
   if (lastDH) HIP_CHECK(hipMemcpy(lastDH, h->lastDH.p, sizeof(double) * h->nrep, hipMemcpyDeviceToHost));
 }
 
+// Potential energies after each of the last `cycles` cycles (at most 1024
+// and at most the cycles run), oldest first: U[c * nrep + replica].  Returns
+// the number of cycles copied.  Every replica runs the same cycles, so the
+// first replica's cycle counter numbers them.
+int EmDeeHMC_history(EmDeeHMC* h, int cycles, double* U) {
+  HIP_CHECK(hipStreamSynchronize(h->stream));
+  int done = 0;
+  HIP_CHECK(hipMemcpy(&done, h->cycle.p, sizeof(int), hipMemcpyDeviceToHost));
+  int n = std::max(0, std::min({ cycles, HMC_HIST, done }));
+  if (n == 0 || !U) return n;
+  std::vector<double> all((size_t)HMC_HIST * h->nrep);
+  HIP_CHECK(hipMemcpy(all.data(), h->Uhist.p, sizeof(double) * all.size(), hipMemcpyDeviceToHost));
+  for (int c = 0; c < n; c++) {
+    int row = (done - n + c) % HMC_HIST;
+    std::copy(all.begin() + (size_t)row * h->nrep, all.begin() + (size_t)(row + 1) * h->nrep, U + (size_t)c * h->nrep);
+  }
+  return n;
+}
+
 // Chosen variant {block size, cell division} and the autotuner's timings.
 void EmDeeHMC_variant(EmDeeHMC* h, int* blockSize, int* cellDivision, double* tuneMs) {
   if (blockSize) *blockSize = VARIANTS[h->variant].bs;
@@ -649,7 +680,7 @@ void EmDeeHMC_destroy(EmDeeHMC* h) {
   if (h->graph) (void)hipGraphDestroy(h->graph);
   h->desc.release(); h->order.release(); h->cycle.release(); h->U.release(); h->lastDH.release(); h->accepted.release();
   for (DBuf<double4>* b : { &h->R, &h->P, &h->F, &h->Rsave, &h->Fsave }) b->release();
-  h->nlist.release(); h->nnb.release(); h->qbuild.release();
+  h->nlist.release(); h->nnb.release(); h->qbuild.release(); h->lstate.release(); h->Uhist.release();
   (void)hipStreamDestroy(h->stream);
   delete h;
 }

@@ -119,6 +119,7 @@ GPU_SIGNATURES = {
     "EmDeeHMC_run": (None, [c_void_p, c_int, c_int]),
     "EmDeeHMC_synchronize": (None, [c_void_p]),
     "EmDeeHMC_get": (None, [c_void_p, _dp, _dp, POINTER(ctypes.c_longlong), _dp]),
+    "EmDeeHMC_history": (c_int, [c_void_p, c_int, _dp]),
     "EmDeeHMC_destroy": (None, [c_void_p]),
     "EmDeeHMC_momenta": (None, [ctypes.c_ulonglong, c_int, c_int, c_int, c_double, _dp]),
     "EmDeeHMC_uniform": (c_double, [ctypes.c_ulonglong, c_int, c_int]),

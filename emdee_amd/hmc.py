@@ -80,6 +80,13 @@ class BatchedHMC:
                               acc.ctypes.data_as(ctypes.POINTER(ctypes.c_longlong)), None)
         return U, acc
 
+    def history(self, cycles):
+        """Potential energies after each of the last `cycles` cycles, recorded
+        on the device: array (cycles, replicas), oldest first."""
+        U = np.zeros((cycles, len(self.N)))
+        n = self.lib.EmDeeHMC_history(self.h, cycles, U.ctypes.data_as(_dp))
+        return U[:n]
+
     def close(self):
         if self.h:
             self.lib.EmDeeHMC_destroy(self.h)

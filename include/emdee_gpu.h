@@ -58,6 +58,11 @@ void EmDeeHMC_variant( EmDeeHMC* h, int* blockSize, int* cellDivision, double* t
 void EmDeeHMC_run( EmDeeHMC* h, int launches, int cycles );
 void EmDeeHMC_synchronize( EmDeeHMC* h );
 void EmDeeHMC_get( EmDeeHMC* h, double* R, double* U, long long* accepted, double* lastDH );
+/* Potential energies after each of the last `cycles` cycles (at most 1024
+   and at most the cycles run), oldest first, U[c * nrep + replica], recorded
+   on the device, so a host can run many cycles without reading back after
+   each; returns the number of cycles copied. */
+int  EmDeeHMC_history( EmDeeHMC* h, int cycles, double* U );
 void EmDeeHMC_destroy( EmDeeHMC* h );
 /* The run's random numbers: momenta (3n, unit mass) and Metropolis uniform
    of a (replica, cycle). */

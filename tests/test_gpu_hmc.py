@@ -62,6 +62,26 @@ def test_graph_replays_and_tuning(gpu):
         assert np.max(np.abs(x - y)) < 1e-10
 
 
+def test_energy_history_matches_per_cycle_reads(gpu):
+    """The potential energies recorded on the device after every cycle equal
+    those read back after each cycle (same run, same cycles)."""
+    N, L, T, Rs = replicas()
+    a = BatchedHMC(gpu, N, L, T, Rs, dt=DT, nsteps=NSTEPS, seed=SEED, precision=1)
+    per_cycle = []
+    for _ in range(5):
+        a.run(launches=1, cycles=1)
+        per_cycle.append(a.energies()[0])
+    hist = a.history(5)
+    a.close()
+    assert hist.shape == (5, len(N))
+    assert np.array_equal(hist, np.array(per_cycle))
+    b = BatchedHMC(gpu, N, L, T, Rs, dt=DT, nsteps=NSTEPS, seed=SEED, precision=1)
+    b.run(launches=5, cycles=1)            # queued back to back, read once
+    assert np.allclose(b.history(5), hist, rtol=1e-12)
+    assert len(b.history(9)) == 5          # no more than the cycles run
+    b.close()
+
+
 def test_full_size_replica_set(gpu):
     """The 512-replica set (256 to 8192 atoms): every replica advances and
     the acceptance rate is that of a well-tuned HMC."""
