@@ -478,21 +478,25 @@ __global__ void __launch_bounds__(256) k_sg_mlp(float* __restrict__ eat, float* 
 // two streams of 32 lanes take the even and the odd near entries; lane g of
 // a stream owns Gaussian g and sums its share of
 // dE/dr_ij = sum_g (dE/dD_i[t_j][g] + dE/dD_j[t_i][g]) dG_g/dr (the pair
-// moves both atoms' descriptors) times -r_ij / r.  The records of the next
-// four entries are loaded while the current four are evaluated.  The 64
-// lanes' sums are combined once per atom by a fixed butterfly.  The list is
-// full, so each atom sums its own force (no atomics).  Virial: -1/2 dE/dr r
-// per entry.  The near list carries the unit vector r_ij / r, so no division
-// per pair.
+// moves both atoms' descriptors) times -r_ij / r.  The wave first copies a
+// chunk of up to 128 near entries into LDS with coalesced loads (one entry
+// per lane), padded to a multiple of eight with entries of zero cutoff
+// factors (j = i), which contribute exactly zero; the streams then read the
+// entries by broadcast from LDS and gather only dE/dD_j.  The 64 lanes' sums
+// are combined once per atom by a fixed butterfly.  The list is full, so
+// each atom sums its own force (no atomics).  Virial: -1/2 dE/dr r per entry.
+// The near list carries the unit vector r_ij / r, so no division per pair.
 template <int NT>
 __global__ void __launch_bounds__(BLOCK) k_sg_force(double* __restrict__ F, double* __restrict__ wpart,
                                                     const float* __restrict__ dEdD, const float4* __restrict__ nearE,
                                                     const float4* __restrict__ nearG, const int* __restrict__ nearN,
                                                     const int* __restrict__ type, SurrogateParams sp, int N,
                                                     int Npad) {
-  constexpr int DIN = NT * SG_NB, AW = BLOCK / 64;
+  constexpr int DIN = NT * SG_NB, AW = BLOCK / 64, CH = 128;
+  __shared__ float4 stE[AW][CH], stG[AW][CH];
   __shared__ double wred[AW];
-  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6, g = lane & (SG_G - 1), st = lane >> 5;
+  const int lane = threadIdx.x & 63, g = lane & (SG_G - 1), st = lane >> 5;
+  const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const int i = blockIdx.x * AW + wv;
   float fx = 0.0f, fy = 0.0f, fz = 0.0f, w = 0.0f;
   if (i < N) {
@@ -503,35 +507,53 @@ __global__ void __launch_bounds__(BLOCK) k_sg_force(double* __restrict__ F, doub
     for (int t = 0; t < NT; t++) ai[t] = dEdD[(size_t)i * DIN + t * SG_NB + g];
     const float4* E = nearE + (size_t)i * SG_NEAR;
     const float4* G = nearG + (size_t)i * SG_NEAR;
-    float4 ev[4], gv[4];
-    auto fetch = [&](int e0) {
+    const float4 padE = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    const float4 padG = make_float4(0.0f, 0.0f, __int_as_float(i), __int_as_float(ti));
+    // byte offset of dE/dD_j[t_i][g] is 4 (j DIN + t_i SG_NB + g)
+    const char* dj = reinterpret_cast<const char*>(dEdD);
+    const unsigned int djo = 4u * (unsigned int)(ti * SG_NB + g);
+    float4* sE = stE[wv];
+    float4* sG = stG[wv];
+    for (int c0 = 0; c0 < cnt; c0 += CH) {
+      const int len = min(cnt - c0, CH);
+      // the previous chunk's reads are done before it is overwritten (LDS
+      // operations of one wave complete in order)
+      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+      __builtin_amdgcn_wave_barrier();
 #pragma unroll
-      for (int u = 0; u < 4; u++) {
-        int e = min(e0 + 2 * u, cnt - 1);
-        ev[u] = E[e];
-        gv[u] = G[e];
+      for (int h = 0; h < CH / 64; h++) {
+        const int e = h * 64 + lane, ec = c0 + min(e, len - 1);
+        float4 ve = E[ec], vg = G[ec];
+        if (e >= len) { ve = padE; vg = padG; }
+        sE[e] = ve;
+        sG[e] = vg;
       }
-    };
-    if (cnt > 0) fetch(st);
-    for (int e0 = st; e0 < cnt; e0 += 8) {
-      float aj[4];
-      float4 ec[4] = { ev[0], ev[1], ev[2], ev[3] }, gc[4] = { gv[0], gv[1], gv[2], gv[3] };
+      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+      __builtin_amdgcn_wave_barrier();
+      for (int k0 = 0; k0 < len; k0 += 8) {
+        float4 ec[4], gc[4];
+        float aj[4];
 #pragma unroll
-      for (int u = 0; u < 4; u++) aj[u] = dEdD[(size_t)__float_as_int(gc[u].z) * DIN + ti * SG_NB + g];
-      if (e0 + 8 < cnt) fetch(e0 + 8);
+        for (int u = 0; u < 4; u++) {
+          ec[u] = sE[k0 + 2 * u + st];
+          gc[u] = sG[k0 + 2 * u + st];
+        }
 #pragma unroll
-      for (int u = 0; u < 4; u++) {
-        if (e0 + 2 * u >= cnt) break;
-        int tj = __float_as_int(gc[u].w);
-        float sel = ai[0];
+        for (int u = 0; u < 4; u++)
+          aj[u] = *reinterpret_cast<const float*>(dj + (4u * DIN * (unsigned int)__float_as_int(gc[u].z) + djo));
 #pragma unroll
-        for (int t = 1; t < NT; t++) sel = (tj == t) ? ai[t] : sel;
-        float du = ec[u].w - mu, gauss = __builtin_amdgcn_exp2f(k2 * du * du);
-        float dEdr = (sel + aj[u]) * (gauss * (gc[u].y - eta2 * du * gc[u].x));
-        fx -= dEdr * ec[u].x;
-        fy -= dEdr * ec[u].y;
-        fz -= dEdr * ec[u].z;
-        w -= dEdr * ec[u].w;
+        for (int u = 0; u < 4; u++) {
+          int tj = __float_as_int(gc[u].w);
+          float sel = ai[0];
+#pragma unroll
+          for (int t = 1; t < NT; t++) sel = (tj == t) ? ai[t] : sel;
+          float du = ec[u].w - mu, gauss = __builtin_amdgcn_exp2f(k2 * du * du);
+          float dEdr = (sel + aj[u]) * (gauss * (gc[u].y - eta2 * du * gc[u].x));
+          fx -= dEdr * ec[u].x;
+          fy -= dEdr * ec[u].y;
+          fz -= dEdr * ec[u].z;
+          w -= dEdr * ec[u].w;
+        }
       }
     }
   }
@@ -776,6 +798,8 @@ static void check_context(const EmDeeSurrogate* s, const Device* d, const Surrog
   if (s->sp.nt != d->ntypes) fatal(task, "the model and the system have different numbers of types");
   if ((double)s->sp.rc > c.Rc) fatal(task, "the descriptor cutoff exceeds the system's cutoff");
   if (!d->built) fatal(task, "the system has no neighbor list yet");
+  // the force kernel gathers dE/dD with 32-bit byte offsets
+  if ((double)d->Npad * s->sp.din * sizeof(float) >= 4294967296.0) fatal(task, "too many atoms for the model's kernels");
 }
 
 double surrogate_cutoff(const EmDeeSurrogate* s) { return (double)s->sp.rc; }
