@@ -227,13 +227,15 @@ constexpr int SG_NEAR = 256;
 constexpr int SG_G = 32;   // lanes per Gaussian set: lane g of a stream owns Gaussian g
 
 // Descriptors, one wavefront per atom.  Phase A: the 64 lanes test the
-// atom's compact-list entries (four index loads, then four position gathers,
-// in flight together) and compact those within rc, in list order (ballot
-// prefix), into LDS and into the near list.  Phase B: two streams of 32
-// lanes take the even and the odd near entries; lane g of a stream owns
-// Gaussian g and sums it over its entries (broadcast LDS reads), one
-// accumulator per neighbour species, so every evaluated Gaussian is of an
-// in-range pair; the two streams' sums are added in a fixed order.
+// atom's compact-list entries (four consecutive indices per lane in one load,
+// then four position gathers in flight together) and compact those within
+// rc, in a fixed order (ballot prefix), into LDS and into the near list.  Each LDS record holds r and the
+// cutoff factor once per neighbour species (zero for the other species), and
+// the records are padded with zeros to a multiple of eight.  Phase B: two
+// streams of 32 lanes take the even and the odd records; lane g of a stream
+// owns Gaussian g and adds it, times each species' factor, to that species'
+// sum (broadcast LDS reads; every evaluated Gaussian is of an in-range pair
+// or a zero-weight pad); the two streams' sums are added in a fixed order.
 // Features are written normalised (mean != null) or raw.
 template <int NT>
 __global__ void __launch_bounds__(BLOCK) k_sg_desc(float* __restrict__ X, float4* __restrict__ nearE,
@@ -242,26 +244,46 @@ __global__ void __launch_bounds__(BLOCK) k_sg_desc(float* __restrict__ X, float4
                                                    const int* __restrict__ cN, const float4* __restrict__ Rf,
                                                    const float* __restrict__ mean, const float* __restrict__ sd,
                                                    SurrogateParams sp, float L, int N, int cap) {
-  constexpr int DIN = NT * SG_NB, AW = BLOCK / 64;
-  __shared__ float2 ent[AW][SG_NEAR];
-  __shared__ int entT[AW][SG_NEAR];
-  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  constexpr int DIN = NT * SG_NB, AW = BLOCK / 64, SR = NT < 4 ? 4 : 8, SQ = SR / 4;
+  __shared__ float4 ent[AW][SG_NEAR * SQ];
+  const int lane = threadIdx.x & 63;
+  const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const int i = blockIdx.x * AW + wv;
-  const bool valid = i < N;
+  if (i >= N) return;   // uniform over the wave; waves share no LDS and no barrier
+  float4* rec = ent[wv];
   const float invL = 1.0f / L, rc2 = sp.rc * sp.rc, kpi = PI_F / sp.rc, k2 = -sp.eta * 1.4426950408889634f;
-  const int n = valid ? cN[i] : 0;
-  const float4 ri = valid ? Rf[i] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+  const int n = cN[i];
+  const float4 ri = Rf[i];
+  const int* __restrict__ row = cJ + (size_t)i * cap;
+  auto put = [&](int pos, float r, float fc, int tj) {
+    float v[SR];
+#pragma unroll
+    for (int k = 0; k < SR; k++) v[k] = 0.0f;
+    v[0] = r;
+#pragma unroll
+    for (int t = 0; t < NT; t++) v[1 + t] = (t == tj) ? fc : 0.0f;
+#pragma unroll
+    for (int q = 0; q < SQ; q++) rec[pos * SQ + q] = make_float4(v[4 * q], v[4 * q + 1], v[4 * q + 2], v[4 * q + 3]);
+  };
   int cnt = 0;
   for (int c0 = 0; c0 < n; c0 += 64 * 4) {
     int j[4];
     float4 rj[4];
-#pragma unroll
-    for (int u = 0; u < 4; u++) {
-      int c = c0 + lane + 64 * u;
-      j[u] = c < n ? cJ[(size_t)i * cap + c] : -1;
+    {
+      // lane takes entries cb..cb+3 with one 16-byte load (rows hold a
+      // multiple of eight entries); a lane past the row's end reads its last
+      // four, all beyond n
+      const int cb = c0 + 4 * lane;
+      const int4 jv = *reinterpret_cast<const int4*>(row + min(cb, cap - 4));
+      j[0] = cb < n ? jv.x : -1;
+      j[1] = cb + 1 < n ? jv.y : -1;
+      j[2] = cb + 2 < n ? jv.z : -1;
+      j[3] = cb + 3 < n ? jv.w : -1;
     }
 #pragma unroll
-    for (int u = 0; u < 4; u++) rj[u] = j[u] >= 0 ? Rf[j[u]] : ri;
+    for (int u = 0; u < 4; u++) rj[u] = Rf[j[u] >= 0 ? j[u] : i];
+    // all four gathers are issued before the first is used
+    __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
     for (int u = 0; u < 4; u++) {
       float dx = sg_wrap(ri.x - rj[u].x, L, invL), dy = sg_wrap(ri.y - rj[u].y, L, invL),
@@ -271,47 +293,52 @@ __global__ void __launch_bounds__(BLOCK) k_sg_desc(float* __restrict__ X, float4
       unsigned long long m = __ballot(in);
       int pos = cnt + __popcll(m & ((1ULL << lane) - 1ULL));
       if (in && pos < SG_NEAR) {
-        float r = sqrtf(r2), fc = 0.5f * (__cosf(kpi * r) + 1.0f), dfc = -0.5f * kpi * __sinf(kpi * r);
-        float ir = 1.0f / r;
+        float ir = __builtin_amdgcn_rsqf(r2), r = r2 * ir;
+        float fc = 0.5f * (__cosf(kpi * r) + 1.0f), dfc = -0.5f * kpi * __sinf(kpi * r);
         int tj = __float_as_int(rj[u].w);
-        ent[wv][pos] = make_float2(r, fc);
-        entT[wv][pos] = tj;
+        put(pos, r, fc, tj);
         nearE[(size_t)i * SG_NEAR + pos] = make_float4(dx * ir, dy * ir, dz * ir, r);
         nearG[(size_t)i * SG_NEAR + pos] = make_float4(fc, dfc, __int_as_float(j[u]), __int_as_float(tj));
       }
       cnt += __popcll(m);
     }
   }
-  if (valid && lane == 0) {
+  if (lane == 0) {
     nearN[i] = min(cnt, SG_NEAR);
     if (cnt > SG_NEAR) atomicMax(over, cnt);
   }
   cnt = min(cnt, SG_NEAR);
-  __syncthreads();
+  const int cpad = (cnt + 7) & ~7;
+  if (cnt + lane < cpad) put(cnt + lane, 0.0f, 0.0f, -1);
+  // the records are read by other lanes of this wave (LDS operations of one
+  // wave complete in order)
+  __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+  __builtin_amdgcn_wave_barrier();
   const int g = lane & (SG_G - 1), st = lane >> 5;
   float acc[NT];
 #pragma unroll
   for (int t = 0; t < NT; t++) acc[t] = 0.0f;
   const float mu = sp.mu0 + g * sp.dmu;
-  for (int e0 = st; e0 < cnt; e0 += 8) {
-    float2 v[4];
-    int tj[4];
+  for (int e0 = 0; e0 < cpad; e0 += 8) {
+    float v[4][SR];
 #pragma unroll
     for (int u = 0; u < 4; u++) {
-      int e = e0 + 2 * u;
-      v[u] = ent[wv][min(e, cnt - 1)];
-      tj[u] = e < cnt ? entT[wv][e] : -1;
+#pragma unroll
+      for (int q = 0; q < SQ; q++) {
+        const float4 w = rec[(e0 + 2 * u + st) * SQ + q];
+        v[u][4 * q] = w.x; v[u][4 * q + 1] = w.y; v[u][4 * q + 2] = w.z; v[u][4 * q + 3] = w.w;
+      }
     }
 #pragma unroll
     for (int u = 0; u < 4; u++) {
-      float du = v[u].x - mu, gv = __builtin_amdgcn_exp2f(k2 * du * du) * v[u].y;
+      float du = v[u][0] - mu, gv = __builtin_amdgcn_exp2f(k2 * du * du);
 #pragma unroll
-      for (int t = 0; t < NT; t++) acc[t] += (t == tj[u]) ? gv : 0.0f;
+      for (int t = 0; t < NT; t++) acc[t] += gv * v[u][1 + t];
     }
   }
 #pragma unroll
   for (int t = 0; t < NT; t++) acc[t] += __shfl_xor(acc[t], SG_G, 64);
-  if (!valid || st != 0) return;
+  if (st != 0) return;
 #pragma unroll
   for (int t = 0; t < NT; t++) {
     int f = t * SG_NB + g;
