@@ -32,6 +32,15 @@
 namespace emdee {
 
 typedef short bf16x4 __attribute__((ext_vector_type(4)));
+typedef float f32x2v __attribute__((ext_vector_type(2)));
+typedef __bf16 bf16x2v __attribute__((ext_vector_type(2)));
+typedef unsigned int u32x2v __attribute__((ext_vector_type(2)));
+// four floats to bf16 (round to nearest even), two hardware conversions
+__device__ inline bf16x4 bf16_4(float a, float b, float c, float d) {
+  const bf16x2v lo = __builtin_convertvector((f32x2v){ a, b }, bf16x2v);
+  const bf16x2v hi = __builtin_convertvector((f32x2v){ c, d }, bf16x2v);
+  return __builtin_bit_cast(bf16x4, ((u32x2v){ __builtin_bit_cast(unsigned int, lo), __builtin_bit_cast(unsigned int, hi) }));
+}
 typedef float f32x4 __attribute__((ext_vector_type(4)));
 
 namespace {
@@ -48,6 +57,8 @@ __host__ __device__ inline unsigned short f2bf(float f) {
   return (unsigned short)(u >> 16);
 }
 __device__ inline float silu(float z) { return z / (1.0f + __expf(-z)); }
+// logistic sigmoid with the hardware reciprocal (network inference)
+__device__ inline float sigm_fast(float z) { return __builtin_amdgcn_rcpf(1.0f + __expf(-z)); }
 __device__ inline float dsilu(float z) {
   float s = 1.0f / (1.0f + __expf(-z));
   return s * (1.0f + z * (1.0f - s));
@@ -419,10 +430,13 @@ __global__ void __launch_bounds__(256) k_sg_mlp(float* __restrict__ eat, float* 
   for (int q = 0; q < KT; q++) {
     float4 v = slot >= 0 ? *reinterpret_cast<const float4*>(X + (size_t)slot * DIN + 16 * q + 4 * g)
                          : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-    xb[q] = bf16x4{ (short)f2bf(v.x), (short)f2bf(v.y), (short)f2bf(v.z), (short)f2bf(v.w) };
+    xb[q] = bf16_4(v.x, v.y, v.z, v.w);
   }
-  f32x4 z1[HT], z2[HT];
+  // d1, d2: SiLU'(Z1), SiLU'(Z2), formed with the forward activations from
+  // one sigmoid per value
+  f32x4 d1[HT], d2[HT];
   bf16x4 hb[HT], gb[HT];
+  float part = 0.0f;
 #pragma unroll
   for (int m = 0; m < HT; m++) {
     f32x4 acc = f32x4{ 0.0f, 0.0f, 0.0f, 0.0f };
@@ -431,12 +445,14 @@ __global__ void __launch_bounds__(256) k_sg_mlp(float* __restrict__ eat, float* 
       bf16x4 a = *reinterpret_cast<const bf16x4*>(lw + Lay::W1T + (16 * m + col) * Lay::S1 + 16 * q + 4 * g);
       acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, xb[q], acc, 0, 0, 0);
     }
+    float h[4];
 #pragma unroll
     for (int r = 0; r < 4; r++) {
-      float z = acc[r] + p[pl.b1() + 16 * m + 4 * g + r];
-      z1[m][r] = z;
-      hb[m][r] = (short)f2bf(silu(z));
+      const float z = acc[r] + p[pl.b1() + 16 * m + 4 * g + r], sg = sigm_fast(z);
+      h[r] = z * sg;
+      d1[m][r] = sg * (1.0f + z * (1.0f - sg));
     }
+    hb[m] = bf16_4(h[0], h[1], h[2], h[3]);
   }
 #pragma unroll
   for (int m = 0; m < HT; m++) {
@@ -447,21 +463,21 @@ __global__ void __launch_bounds__(256) k_sg_mlp(float* __restrict__ eat, float* 
       acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, hb[q], acc, 0, 0, 0);
     }
 #pragma unroll
-    for (int r = 0; r < 4; r++) z2[m][r] = acc[r] + p[pl.b2() + 16 * m + 4 * g + r];
+    for (int r = 0; r < 4; r++) {
+      const float z = acc[r] + p[pl.b2() + 16 * m + 4 * g + r], sg = sigm_fast(z);
+      part += p[pl.w3() + 16 * m + 4 * g + r] * (z * sg);
+      d2[m][r] = sg * (1.0f + z * (1.0f - sg));
+    }
   }
-  float part = 0.0f;
-#pragma unroll
-  for (int m = 0; m < HT; m++)
-#pragma unroll
-    for (int r = 0; r < 4; r++) part += p[pl.w3() + 16 * m + 4 * g + r] * silu(z2[m][r]);
   part += __shfl_xor(part, 16, 64);
   part += __shfl_xor(part, 32, 64);
   const float e = nrm.ystd[s] * (part + p[pl.b3()]) + nrm.ymean[s];
   const float seed = slot >= 0 ? nrm.ystd[s] : 0.0f;
 #pragma unroll
-  for (int m = 0; m < HT; m++)
-#pragma unroll
-    for (int r = 0; r < 4; r++) gb[m][r] = (short)f2bf(seed * p[pl.w3() + 16 * m + 4 * g + r] * dsilu(z2[m][r]));
+  for (int m = 0; m < HT; m++) {
+    const float* w3 = p + pl.w3() + 16 * m + 4 * g;
+    gb[m] = bf16_4(seed * w3[0] * d2[m][0], seed * w3[1] * d2[m][1], seed * w3[2] * d2[m][2], seed * w3[3] * d2[m][3]);
+  }
   // G1 = (W2 G2) * SiLU'(Z1), into hb
 #pragma unroll
   for (int m = 0; m < HT; m++) {
@@ -471,8 +487,7 @@ __global__ void __launch_bounds__(256) k_sg_mlp(float* __restrict__ eat, float* 
       bf16x4 a = *reinterpret_cast<const bf16x4*>(lw + Lay::W2 + (16 * m + col) * Lay::SH + 16 * q + 4 * g);
       acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, gb[q], acc, 0, 0, 0);
     }
-#pragma unroll
-    for (int r = 0; r < 4; r++) hb[m][r] = (short)f2bf(acc[r] * dsilu(z1[m][r]));
+    hb[m] = bf16_4(acc[0] * d1[m][0], acc[1] * d1[m][1], acc[2] * d1[m][2], acc[3] * d1[m][3]);
   }
   // dE/dX = W1 G1
   f32x4 dx[KT];
@@ -489,7 +504,9 @@ __global__ void __launch_bounds__(256) k_sg_mlp(float* __restrict__ eat, float* 
 #pragma unroll
     for (int m = 0; m < KT; m++) {
       int f = 16 * m + 4 * g;
-      float4 v = make_float4(dx[m][0] / xstd[f], dx[m][1] / xstd[f + 1], dx[m][2] / xstd[f + 2], dx[m][3] / xstd[f + 3]);
+      const float4 sd = *reinterpret_cast<const float4*>(xstd + f);
+      float4 v = make_float4(dx[m][0] * __builtin_amdgcn_rcpf(sd.x), dx[m][1] * __builtin_amdgcn_rcpf(sd.y),
+                             dx[m][2] * __builtin_amdgcn_rcpf(sd.z), dx[m][3] * __builtin_amdgcn_rcpf(sd.w));
       *reinterpret_cast<float4*>(dEdD + (size_t)slot * DIN + f) = v;
     }
     if (g == 0) eat[slot] = e;
