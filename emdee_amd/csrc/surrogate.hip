@@ -386,50 +386,75 @@ __global__ void __launch_bounds__(256) k_sg_mlp(float* __restrict__ eat, float* 
   constexpr int DIN = NT * SG_NB, KT = DIN / 16, HT = SG_H / 16;
   using Lay = SgLds<DIN>;
   __shared__ unsigned short lw[Lay::SIZE];
+  __shared__ float lb[3 * SG_H];   // b1, b2, w3
   __shared__ double wsum[4];
   const int s = grpSpecies[blockIdx.x];
   if (s < 0) {   // uniform over the block
     if (threadIdx.x == 0) epart[blockIdx.x] = 0.0;
     return;
   }
-  // stage the species' weights: 16-byte chunks into padded rows, eight
-  // chunks per thread loaded before any is stored, so that the loads overlap
-  {
-    auto stage = [&](const unsigned short* src, int rows, int cols, int dst, int stride) {
-      const int per = cols / 8, total = rows * per;
-      for (int c0 = 0; c0 < total; c0 += 8 * 256) {
-        uint4 v[8];
-#pragma unroll
-        for (int u = 0; u < 8; u++) {
-          int c = c0 + u * 256 + threadIdx.x;
-          if (c < total) v[u] = *reinterpret_cast<const uint4*>(src + (size_t)(c / per) * cols + 8 * (c % per));
-        }
-#pragma unroll
-        for (int u = 0; u < 8; u++) {
-          int c = c0 + u * 256 + threadIdx.x;
-          if (c < total) {
-            int r = c / per, k = 8 * (c % per);
-            *reinterpret_cast<uint2*>(lw + dst + r * stride + k) = make_uint2(v[u].x, v[u].y);
-            *reinterpret_cast<uint2*>(lw + dst + r * stride + k + 4) = make_uint2(v[u].z, v[u].w);
-          }
-        }
-      }
-    };
-    stage(w1t + (size_t)s * DIN * SG_H, SG_H, DIN, Lay::W1T, Lay::S1);
-    stage(w2t + (size_t)s * SG_H * SG_H, SG_H, SG_H, Lay::W2T, Lay::SH);
-    stage(w2 + (size_t)s * SG_H * SG_H, SG_H, SG_H, Lay::W2, Lay::SH);
-    stage(w1 + (size_t)s * DIN * SG_H, DIN, SG_H, Lay::W1, Lay::SH);
-  }
-  __syncthreads();
+  // Loads in the order of use, since a wait for one load also waits for
+  // every older one: W1^T (16-byte chunks), the biases b1, b2 and output
+  // weights w3, the tile's slots and features, then W2^T, W2 and W1.  Each
+  // matrix is written into its padded LDS rows just before the product that
+  // reads it, so the later matrices arrive meanwhile.
+  constexpr int C1 = SG_H * DIN / 8 / 256, C2 = SG_H * SG_H / 8 / 256;
+  static_assert(C1 * 8 * 256 == SG_H * DIN && C2 * 8 * 256 == SG_H * SG_H, "whole chunks per thread");
+  static_assert(SG_H <= 256, "one bias per thread");
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6, col = lane & 15, g = lane >> 4;
-  const int slot = rowSlot[blockIdx.x * 64 + wv * 16 + col];
   const ParamLayout pl{ DIN, SG_H };
   const float* p = params + (size_t)s * P;
+  uint4 cw1t[C1], cw2t[C2], cw2[C2], cw1[C1];
+  float bv[3];
+  const uint4* s1t = reinterpret_cast<const uint4*>(w1t + (size_t)s * DIN * SG_H);
+#pragma unroll
+  for (int u = 0; u < C1; u++) cw1t[u] = s1t[u * 256 + threadIdx.x];
+  if (threadIdx.x < SG_H) {
+    bv[0] = p[pl.b1() + threadIdx.x];
+    bv[1] = p[pl.b2() + threadIdx.x];
+    bv[2] = p[pl.w3() + threadIdx.x];
+  }
+  const int slot = rowSlot[blockIdx.x * 64 + wv * 16 + col];
+  float4 xv[KT];
+#pragma unroll
+  for (int q = 0; q < KT; q++)   // unconditional (slot 0 for an empty row), zeroed when converted
+    xv[q] = *reinterpret_cast<const float4*>(X + (size_t)max(slot, 0) * DIN + 16 * q + 4 * g);
+  __builtin_amdgcn_sched_barrier(0);   // keep the order of the loads
+  {
+    const uint4* s2t = reinterpret_cast<const uint4*>(w2t + (size_t)s * SG_H * SG_H);
+    const uint4* s2 = reinterpret_cast<const uint4*>(w2 + (size_t)s * SG_H * SG_H);
+    const uint4* s1 = reinterpret_cast<const uint4*>(w1 + (size_t)s * DIN * SG_H);
+#pragma unroll
+    for (int u = 0; u < C2; u++) cw2t[u] = s2t[u * 256 + threadIdx.x];
+#pragma unroll
+    for (int u = 0; u < C2; u++) cw2[u] = s2[u * 256 + threadIdx.x];
+#pragma unroll
+    for (int u = 0; u < C1; u++) cw1[u] = s1[u * 256 + threadIdx.x];
+  }
+  // chunk c of a matrix with `cols` columns: row c / (cols / 8), column 8 (c % (cols / 8))
+  auto put = [&](const uint4* v, int nchunk, int cols, int dst, int stride) {
+    const int per = cols / 8;
+#pragma unroll
+    for (int u = 0; u < nchunk; u++) {
+      const int c = u * 256 + threadIdx.x, r = c / per, k = 8 * (c % per);
+      *reinterpret_cast<uint2*>(lw + dst + r * stride + k) = make_uint2(v[u].x, v[u].y);
+      *reinterpret_cast<uint2*>(lw + dst + r * stride + k + 4) = make_uint2(v[u].z, v[u].w);
+    }
+  };
+  put(cw1t, C1, DIN, Lay::W1T, Lay::S1);
+  if (threadIdx.x < SG_H) {
+    lb[threadIdx.x] = bv[0];
+    lb[SG_H + threadIdx.x] = bv[1];
+    lb[2 * SG_H + threadIdx.x] = bv[2];
+  }
+  __syncthreads();
+  const float* b1 = lb;
+  const float* b2 = lb + SG_H;
+  const float* w3 = lb + 2 * SG_H;
   bf16x4 xb[KT];
 #pragma unroll
   for (int q = 0; q < KT; q++) {
-    float4 v = slot >= 0 ? *reinterpret_cast<const float4*>(X + (size_t)slot * DIN + 16 * q + 4 * g)
-                         : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    const float4 v = slot >= 0 ? xv[q] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
     xb[q] = bf16_4(v.x, v.y, v.z, v.w);
   }
   // d1, d2: SiLU'(Z1), SiLU'(Z2), formed with the forward activations from
@@ -448,12 +473,14 @@ __global__ void __launch_bounds__(256) k_sg_mlp(float* __restrict__ eat, float* 
     float h[4];
 #pragma unroll
     for (int r = 0; r < 4; r++) {
-      const float z = acc[r] + p[pl.b1() + 16 * m + 4 * g + r], sg = sigm_fast(z);
+      const float z = acc[r] + b1[16 * m + 4 * g + r], sg = sigm_fast(z);
       h[r] = z * sg;
       d1[m][r] = sg * (1.0f + z * (1.0f - sg));
     }
     hb[m] = bf16_4(h[0], h[1], h[2], h[3]);
   }
+  put(cw2t, C2, SG_H, Lay::W2T, Lay::SH);
+  __syncthreads();
 #pragma unroll
   for (int m = 0; m < HT; m++) {
     f32x4 acc = f32x4{ 0.0f, 0.0f, 0.0f, 0.0f };
@@ -464,8 +491,8 @@ __global__ void __launch_bounds__(256) k_sg_mlp(float* __restrict__ eat, float* 
     }
 #pragma unroll
     for (int r = 0; r < 4; r++) {
-      const float z = acc[r] + p[pl.b2() + 16 * m + 4 * g + r], sg = sigm_fast(z);
-      part += p[pl.w3() + 16 * m + 4 * g + r] * (z * sg);
+      const float z = acc[r] + b2[16 * m + 4 * g + r], sg = sigm_fast(z);
+      part += w3[16 * m + 4 * g + r] * (z * sg);
       d2[m][r] = sg * (1.0f + z * (1.0f - sg));
     }
   }
@@ -475,9 +502,12 @@ __global__ void __launch_bounds__(256) k_sg_mlp(float* __restrict__ eat, float* 
   const float seed = slot >= 0 ? nrm.ystd[s] : 0.0f;
 #pragma unroll
   for (int m = 0; m < HT; m++) {
-    const float* w3 = p + pl.w3() + 16 * m + 4 * g;
-    gb[m] = bf16_4(seed * w3[0] * d2[m][0], seed * w3[1] * d2[m][1], seed * w3[2] * d2[m][2], seed * w3[3] * d2[m][3]);
+    const float* w = w3 + 16 * m + 4 * g;
+    gb[m] = bf16_4(seed * w[0] * d2[m][0], seed * w[1] * d2[m][1], seed * w[2] * d2[m][2], seed * w[3] * d2[m][3]);
   }
+  put(cw2, C2, SG_H, Lay::W2, Lay::SH);
+  put(cw1, C1, SG_H, Lay::W1, Lay::SH);
+  __syncthreads();
   // G1 = (W2 G2) * SiLU'(Z1), into hb
 #pragma unroll
   for (int m = 0; m < HT; m++) {
