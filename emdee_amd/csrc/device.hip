@@ -5,10 +5,12 @@
 //     holds original atom perm[s].  Coordinates R, momenta P, forces F (one
 //     set per model layer), scaled coordinates Rs = R/L and the positions R0
 //     at the latest list build are FP64 [3][Npad].
-//   * the Verlet list is a FULL list in ELL layout: neighbor k of slot s is
-//     nbr[k*Npad + s], so lanes of a wavefront (consecutive slots) read
-//     consecutive words.  A full list (every pair stored twice) lets one
-//     thread own one atom's force: no atomics, deterministic sums.
+//   * the Verlet list is a FULL list in packets of four: entries 4p..4p+3 of
+//     slot s are the 16 bytes at int offset 4 (p*Npad + s) (nb_at), so a lane
+//     reads or writes four of its entries with one access and lanes of a
+//     wavefront (consecutive slots) touch consecutive packets.  A full list
+//     (every pair stored twice) lets one thread own one atom's force: no
+//     atomics, deterministic sums.
 //   * the list is rebuilt when the reference trigger fires
 //     (neighbor_lists.f90:41-59, 175-195): cells of side >= (rc+skin)/3, a
 //     counting sort of atoms by cell (stable: ascending original index within
@@ -513,7 +515,7 @@ __device__ __forceinline__ void scan_range(int a, int b, int i, double xi, doubl
       bool ok = j < b && accept(j);
       unsigned int g = (unsigned int)(__ballot(ok) >> gbase) & ((1u << S) - 1u);
       int pos = cnt + __popc(g & ((1u << sub) - 1u));
-      if (ok && pos < cap) nbr[(size_t)pos * Npad + i] = j;
+      if (ok && pos < cap) nbr[nb_at(pos, i, Npad)] = j;
       cnt += __popc(g);
     }
     return;
@@ -522,7 +524,6 @@ __device__ __forceinline__ void scan_range(int a, int b, int i, double xi, doubl
     // rows are short (a few candidates), so every row is taken four
     // candidates at a time with validity masks; the rare band test is
     // hoisted out of the per-candidate path
-    int* __restrict__ col = nbr + i;
     for (int j = a; j < b; j += 4) {
       int4 q[4];
 #pragma unroll
@@ -549,7 +550,7 @@ __device__ __forceinline__ void scan_range(int a, int b, int i, double xi, doubl
 #pragma unroll
       for (int u = 0; u < 4; u++) {
         if (in[u]) {
-          if (cnt < cap) col[(size_t)cnt * Npad] = j + u;
+          if (cnt < cap) nbr[nb_at(cnt, i, Npad)] = j + u;
           cnt++;
         }
       }
@@ -573,7 +574,7 @@ __device__ __forceinline__ void scan_range(int a, int b, int i, double xi, doubl
           for (int e = e0; e < e1; e++) if (exItem[e] == oj) return;
         }
       }
-      if (cnt < cap) nbr[(size_t)cnt * Npad + i] = j;
+      if (cnt < cap) nbr[nb_at(cnt, i, Npad)] = j;
       cnt++;
     }
   };
@@ -662,7 +663,7 @@ k_build_list(int* __restrict__ nbr, int* __restrict__ nnb, int* __restrict__ fla
       // pad the list to a multiple of 4 with the atom itself (zero separation):
       // the mixed-form kernel reads whole groups of four without a bound test
       if (cnt <= cap)
-        for (int k = cnt; k < ((cnt + 3) & ~3); k++) nbr[(size_t)k * Npad + i] = i;
+        for (int k = cnt; k < ((cnt + 3) & ~3); k++) nbr[nb_at(k, i, Npad)] = i;
       if (cnt > cap) atomicMax(&flags[0], cnt);
     }
   }
@@ -700,6 +701,7 @@ k_build_list_seg(int* __restrict__ nbr, int* __restrict__ nnb, int* __restrict__
                  const double4* __restrict__ Rs4, const int* __restrict__ key, const int* __restrict__ start, int M,
                  double xRc2, int cap, int N, int Npad, const uint2* __restrict__ P16, int lo, int hi) {
   __shared__ int2 seg[SEG_MAX * BLOCK];
+  __shared__ int ring[8 * BLOCK];
   int blk = xcd_block();
   int i = blk * BLOCK + threadIdx.x;
   int cnt = 0;
@@ -733,11 +735,24 @@ k_build_list_seg(int* __restrict__ nbr, int* __restrict__ nnb, int* __restrict__
     // queued band candidates, in four named registers (an array would be
     // indexed by nq and kept in scratch memory)
     int bq0 = 0, bq1 = 0, bq2 = 0, bq3 = 0, nq = 0;
-    unsigned int off = (unsigned int)i;     // element offset of entry cnt of slot i
+    // Entries go to a lane-private ring of eight in LDS (entry c at slot
+    // c % 8); a packet of four is stored with one 16-byte store once it is
+    // complete.  Flushes follow each group of at most four appends, so at
+    // most one packet completes between two flushes and the other half of
+    // the ring is never overwritten before it is stored.
+    int* __restrict__ rg = ring + threadIdx.x;   // ring slot r at rg[r * BLOCK]
+    int4v* __restrict__ pk = reinterpret_cast<int4v*>(nbr) + i;   // packet p at pk[p * Npad]
+    const int capP = cap >> 2;
     auto append = [&](int j) __attribute__((always_inline)) {
-      if (cnt < cap) nbr[off] = j;
-      off += (unsigned int)Npad;
+      rg[(cnt & 7) * BLOCK] = j;
       cnt++;
+    };
+    auto flush = [&](int before) __attribute__((always_inline)) {
+      const int p = before >> 2;
+      if ((cnt >> 2) > p && p < capP) {
+        const int r = (p & 1) * 4;
+        pk[(size_t)p * Npad] = (int4v){ rg[r * BLOCK], rg[(r + 1) * BLOCK], rg[(r + 2) * BLOCK], rg[(r + 3) * BLOCK] };
+      }
     };
     auto exact = [&](int j) __attribute__((always_inline)) {
       double4 rj = Rs4[j];
@@ -833,6 +848,7 @@ k_build_list_seg(int* __restrict__ nbr, int* __restrict__ nnb, int* __restrict__
         // one append site: increments of the count and of the queue length
         // on exclusive paths would otherwise be merged into a store through
         // a pointer, which keeps both in scratch memory
+        const int before = cnt;
 #pragma unroll
         for (int u = 0; u < 4; u++) {
           const int j = qa + u;
@@ -850,6 +866,7 @@ k_build_list_seg(int* __restrict__ nbr, int* __restrict__ nnb, int* __restrict__
           }
           if (take) append(j);
         }
+        flush(before);
       };
       uint2 q0[4], q1[4], q2[4];
       int a0, b0, a1, b1, a2, b2;
@@ -869,15 +886,22 @@ k_build_list_seg(int* __restrict__ nbr, int* __restrict__ nnb, int* __restrict__
     }
     // the band candidates, by the FP64 test
     bool t0 = nq > 0 && exact(bq0), t1 = nq > 1 && exact(bq1), t2 = nq > 2 && exact(bq2), t3 = nq > 3 && exact(bq3);
-    if (t0) append(bq0);
-    if (t1) append(bq1);
-    if (t2) append(bq2);
-    if (t3) append(bq3);
+    {
+      const int before = cnt;
+      if (t0) append(bq0);
+      if (t1) append(bq1);
+      if (t2) append(bq2);
+      if (t3) append(bq3);
+      flush(before);
+    }
     nnb[i] = cnt;
-    // pad the list to a multiple of 4 with the atom itself (zero separation):
-    // the mixed-form kernel reads whole groups of four without a bound test
-    if (cnt <= cap)
-      for (int k = cnt; k < ((cnt + 3) & ~3); k++) nbr[(size_t)k * Npad + i] = i;
+    // the last packet, padded with the atom itself (zero separation): the
+    // mixed-form kernel reads whole groups of four without a bound test
+    if (cnt & 3) {
+      const int n = cnt;
+      while (cnt & 3) append(i);
+      flush(n);
+    }
     if (cnt > cap) atomicMax(&flags[0], cnt);
   }
   double v[1] = { (double)cnt };
@@ -1057,7 +1081,6 @@ k_pairs(double* __restrict__ F, double* __restrict__ partials,
     DevPair pu;
     if (UNIFORM) pu = pairs[0];   // uniform address: scalar loads
     const int n = nnb[i];
-    const int* __restrict__ col = nbr + i;
     auto one = [&](int j, const double4& rj) {
       double dx = ri.x - rj.x, dy = ri.y - rj.y, dz = ri.z - rj.z;
       dx = dx - rint(dx); dy = dy - rint(dy); dz = dz - rint(dz);
@@ -1096,25 +1119,23 @@ k_pairs(double* __restrict__ F, double* __restrict__ partials,
     if (TPA == 1) {
       int k = 0;
       for (; k + 4 <= n; k += 4) {
-        int j0 = __builtin_nontemporal_load(col + (size_t)(k + 0) * Npad);
-        int j1 = __builtin_nontemporal_load(col + (size_t)(k + 1) * Npad);
-        int j2 = __builtin_nontemporal_load(col + (size_t)(k + 2) * Npad);
-        int j3 = __builtin_nontemporal_load(col + (size_t)(k + 3) * Npad);
+        const int4v jv = nb_packet(nbr, k >> 2, i, Npad);
+        const int j0 = jv.x, j1 = jv.y, j2 = jv.z, j3 = jv.w;
         double4 r0 = Rs4[j0], r1 = Rs4[j1], r2 = Rs4[j2], r3 = Rs4[j3];
         one(j0, r0); one(j1, r1); one(j2, r2); one(j3, r3);
       }
       for (; k < n; k++) {
-        int j = __builtin_nontemporal_load(col + (size_t)k * Npad);
+        int j = nbr[nb_at(k, i, Npad)];
         one(j, Rs4[j]);
       }
     } else {
       int k = sub;
       for (; k + TPA < n; k += 2 * TPA) {
-        int j0 = col[(size_t)k * Npad], j1 = col[(size_t)(k + TPA) * Npad];
+        int j0 = nbr[nb_at(k, i, Npad)], j1 = nbr[nb_at(k + TPA, i, Npad)];
         double4 r0 = Rs4[j0], r1 = Rs4[j1];
         one(j0, r0); one(j1, r1);
       }
-      if (k < n) { int j = col[(size_t)k * Npad]; one(j, Rs4[j]); }
+      if (k < n) { int j = nbr[nb_at(k, i, Npad)]; one(j, Rs4[j]); }
     }
   }
   if (TPA > 1) {
@@ -1215,7 +1236,6 @@ k_pairs_mx(double* __restrict__ F, double* __restrict__ partials, const int4* __
     DevPair pu;
     if (UNIFORM) pu = pairs[0];   // uniform address: scalar loads, no VGPRs
     const int n = nnb[i];
-    const int* __restrict__ col = nbr + i;
     auto one = [&](int j, const int4& qj, MixedAcc& a) {
       float dx = (float)(int)((unsigned int)qi.x - (unsigned int)qj.x);
       float dy = (float)(int)((unsigned int)qi.y - (unsigned int)qj.y);
@@ -1299,31 +1319,32 @@ k_pairs_mx(double* __restrict__ F, double* __restrict__ partials, const int4* __
       // next twelve entries are loaded before this block's terms, so their
       // latency hides behind the arithmetic.
       int k = 0;
-      int jn[12];
+      int4v jn[3];
       if (n4 >= 12) {
 #pragma unroll
-        for (int u = 0; u < 12; u++) jn[u] = __builtin_nontemporal_load(col + (size_t)u * Npad);
+        for (int u = 0; u < 3; u++) jn[u] = nb_packet(nbr, u, i, Npad);
       }
       for (; k + 12 <= n4; k += 12) {
         int j[12];
         int4 q[12];
 #pragma unroll
-        for (int u = 0; u < 12; u++) j[u] = jn[u];
+        for (int u = 0; u < 3; u++) {
+          j[4 * u] = jn[u].x; j[4 * u + 1] = jn[u].y; j[4 * u + 2] = jn[u].z; j[4 * u + 3] = jn[u].w;
+        }
 #pragma unroll
         for (int u = 0; u < 12; u++) q[u] = Q4[j[u]];
         if (k + 24 <= n4) {
 #pragma unroll
-          for (int u = 0; u < 12; u++) jn[u] = __builtin_nontemporal_load(col + (size_t)(k + 12 + u) * Npad);
+          for (int u = 0; u < 3; u++) jn[u] = nb_packet(nbr, (k >> 2) + 3 + u, i, Npad);
         }
         group4(j, q);
         group4(j + 4, q + 4);
         group4(j + 8, q + 8);
       }
       for (; k < n4; k += 4) {
-        int j[4];
+        const int4v jv = nb_packet(nbr, k >> 2, i, Npad);
+        int j[4] = { jv.x, jv.y, jv.z, jv.w };
         int4 q[4];
-#pragma unroll
-        for (int u = 0; u < 4; u++) j[u] = __builtin_nontemporal_load(col + (size_t)(k + u) * Npad);
 #pragma unroll
         for (int u = 0; u < 4; u++) q[u] = Q4[j[u]];
         group4(j, q);
@@ -1331,10 +1352,8 @@ k_pairs_mx(double* __restrict__ F, double* __restrict__ partials, const int4* __
     } else if (TPA == 1) {
       int k = 0;
       for (; k + 4 <= n; k += 4) {
-        int j0 = __builtin_nontemporal_load(col + (size_t)(k + 0) * Npad);
-        int j1 = __builtin_nontemporal_load(col + (size_t)(k + 1) * Npad);
-        int j2 = __builtin_nontemporal_load(col + (size_t)(k + 2) * Npad);
-        int j3 = __builtin_nontemporal_load(col + (size_t)(k + 3) * Npad);
+        const int4v jv = nb_packet(nbr, k >> 2, i, Npad);
+        const int j0 = jv.x, j1 = jv.y, j2 = jv.z, j3 = jv.w;
         int4 q0 = Q4[j0], q1 = Q4[j1], q2 = Q4[j2], q3 = Q4[j3];
         MixedAcc a = { 0, 0, 0, 0, 0, 0, 0 };
         one(j0, q0, a); one(j1, q1, a); one(j2, q2, a); one(j3, q3, a);
@@ -1342,15 +1361,15 @@ k_pairs_mx(double* __restrict__ F, double* __restrict__ partials, const int4* __
       }
       MixedAcc a = { 0, 0, 0, 0, 0, 0, 0 };
       for (; k < n; k++) {
-        int j = __builtin_nontemporal_load(col + (size_t)k * Npad);
+        int j = nbr[nb_at(k, i, Npad)];
         one(j, Q4[j], a);
       }
       flush(a);
     } else {
       int k = sub;
       for (; k + 3 * TPA < n; k += 4 * TPA) {
-        int j0 = col[(size_t)k * Npad], j1 = col[(size_t)(k + TPA) * Npad];
-        int j2 = col[(size_t)(k + 2 * TPA) * Npad], j3 = col[(size_t)(k + 3 * TPA) * Npad];
+        int j0 = nbr[nb_at(k, i, Npad)], j1 = nbr[nb_at(k + TPA, i, Npad)];
+        int j2 = nbr[nb_at(k + 2 * TPA, i, Npad)], j3 = nbr[nb_at(k + 3 * TPA, i, Npad)];
         int4 q0 = Q4[j0], q1 = Q4[j1], q2 = Q4[j2], q3 = Q4[j3];
         MixedAcc a = { 0, 0, 0, 0, 0, 0, 0 };
         one(j0, q0, a); one(j1, q1, a); one(j2, q2, a); one(j3, q3, a);
@@ -1358,7 +1377,7 @@ k_pairs_mx(double* __restrict__ F, double* __restrict__ partials, const int4* __
       }
       MixedAcc a = { 0, 0, 0, 0, 0, 0, 0 };
       for (; k < n; k += TPA) {
-        int j = col[(size_t)k * Npad];
+        int j = nbr[nb_at(k, i, Npad)];
         one(j, Q4[j], a);
       }
       flush(a);
@@ -1574,7 +1593,7 @@ __device__ __forceinline__ void rdf_atom(unsigned long long* __restrict__ count,
   double ri[3] = { __dmul_rn(invL, R[s]), __dmul_rn(invL, R[Npad + s]), __dmul_rn(invL, R[2 * Npad + s]) };
   int n = nnb[s];
   for (int k = 0; k < n; k++) {
-    int t = nbr[(size_t)k * Npad + s];
+    int t = nbr[nb_at(k, s, Npad)];
     if (perm[t] <= a) continue;
     int jt = type[t] + 1;
     if (!on(it, jt)) continue;
@@ -2110,7 +2129,7 @@ int dev_neighbor_pairs(Device* d, int max, int* out) {
   int n = 0;
   for (int s = 0; s < N; s++)
     for (int k = 0; k < nnb[s]; k++) {
-      int t = nbr[(size_t)k * Np + s];
+      int t = nbr[nb_at(k, s, Np)];
       int a = perm[s] + 1, b = perm[t] + 1;
       if (a < b) {
         if (n < max) { out[2 * n] = a; out[2 * n + 1] = b; }

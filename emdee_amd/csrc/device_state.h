@@ -23,6 +23,19 @@ constexpr int NXCD = 8;
 
 static inline int div_up(long a, long b) { return (int)((a + b - 1) / b); }
 
+// Verlet list layout: each slot's entries in packets of four consecutive
+// ints, one row of Npad packets per four entries, so entry k of slot s is at
+// nb_at(k, s, Npad) and four entries of a slot are one 16-byte access (the
+// capacity is a multiple of eight).
+__host__ __device__ __forceinline__ size_t nb_at(int k, int s, int Npad) {
+  return 4 * ((size_t)(k >> 2) * Npad + s) + (k & 3);
+}
+typedef int int4v __attribute__((ext_vector_type(4)));
+// entries 4p .. 4p+3 of slot s
+__device__ __forceinline__ int4v nb_packet(const int* __restrict__ nbr, int p, int s, int Npad) {
+  return __builtin_nontemporal_load(reinterpret_cast<const int4v*>(nbr) + ((size_t)p * Npad + s));
+}
+
 // Blocks are dispatched round-robin over the 8 XCDs (block b runs on XCD
 // b % 8).  Remap so that XCD x processes one contiguous range of slots: the
 // neighbor positions a wave gathers then mostly live in that XCD's L2.

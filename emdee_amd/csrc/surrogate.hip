@@ -165,7 +165,7 @@ __global__ void __launch_bounds__(BLOCK) k_sg_list(int* __restrict__ cJ, int* __
     int k = k0 + sub, j = 0;
     bool keep = false;
     if (k < n) {
-      j = nbr[(size_t)k * Npad + i];
+      j = nbr[nb_at(k, i, Npad)];
       double dx = xi - R0[j], dy = yi - R0[Npad + j], dz = zi - R0[2 * Npad + j];
       dx -= L * rint(dx * invL);
       dy -= L * rint(dy * invL);
