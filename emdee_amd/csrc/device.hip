@@ -786,8 +786,11 @@ k_build_list_seg(int* __restrict__ nbr, int* __restrict__ nnb, int* __restrict__
         unsigned int i0 = row + (unsigned int)(on ? (wlo ? xa + M : xa) : 0);
         unsigned int i1 = row + (unsigned int)(on ? (wlo ? M : min(xb + 1, M)) : 0);
         unsigned int i3 = row + (unsigned int)(two ? (wlo ? xb + 1 : xb - M + 1) : 0);
-        // slots of the row, in order
-        int s0 = start[i0], s1 = start[i1], w0 = start[row], w1 = start[i3];
+        // slots of the row, in order; a culled row loads nothing, a row that
+        // does not wrap loads only its first range
+        int s0 = 0, s1 = 0, w0 = 0, w1 = 0;
+        if (on) { s0 = start[i0]; s1 = start[i1]; }
+        if (two) { w0 = start[row]; w1 = start[i3]; }
         if (t < CELL_REACH) {
           sa[2 * t] = s0; sb[2 * t] = s1; sa[2 * t + 1] = w0; sb[2 * t + 1] = w1;
         } else if (t > CELL_REACH) {
