@@ -1188,7 +1188,7 @@ __device__ __forceinline__ void pair_body_mx(const DevPair& p, const DevCoul& cm
   pair_model<float>(p, invR2, E, W);
   if (p.modifier != MOD_NONE) {
     float invR = (p.modifier == MOD_SHIFTED_FORCE || p.modifier == MOD_SMOOTHED || p.modifier == MOD_SHIFTED_SMOOTHED)
-                     ? sqrtf(invR2) : 0.0f;
+                     ? sqrt_fast(invR2) : 0.0f;
     apply_modifier<float>(p.modifier, float(p.eshift), float(p.fshift), float(p.factor), float(p.Rm2fac), invR,
                           invR2, E, W);
   }
@@ -1196,7 +1196,7 @@ __device__ __forceinline__ void pair_body_mx(const DevPair& p, const DevCoul& cm
   a.Wp += W;
   float Wsum = W;
   if (jc) {
-    float invR = sqrtf(invR2);
+    float invR = sqrt_fast(invR2);
     coul_model<float, ENERGY>(cm, invR, invR2, E, W);
     apply_modifier<float>(cm.modifier, float(cm.eshift), float(cm.fshift), float(cm.factor), float(cm.Rm2fac), invR,
                           invR2, E, W);
@@ -1258,7 +1258,7 @@ k_pairs_mx(double* __restrict__ F, double* __restrict__ partials, const int4* __
           float W = c.y * (sr12 + sr12 - sr6);
           if (ENERGY) a.Ep += c.x * (sr12 - sr6) + c.w;
           a.Wp += W;
-          float invR = sqrtf(invR2), E, Wq;
+          float invR = sqrt_fast(invR2), E, Wq;
           coul_model<float, ENERGY>(cm, invR, invR2, E, Wq);
           apply_modifier<float>(cm.modifier, float(cm.eshift), float(cm.fshift), float(cm.factor), float(cm.Rm2fac),
                                 invR, invR2, E, Wq);

@@ -21,6 +21,15 @@ __device__ __forceinline__ T rcp(T x) { return T(1) / x; }
 template <>
 __device__ __forceinline__ float rcp<float>(float x) { return __builtin_amdgcn_rcpf(x); }
 
+// a / b and sqrt: IEEE for double (the plain form keeps the reference
+// arithmetic); the hardware reciprocal and square root (about 1 ulp) for the
+// float terms of the mixed form
+template <typename T>
+__device__ __forceinline__ T quo(T a, T b) { return a / b; }
+template <>
+__device__ __forceinline__ float quo<float>(float a, float b) { return a * __builtin_amdgcn_rcpf(b); }
+__device__ __forceinline__ float sqrt_fast(float x) { return __builtin_amdgcn_sqrtf(x); }
+
 template <typename T>
 __device__ __forceinline__ T uerfc_dev(T x, T expmx2) {
   // Abramowitz & Stegun 7.1.26 (math.f90:35-40, 685-691)
@@ -41,8 +50,8 @@ __device__ __forceinline__ void pair_model(const DevPair& p, T invR2, T& E, T& W
     E = T(p.c0) * (sr12 - sr6);
     W = T(p.c1) * (sr12 + sr12 - sr6);
   } else if (p.kind == PAIR_SOFTCORE_CUT) {
-    T rsig2 = T(p.c2) / invR2, rsig6 = rsig2 * rsig2 * rsig2;
-    T sinv = T(1) / (rsig6 + T(p.c3)), sinvSq = sinv * sinv, sinvCb = sinv * sinvSq;
+    T rsig2 = quo<T>(T(p.c2), invR2), rsig6 = rsig2 * rsig2 * rsig2;
+    T sinv = rcp<T>(rsig6 + T(p.c3)), sinvSq = sinv * sinv, sinvCb = sinv * sinvSq;
     E = T(p.c0) * (sinvSq - sinv);
     W = T(p.c1) * rsig6 * (sinvCb + sinvCb - sinvSq);
   } else {
@@ -61,14 +70,14 @@ __device__ __forceinline__ void apply_modifier(int mod, T eshift, T fshift, T fa
       E = E + eshift;
       break;
     case MOD_SHIFTED_FORCE: {
-      T rFc = fshift / invR;
+      T rFc = quo<T>(fshift, invR);
       W = W - rFc;
       E = E + eshift + rFc;
       break;
     }
     case MOD_SMOOTHED: case MOD_SHIFTED_SMOOTHED: {
       E = E + eshift;
-      T r2fac = factor / invR;
+      T r2fac = quo<T>(factor, invR);
       if (r2fac > Rm2fac) {
         T u = r2fac - Rm2fac, u2 = u * u, u3 = u * u2;
         T G = T(1) + u3 * (T(15) * u - T(6) * u2 - T(10));
@@ -80,7 +89,7 @@ __device__ __forceinline__ void apply_modifier(int mod, T eshift, T fshift, T fa
     }
     case MOD_SQUARE_SMOOTHED: case MOD_SHIFTED_SQUARE_SMOOTHED: {
       E = E + eshift;
-      T r2fac = factor / invR2;
+      T r2fac = quo<T>(factor, invR2);
       if (r2fac > Rm2fac) {
         T u = r2fac - Rm2fac, u2 = u * u, u3 = u * u2;
         T G = T(1) + u3 * (T(15) * u - T(6) * u2 - T(10));
@@ -106,19 +115,19 @@ __device__ __forceinline__ void coul_model(const DevCoul& c, T invR, T invR2, T&
       E = invR; W = invR;
       break;
     case COUL_SF: {
-      T rFc = T(c.fshift) / invR;
+      T rFc = quo<T>(T(c.fshift), invR);
       E = invR + T(c.eshift) + rFc;
       W = invR - rFc;
       break;
     }
     case COUL_DAMPED: case COUL_LONG: {
-      T x = T(c.alpha) / invR, expmx2 = dexp<T>(-x * x);
+      T x = quo<T>(T(c.alpha), invR), expmx2 = dexp<T>(-x * x);
       E = uerfc_dev<T>(x, expmx2) * invR;
       W = E + T(c.beta) * expmx2;
       break;
     }
     case COUL_DAMPED_SMOOTHED: {
-      T r = T(1) / invR, x = T(c.alpha) * r, expmx2 = dexp<T>(-x * x);
+      T r = rcp<T>(invR), x = T(c.alpha) * r, expmx2 = dexp<T>(-x * x);
       E = uerfc_dev<T>(x, expmx2) * invR;
       W = E + T(c.beta) * expmx2;
       if (r > T(c.Rm)) {
@@ -131,11 +140,11 @@ __device__ __forceinline__ void coul_model(const DevCoul& c, T invR, T invR2, T&
       break;
     }
     case COUL_DAMPED_SQUARE_SMOOTHED: {
-      T x = T(c.alpha) / invR, expmx2 = dexp<T>(-x * x);
+      T x = quo<T>(T(c.alpha), invR), expmx2 = dexp<T>(-x * x);
       E = uerfc_dev<T>(x, expmx2) * invR;
       W = E + T(c.beta) * expmx2;
       if (invR < T(c.invRm)) {
-        T r2 = T(1) / invR2, u = T(c.factor) * (r2 - T(c.Rm2)), u2 = u * u, u3 = u * u2;
+        T r2 = rcp<T>(invR2), u = T(c.factor) * (r2 - T(c.Rm2)), u2 = u * u, u3 = u * u2;
         T G = T(1) + u3 * (T(15) * u - T(6) * u2 - T(10));
         T WG = T(-60) * u2 * (T(2) * u - u2 - T(1)) * T(c.factor) * r2;
         W = W * G + E * WG;
@@ -147,7 +156,7 @@ __device__ __forceinline__ void coul_model(const DevCoul& c, T invR, T invR2, T&
       W = invR;
       E = (c.kind == COUL_SQUARE_SMOOTHED) ? W : W + T(c.eshift);
       if (invR < T(c.invRm)) {
-        T r2 = T(1) / invR2, u = T(c.factor) * (r2 - T(c.Rm2)), u2 = u * u, u3 = u * u2;
+        T r2 = rcp<T>(invR2), u = T(c.factor) * (r2 - T(c.Rm2)), u2 = u * u, u3 = u * u2;
         T G = T(1) + u3 * (T(15) * u - T(6) * u2 - T(10));
         T WG = T(-60) * u2 * (T(2) * u - u2 - T(1)) * T(c.factor) * r2;
         if (!ENERGY && c.kind == COUL_SQUARE_SMOOTHED) W = W * (G + WG);
