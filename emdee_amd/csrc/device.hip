@@ -61,6 +61,8 @@ static void post_wait(Device* d, unsigned long long seq) {
   std::atomic_thread_fence(std::memory_order_acquire);
 }
 
+void dev_post_wait(Device* d, unsigned long long seq) { post_wait(d, seq); }
+
 __global__ void k_iota(int* p, int n) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) p[i] = i;

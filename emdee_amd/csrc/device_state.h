@@ -121,9 +121,10 @@ __global__ void k_reduce_post(double* __restrict__ host, unsigned long long* __r
 
 // ------------------------------------------------------------ device state
 
-// Slot of the pair sums in the posted-results block (64 doubles); the other
-// posts use slot 0.
-constexpr int POST_PAIRS = 8;
+// Slots in the posted-results block (64 doubles): the pair sums, the Ewald
+// energy and virial, the learned model's energy, virial and overflow flag;
+// the other posts use slot 0.
+constexpr int POST_PAIRS = 8, POST_EWALD = 16, POST_MODEL = 24;
 
 template <typename T>
 struct DBuf {
@@ -241,6 +242,9 @@ struct Device {
 
 
 void launch_check(const char* what);
+// Waits until the post with sequence number seq has arrived in pinned host
+// memory (posts arrive in stream order).
+void dev_post_wait(Device* d, unsigned long long seq);
 void dev_ewald_release(Device* d);
 void dev_structs_release(Device* d);
 void dev_bodies_shadow_pre(Device* d, double* sh, double dt, int mode);

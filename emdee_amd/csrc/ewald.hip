@@ -42,7 +42,7 @@ struct EwaldState {
   DBuf<int4> kn;
   DBuf<double4> kv;
   DBuf<int> item, groupType, typeGroups;
-  DBuf<double> q, lambda, Fpart, part, result, selfc;
+  DBuf<double> q, lambda, Fpart, part, selfc;
   DBuf<double> vfac;      // per k: 1 - k^2 / (2 alpha^2), when the virial of the sum is wanted
   DBuf<double2> T, Spart, S, sigma;
   std::vector<int> hostTypeGroups;
@@ -57,7 +57,7 @@ void dev_ewald_release(Device* d) {
   EwaldState* e = d->ewald;
   if (!e) return;
   e->kn.release(); e->kv.release(); e->item.release(); e->groupType.release(); e->typeGroups.release();
-  e->q.release(); e->lambda.release(); e->Fpart.release(); e->part.release(); e->result.release(); e->selfc.release();
+  e->q.release(); e->lambda.release(); e->Fpart.release(); e->part.release(); e->selfc.release();
   e->vfac.release(); e->T.release(); e->Spart.release(); e->S.release(); e->sigma.release();
   delete e;
   d->ewald = nullptr;
@@ -344,7 +344,6 @@ static void ewald_state_setup(EwaldState* e, const EwaldSetup& s) {
   e->Fpart.alloc((size_t)e->ksF * 4 * s.nslots);
   if (s.nt > EW_SIGMA_MAXT) fatal("kspace set-up", "more than 8 charged types");
   e->part.alloc(2 * (size_t)div_up(s.nvecs, BLOCK / 64));
-  e->result.alloc(2);
 }
 
 void dev_ewald_setup(Device* d, const EwaldSetup& s) {
@@ -382,8 +381,15 @@ static double ewald_eval(Device* d, EwaldState* e, double* F, const double* lam,
   hipLaunchKernelGGL(k_ew_sigma, dim3(gk), dim3(BLOCK), 0, d->stream, e->S.p, e->sigma.p, e->part.p, e->Spart.p,
                      e->typeGroups.p, lam, e->kv.p, vf, e->nt, e->nvecs);
   launch_check("ewald sigma");
-  if (vf) hipLaunchKernelGGL(k_reduce_partials<2>, dim3(1), dim3(BLOCK), 0, d->stream, e->result.p, e->part.p, gk);
-  else hipLaunchKernelGGL(k_reduce_partials<1>, dim3(1), dim3(BLOCK), 0, d->stream, e->result.p, e->part.p, gk);
+  // energy (and virial) straight into pinned host memory: the host reads
+  // them as soon as they arrive, while the force kernels below still run
+  const unsigned long long seq = ++d->seq;
+  if (vf)
+    hipLaunchKernelGGL(k_reduce_post<2>, dim3(1), dim3(BLOCK), 0, d->stream, d->hpostDev + POST_EWALD, d->hseqDev, seq,
+                       e->part.p, gk);
+  else
+    hipLaunchKernelGGL(k_reduce_post<1>, dim3(1), dim3(BLOCK), 0, d->stream, d->hpostDev + POST_EWALD, d->hseqDev, seq,
+                       e->part.p, gk);
   hipLaunchKernelGGL(k_ew_forces, dim3(e->nslots / EW_BLOCK, e->ksF), dim3(EW_BLOCK), 0, d->stream, e->Fpart.p,
                      e->T.p, e->q.p, e->kn.p, e->kv.p, e->sigma.p, e->groupType.p, e->nm, e->nslots, e->nvecs,
                      e->kchF);
@@ -392,10 +398,9 @@ static double ewald_eval(Device* d, EwaldState* e, double* F, const double* lam,
                      eatom, e->Fpart.p,
                      e->item.p, d->iperm.p, e->q.p, e->groupType.p, selfc, e->nslots, e->ksF, Np);
   launch_check("ewald force add");
-  HIP_CHECK(hipMemcpyAsync(d->hresult, e->result.p, 2 * sizeof(double), hipMemcpyDeviceToHost, d->stream));
-  HIP_CHECK(hipStreamSynchronize(d->stream));
-  if (W) *W = d->hresult[1];
-  return d->hresult[0];
+  dev_post_wait(d, seq);
+  if (W) *W = d->hpost[POST_EWALD + 1];
+  return d->hpost[POST_EWALD];
 }
 
 double dev_ewald_compute(Device* d, int layer, double L) {
@@ -436,7 +441,7 @@ int dev_lowk_vectors(const EwaldState* e) { return e ? e->nvecs : 0; }
 void dev_lowk_release(EwaldState* e) {
   if (!e) return;
   e->kn.release(); e->kv.release(); e->item.release(); e->groupType.release(); e->typeGroups.release();
-  e->q.release(); e->lambda.release(); e->Fpart.release(); e->part.release(); e->result.release(); e->selfc.release();
+  e->q.release(); e->lambda.release(); e->Fpart.release(); e->part.release(); e->selfc.release();
   e->vfac.release(); e->T.release(); e->Spart.release(); e->S.release(); e->sigma.release();
   delete e;
 }

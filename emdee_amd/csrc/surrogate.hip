@@ -109,7 +109,7 @@ struct EmDeeSurrogate {
   long listVersion = -1;
   long inferRows = 0;
   DBuf<float> Xi, dEdD, eat;
-  DBuf<double> epart, wpart, eres;
+  DBuf<double> epart, wpart;
   // exact long-range channel: the reciprocal sum over |k| <= kcut (0: none)
   double kcut = 0.0;
   EwaldState* lowk = nullptr;
@@ -1045,6 +1045,26 @@ static void repack(EmDeeSurrogate* s) {
   launch_check("surrogate pack");
 }
 
+// The model's energy and virial (fixed-order sums of the MLP group partials
+// and the force kernel's block partials) and the near-list overflow count,
+// posted into pinned host memory with sequence number seq.
+__global__ void __launch_bounds__(BLOCK) k_sg_post(double* __restrict__ host, unsigned long long* __restrict__ hostSeq,
+                                                   unsigned long long seq, const double* __restrict__ epart, int ne,
+                                                   const double* __restrict__ wpart, int nw,
+                                                   const int* __restrict__ over) {
+  double v[2] = { 0.0, 0.0 };
+  for (int b = threadIdx.x; b < ne; b += BLOCK) v[0] += epart[b];
+  for (int b = threadIdx.x; b < nw; b += BLOCK) v[1] += wpart[b];
+  block_sum<2>(v);
+  if (threadIdx.x == 0) {
+    host[0] = v[0];
+    host[1] = v[1];
+    host[2] = (double)*over;
+    __threadfence_system();
+    *(volatile unsigned long long*)hostSeq = seq;
+  }
+}
+
 __global__ void k_sg_add_atom(double* __restrict__ dst, const float* __restrict__ src, int n) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) dst[i] += (double)src[i];
@@ -1067,7 +1087,6 @@ double dev_surrogate_apply(Device* d, EmDeeSurrogate* s, int layer, const Surrog
     s->dEdD.alloc((size_t)Np * din);
     s->eat.alloc(Np);
     s->wpart.alloc(div_up(Np, BLOCK / 64));
-    s->eres.alloc(2);
     s->inferRows = Np;
   }
   launch_desc(s, d, c, s->Xi.p, true);
@@ -1089,22 +1108,20 @@ double dev_surrogate_apply(Device* d, EmDeeSurrogate* s, int layer, const Surrog
   switch (s->sp.nt) { case 1: SG_RUN(1); break; case 2: SG_RUN(2); break; case 3: SG_RUN(3); break; default: SG_RUN(4); }
 #undef SG_RUN
   launch_check("surrogate network and forces");
-  hipLaunchKernelGGL(k_reduce_partials<1>, dim3(1), dim3(BLOCK), 0, d->stream, s->eres.p, s->epart.p, s->ngroups);
-  hipLaunchKernelGGL(k_reduce_partials<1>, dim3(1), dim3(BLOCK), 0, d->stream, s->eres.p + 1, s->wpart.p, gf);
+  const unsigned long long seq = ++d->seq;
+  hipLaunchKernelGGL(k_sg_post, dim3(1), dim3(BLOCK), 0, d->stream, d->hpostDev + POST_MODEL, d->hseqDev, seq,
+                     s->epart.p, s->ngroups, s->wpart.p, gf, s->nearOver.p);
   if (d->wantAtomE)
     hipLaunchKernelGGL(k_sg_add_atom, dim3(div_up(N, BLOCK)), dim3(BLOCK), 0, d->stream, d->eatom.p, s->eat.p, N);
   launch_check("surrogate sums");
-  // results into the device's pinned block (slots 48..50), read after the
-  // one wait below (the low-k evaluation ends with it when it runs)
-  double* hr = d->hresult + 48;
-  HIP_CHECK(hipMemcpyAsync(hr, s->eres.p, 2 * sizeof(double), hipMemcpyDeviceToHost, d->stream));
-  HIP_CHECK(hipMemcpyAsync(reinterpret_cast<int*>(hr + 2), s->nearOver.p, sizeof(int), hipMemcpyDeviceToHost, d->stream));
   double El = 0.0, Wl = 0.0;
+  // the low-k evaluation waits for its own post, which arrives after this one
   if (EwaldState* lk = ensure_lowk(s, d, c, task))
     El = dev_lowk_compute(d, lk, F, d->wantAtomE ? d->eatom.p : nullptr, c.L, &Wl);
   else
-    HIP_CHECK(hipStreamSynchronize(d->stream));
-  if (*reinterpret_cast<int*>(hr + 2)) fatal(task, "more neighbours within the descriptor cutoff than the model holds (256)");
+    dev_post_wait(d, seq);
+  const double* hr = d->hpost + POST_MODEL;
+  if (hr[2] != 0.0) fatal(task, "more neighbours within the descriptor cutoff than the model holds (256)");
   *W = hr[1] + Wl;
   return hr[0] + El;
 }
@@ -1344,7 +1361,7 @@ void EmDeeSurrogate_destroy(EmDeeSurrogate* s) {
   s->w1t.release(); s->w1.release(); s->w2t.release(); s->w2.release(); s->Ss.release();
   s->cJ.release(); s->cN.release(); s->cOver.release(); s->rowSlot.release(); s->grpSpecies.release(); s->Rf.release();
   s->nearE.release(); s->nearG.release(); s->nearN.release(); s->nearOver.release();
-  s->Xi.release(); s->dEdD.release(); s->eat.release(); s->epart.release(); s->wpart.release(); s->eres.release();
+  s->Xi.release(); s->dEdD.release(); s->eat.release(); s->epart.release(); s->wpart.release();
   s->lowkAtom.release();
   dev_lowk_release(s->lowk);
   s->Fw.release(); s->red.release();
