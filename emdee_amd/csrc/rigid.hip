@@ -380,6 +380,105 @@ __device__ void rotate_exact(BodyReg& b, double dt) {
   mul43(B, twoIw, b.pi);
 }
 
+// rotate_exact over the two lanes h = 0, 1 of a body (lanes 2b, 2b+1 of one
+// wavefront).  Its longest chain runs u0 -> Jacobi functions at u -> theta at
+// sn; lane 0 takes that chain and the atan at sn, lane 1 takes K and theta
+// and the atan at s0.  The lanes swap their values and both finish with the
+// expressions of rotate_exact, in its order (theta at 1 only when the angle
+// wraps, as there), so both hold the same new state.
+__device__ __forceinline__ double swap_lane(double v) { return __shfl_xor(v, 1, 64); }
+__device__ void rotate_exact2(BodyReg& b, double dt, int h) {
+  double w0[3], Iw[3];
+  for (int k = 0; k < 3; k++) { w0[k] = b.omega[k]; Iw[k] = b.MoI[k] * w0[k]; }
+  double Lsq = Iw[1] * Iw[1] + Iw[2] * Iw[2];
+  if (Lsq < EPSD) { rotate_uniaxial(b, 1, dt); return; }
+  Lsq = Iw[0] * Iw[0] + Lsq;
+  double L = sqrt(Lsq), TwoKr = Iw[0] * w0[0] + Iw[1] * w0[1] + Iw[2] * w0[2];
+  double r1 = Lsq - TwoKr * b.MoI[2], r3 = TwoKr * b.MoI[0] - Lsq;
+  double l1 = b.I223 * r1, l3 = b.I212 * r3, lmin = fmin(l1, l3);
+  double a[3] = { sgn(1.0, w0[0]) * sqrt(b.I113 * r1), sqrt(lmin), sgn(1.0, w0[2]) * sqrt(b.I313 * r3) };
+  double m = lmin / fmax(l1, l3);
+  double s0 = w0[1] / a[1], c0, u0 = 0.0, K = 0.0;
+  const bool inner = fabs(s0) < 1.0;
+  if (h == 1 || !inner) K = carlson_rf(0.0, 1.0 - m, 1.0);
+  if (inner) {
+    c0 = (l1 < l3) ? w0[0] / a[0] : w0[2] / a[2];
+    if (h == 0) u0 = s0 * carlson_rf(1.0 - s0 * s0, 1.0 - m * s0 * s0, 1.0);
+  } else {
+    a[1] = fabs(w0[1]);
+    s0 = sgn(1.0, s0);
+    c0 = 0.0;
+    u0 = sgn(K, s0);
+  }
+  double wp = b.m312 * a[0] * a[2] / a[1];
+  double alpha = b.MoI[0] * a[0] / L;
+  const bool first = l1 < l3;
+  double eta, tp, C, d0 = 0.0;
+  if (first) {
+    d0 = w0[2] / a[2];
+    eta = alpha * alpha / (1.0 - alpha * alpha);
+    tp = eta;
+    C = sqrt(m + eta);
+  } else {
+    eta = alpha * alpha;
+    eta = eta / (1.0 - eta);
+    tp = m * eta;
+    C = sqrt(1.0 + tp);
+  }
+  // lane 0: x = {u0, sn, cn, dn, theta(sn), atan at sn}; lane 1: x = {K,
+  // theta(s0), atan at s0}
+  double x[6] = { 0.0, 0.0, 0.0, 0.0, 0.0, 0.0 };
+  if (h == 0) {
+    double jac[3];
+    jacobi(wp * dt + u0, m, jac);
+    x[0] = u0; x[1] = jac[0]; x[2] = jac[1]; x[3] = jac[2];
+    x[4] = theta_fn(jac[0], tp, m);
+    x[5] = atan(C * jac[0] / (first ? jac[2] : jac[1]));
+  } else {
+    x[0] = K;
+    x[1] = theta_fn(s0, tp, m);
+    x[2] = atan(C * s0 / (first ? d0 : c0));
+  }
+  double y[6];
+  for (int k = 0; k < 6; k++) y[k] = swap_lane(x[k]);
+  const double* p0 = h == 0 ? x : y;   // lane 0's values
+  const double* p1 = h == 0 ? y : x;   // lane 1's values
+  if (inner) u0 = p0[0];
+  K = p1[0];
+  double inv2K = 0.5 / K;
+  int i0 = inner ? staircase(u0 * inv2K) : 0;
+  double u = wp * dt + u0;
+  int jump = staircase(u * inv2K) - i0;
+  double sn = p0[1], cn = p0[2], dn = p0[3], deltaF;
+  if (first) {
+    b.omega[0] = a[0] * cn; b.omega[1] = a[1] * sn; b.omega[2] = a[2] * dn;
+    double r = u - u0 + sgn(1.0, cn) * p0[4] - sgn(1.0, c0) * p1[1] + (alpha / C) * (p0[5] - p1[2]);
+    if (jump != 0) r = r + jump * 2.0 * theta_fn(1.0, tp, m);
+    deltaF = (eta + 1.0) * r;
+  } else {
+    b.omega[0] = a[0] * dn; b.omega[1] = a[1] * sn; b.omega[2] = a[2] * cn;
+    double r = u - u0 + sgn(1.0, cn) * p0[4] - sgn(1.0, c0) * p1[1] + (alpha / C) * (p0[5] - p1[2]);
+    if (jump != 0) r = r + jump * (2.0 * theta_fn(1.0, tp, m) + (alpha / C) * PI_D);
+    deltaF = (eta + 1.0) * r;
+  }
+  double phi = (Lsq * (u - u0) + r3 * deltaF) / (2.0 * L * b.MoI[0] * wp);
+  double z0[4] = { Iw[2], Iw[1], L - Iw[0], 0.0 };
+  for (int k = 0; k < 3; k++) Iw[k] = b.MoI[k] * b.omega[k];
+  double cph = cos(phi), sph = sin(phi);
+  double z[4] = { Iw[2] * cph - Iw[1] * sph, Iw[1] * cph + Iw[2] * sph, (L - Iw[0]) * cph, (L - Iw[0]) * sph };
+  double Cz[12], Cz0[12], t3[3], t4[4], qn[4];
+  matC(z, Cz); matC(z0, Cz0);
+  mul34t(Cz0, b.q, t3);
+  mul43(Cz, t3, t4);
+  double dq = z0[0] * b.q[0] + z0[1] * b.q[1] + z0[2] * b.q[2] + z0[3] * b.q[3];
+  for (int k = 0; k < 4; k++) qn[k] = z[k] * dq + t4[k];
+  double nrm = sqrt(qn[0] * qn[0] + qn[1] * qn[1] + qn[2] * qn[2] + qn[3] * qn[3]);
+  for (int k = 0; k < 4; k++) b.q[k] = qn[k] / nrm;
+  double B[12], twoIw[3] = { 2.0 * Iw[0], 2.0 * Iw[1], 2.0 * Iw[2] };
+  matB(b.q, B);
+  mul43(B, twoIw, b.pi);
+}
+
 // delta_k = Ct(q) (B(q) d_k) for every member; R_k = rcm + delta_k when R given
 __device__ void place_members(const BodyState& B, int b, const double* q, double* R, const int* iperm, int Npad) {
   double Bm[12], Cm[12];
@@ -506,20 +605,25 @@ __global__ void k_body_boost(BodyState B, const double* __restrict__ F, const in
 // the rotation branch, as in the reference.
 // (256-thread launch bounds: the rotation's long FP64 chains keep their
 // values in registers instead of spilling at the default 1024-thread bound)
+// Two lanes per body (lanes 2b, 2b+1): the exact rotation splits its chain
+// over them (rotate_exact2); lane 0 stores.
 __global__ void __launch_bounds__(BLOCK) k_body_move(BodyState B, double* __restrict__ R, const int* __restrict__ iperm, double CR, double CP,
                             double dt, int translate, int rotate, int mode, int Npad) {
-  int b = blockIdx.x * blockDim.x + threadIdx.x;
-  if (b >= B.nb) return;
+  const int gt = blockIdx.x * blockDim.x + threadIdx.x, b = gt >> 1, h = gt & 1;
+  if (b >= B.nb) return;   // both lanes of a body
   int nb = B.nb;
-  if (translate)
+  if (translate && h == 0)
     for (int x = 0; x < 3; x++)
       B.rcm.p[x * nb + b] = CR * B.rcm.p[x * nb + b] + CP * B.invMass.p[b] * B.pcm.p[x * nb + b];
   if (rotate) {
     BodyReg r;
     load_body(B, b, r);
-    if (mode == 0) rotate_exact(r, dt); else rotate_no_squish(r, dt, mode);
-    store_motion(B, b, r);
-    place_members(B, b, r.q, R, iperm, Npad);
+    if (mode == 0) rotate_exact2(r, dt, h);
+    else if (h == 0) rotate_no_squish(r, dt, mode);
+    if (h == 0) {
+      store_motion(B, b, r);
+      place_members(B, b, r.q, R, iperm, Npad);
+    }
   }
 }
 
@@ -748,7 +852,7 @@ void dev_bodies_move(Device* d, double CR, double CP, double dt, bool translate,
   d->posWrites++;
   BodyState& B = d->bodies;
   if (B.nb == 0) return;
-  hipLaunchKernelGGL(k_body_move, dim3(div_up(B.nb, BLOCK)), dim3(BLOCK), 0, d->stream, B, d->R.p, d->iperm.p, CR, CP, dt,
+  hipLaunchKernelGGL(k_body_move, dim3(div_up(2L * B.nb, BLOCK)), dim3(BLOCK), 0, d->stream, B, d->R.p, d->iperm.p, CR, CP, dt,
                      (int)translate, (int)rotate, mode, d->Npad);
   launch_check("body move");
 }
