@@ -91,3 +91,40 @@ def test_million_atom_lj(gpu, oracle, mixed):
 def test_generic_one_thread_kernels(gpu, oracle, mixed):
     compare(gpu, oracle, 32, lambda lib: lib.EmDee_shifted_force(lib.EmDee_pair_lj_cut(1.0, 1.0)), mixed, 20,
             "lj_sf", 0.08)
+
+
+def cluster_box(nc, seed):
+    """An fcc box with its first 392 atoms moved onto a 7 x 7 x 8 grid of
+    spacing 0.3 sigma at the centre: those atoms hold about 470 list entries
+    each, about four times the list capacity the build starts from."""
+    R, L = fcc_box(nc, 0.8442, 0.02, seed=seed)
+    g = np.arange(7) * 0.3
+    cl = np.stack(np.meshgrid(g, g, np.arange(8) * 0.3, indexing="ij"), -1).reshape(-1, 3) + 0.5 * L + 0.05
+    R = np.array(R, copy=True)
+    R[: len(cl)] = cl
+    return R, L
+
+
+@pytest.mark.parametrize("nc,mixed", [(10, False), (10, True), (64, False), (64, True)])
+def test_list_capacity_overflow(gpu, oracle, nc, mixed):
+    # 4,000 atoms take the 16-lane build, 1,048,576 the one-lane segment
+    # build: both must drop the entries past the capacity, report the
+    # longest list and be rebuilt with room for it
+    R, L = cluster_box(nc, 29)
+    N = len(R)
+    model = lambda lib: lib.EmDee_pair_lj_cut(1.0, 1.0)
+    mg = make(gpu, R, L, model, mixed)
+    mo = make(oracle, R, L, model, False, THREADS)
+    gpu.compute_forces(mg)
+    oracle.compute_forces(mo)
+    kg, ko = pair_keys(gpu.neighbor_pairs(mg), N), pair_keys(oracle_pair_array(oracle, mo), N)
+    print(f"cluster N={N} mixed={mixed}: {len(kg)} pairs on the GPU, {len(ko)} in the oracle", flush=True)
+    assert len(kg) == len(ko) and np.array_equal(kg, ko)
+    ferr, eerr, _ = errors(gpu, oracle, mg, mo, N)
+    print(f"cluster N={N} mixed={mixed}: force rms {ferr:.2e} energy {eerr:.2e}", flush=True)
+    # The cluster's 0.3 sigma pairs dominate the energy.  The mixed form's
+    # positions are 32-bit fixed point (L / 2^32 = 2.5e-8 sigma at 1M atoms),
+    # a relative error of 8e-8 at 0.3 sigma, which r^-12 raises about twelve
+    # times: the energy bound is 1e-5 there (1e-6 for the FP64 form).
+    assert ferr < (FORCE_RMS if mixed else 1e-9) and eerr < (1e-5 if mixed else 1e-6)
+    gpu.EmDeeGPU_destroy(mg)
