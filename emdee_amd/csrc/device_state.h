@@ -122,9 +122,10 @@ __global__ void k_reduce_post(double* __restrict__ host, unsigned long long* __r
 // ------------------------------------------------------------ device state
 
 // Slots in the posted-results block (64 doubles): the pair sums, the Ewald
-// energy and virial, the learned model's energy, virial and overflow flag;
-// the other posts use slot 0.
-constexpr int POST_PAIRS = 8, POST_EWALD = 16, POST_MODEL = 24;
+// energy and virial, the learned model's energy, virial and overflow flag,
+// the rigid bodies' kinetic-energy terms and their virial; the other posts
+// use slot 0.
+constexpr int POST_PAIRS = 8, POST_EWALD = 16, POST_MODEL = 24, POST_BODY_KE = 32, POST_BODY_W = 40;
 
 template <typename T>
 struct DBuf {

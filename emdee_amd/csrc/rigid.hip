@@ -838,10 +838,12 @@ void dev_bodies_boost(Device* d, int layer, double CP, double CF, bool translate
   const double* F = d->F.p + 3 * (size_t)d->Npad * layer;
   if (twoKE6) {
     hipLaunchKernelGGL(k_body_boost<true>, dim3(g), dim3(BLOCK), 0, d->stream, B, F, d->iperm.p, CP, CF, (int)translate, (int)rotate, d->Npad);
-    hipLaunchKernelGGL(k_body_reduce<6>, dim3(1), dim3(BLOCK), 0, d->stream, d->result.p + 16, B.part.p, g);
-    HIP_CHECK(hipMemcpyAsync(d->hresult + 16, d->result.p + 16, 6 * sizeof(double), hipMemcpyDeviceToHost, d->stream));
-    HIP_CHECK(hipStreamSynchronize(d->stream));
-    for (int k = 0; k < 6; k++) twoKE6[k] = d->hresult[16 + k];
+    // the totals straight into pinned host memory (same fixed-order sums)
+    const unsigned long long seq = ++d->seq;
+    hipLaunchKernelGGL(k_reduce_post<6>, dim3(1), dim3(BLOCK), 0, d->stream, d->hpostDev + POST_BODY_KE, d->hseqDev, seq,
+                       B.part.p, g);
+    dev_post_wait(d, seq);
+    for (int k = 0; k < 6; k++) twoKE6[k] = d->hpost[POST_BODY_KE + k];
   } else {
     hipLaunchKernelGGL(k_body_boost<false>, dim3(g), dim3(BLOCK), 0, d->stream, B, F, d->iperm.p, CP, CF, (int)translate, (int)rotate, d->Npad);
   }
@@ -862,10 +864,11 @@ double dev_bodies_virial(Device* d, int layer) {
   if (B.nb == 0) return 0.0;
   int g = div_up(B.nb, BLOCK);
   hipLaunchKernelGGL(k_body_virial, dim3(g), dim3(BLOCK), 0, d->stream, B, d->F.p + 3 * (size_t)d->Npad * layer, d->iperm.p, d->Npad);
-  hipLaunchKernelGGL(k_body_reduce<1>, dim3(1), dim3(BLOCK), 0, d->stream, d->result.p + 24, B.part.p, g);
-  HIP_CHECK(hipMemcpyAsync(d->hresult + 24, d->result.p + 24, sizeof(double), hipMemcpyDeviceToHost, d->stream));
-  HIP_CHECK(hipStreamSynchronize(d->stream));
-  return -d->hresult[24];
+  const unsigned long long seq = ++d->seq;
+  hipLaunchKernelGGL(k_reduce_post<1>, dim3(1), dim3(BLOCK), 0, d->stream, d->hpostDev + POST_BODY_W, d->hseqDev, seq,
+                     B.part.p, g);
+  dev_post_wait(d, seq);
+  return -d->hpost[POST_BODY_W];
 }
 
 void dev_bodies_assign_momenta(Device* d, double* twoKE6) {
@@ -875,10 +878,11 @@ void dev_bodies_assign_momenta(Device* d, double* twoKE6) {
   if (B.nb == 0) return;
   int g = div_up(B.nb, BLOCK);
   hipLaunchKernelGGL(k_body_assign, dim3(g), dim3(BLOCK), 0, d->stream, B, d->stage.p);
-  hipLaunchKernelGGL(k_body_reduce<6>, dim3(1), dim3(BLOCK), 0, d->stream, d->result.p + 16, B.part.p, g);
-  HIP_CHECK(hipMemcpyAsync(d->hresult + 16, d->result.p + 16, 6 * sizeof(double), hipMemcpyDeviceToHost, d->stream));
-  HIP_CHECK(hipStreamSynchronize(d->stream));
-  for (int k = 0; k < 6; k++) twoKE6[k] = d->hresult[16 + k];
+  const unsigned long long seq = ++d->seq;
+  hipLaunchKernelGGL(k_reduce_post<6>, dim3(1), dim3(BLOCK), 0, d->stream, d->hpostDev + POST_BODY_KE, d->hseqDev, seq,
+                     B.part.p, g);
+  dev_post_wait(d, seq);
+  for (int k = 0; k < 6; k++) twoKE6[k] = d->hpost[POST_BODY_KE + k];
 }
 
 void dev_bodies_particle_momenta(Device* d) {
