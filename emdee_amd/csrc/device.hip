@@ -1759,6 +1759,8 @@ Device* dev_create(const DeviceSetup& s) {
   for (int i = 0; i < s.N; i++) { im[i] = s.invMass[i]; q[i] = s.charge[i]; free_[i] = s.freeFlag[i]; chg[i] = std::fabs(s.charge[i]) > 2.220446049250313e-16; }
   d->uniformFree = s.N > 0;
   for (int i = 0; i < s.N; i++) d->uniformFree = d->uniformFree && free_[i] == 1 && im[i] == im[0];
+  d->nfree = 0;
+  for (int i = 0; i < s.N; i++) d->nfree += free_[i] != 0;
   d->invMass0 = s.N > 0 ? im[0] : 1.0;
   HIP_CHECK(hipMemcpyAsync(d->type.p, s.type.data(), sizeof(int) * s.N, hipMemcpyHostToDevice, d->stream));
   HIP_CHECK(hipMemcpyAsync(d->body.p, s.body.data(), sizeof(int) * s.N, hipMemcpyHostToDevice, d->stream));
@@ -2274,7 +2276,9 @@ void dev_pairs_result(Device* d, ForceSums* out) {
 
 void dev_boost(Device* d, int layer, double CP, double CF, bool translate, double* twoKE) {
   int N = d->N, Np = d->Npad;
-  if (N == 0) { if (twoKE) twoKE[0] = twoKE[1] = twoKE[2] = 0; return; }
+  // no free atoms (all in rigid bodies): nothing to kick, and their kinetic
+  // energy is the sum of no terms
+  if (N == 0 || d->nfree == 0) { if (twoKE) twoKE[0] = twoKE[1] = twoKE[2] = 0; return; }
   int g = div_up(N, BLOCK);
   const double* F = d->F.p + 3 * (size_t)Np * layer;
   const int g2 = div_up(div_up(N, 2), BLOCK);
@@ -2307,7 +2311,7 @@ void dev_kinetic(Device* d, double* twoKE) { dev_boost(d, 0, 1.0, 0.0, false, tw
 
 void dev_displace(Device* d, double CR, double CP) {
   int N = d->N;
-  if (N == 0) return;
+  if (N == 0 || d->nfree == 0) return;   // no free atom moves (rigid bodies move in dev_bodies_move)
   d->posWrites++;
   if (d->uniformFree && d->trigKnown) {
     // the drift with the next trigger's pass A, in the last trigger's set-up

@@ -189,6 +189,7 @@ struct Device {
   int nexcl = 0;
   bool allInteract = false;   // every type pair has a model (interaction table all ones)
   bool uniformFree = false;   // every atom free and of one mass: the integrator skips the per-atom reads
+  int nfree = 0;              // atoms in no rigid body: none, and the kick and drift of free atoms are skipped
   double invMass0 = 1.0;      // that mass's inverse
   std::vector<int> hostType;
   // reductions
