@@ -58,6 +58,7 @@ struct SharedPhase {
 struct System {
   int natoms = 0, ntypes = 1, nlayers = 1, layer = 1, nthreads = 1;
   int nbodies = 0, nfree = 0;
+  bool pairsQueued = false;   // forces_begin queued the pair kernels for forces_end to read
   double Rc = 0, skin = 0, RcSq = 0, xRc = 0, xRcSq = 0, skinSq = 0, InRc = 0, InRcSq = 0, xInRcSq = 0;
   double totalMass = 0, startTime = 0;
   bool hasL = false, hasR = false, initialized = false;
@@ -345,7 +346,13 @@ static bool forces_begin(tEmDee* md, bool defer, ForceSums& fs) {
   fs = ForceSums{ 0, 0, 0, 0 };
   bool pairsOnly = me->nbonds + me->nangles + me->ndihedrals == 0 && !me->coul[l - 1].requires_kspace &&
                    !(me->surrogate && me->surrogateLayer == l) && me->nbodies == 0;
-  c.defer = defer && pairsOnly && me->pairs_exist[l];
+  // pairs only: the caller may read the pair sums later (after its kick).
+  // Pairs, Ewald and rigid bodies only: forces_end queues the k-space and
+  // body kernels behind the pair kernels and reads all three after one wait.
+  bool toCaller = defer && pairsOnly && me->pairs_exist[l];
+  me->pairsQueued = !pairsOnly && me->pairs_exist[l] && me->nbonds + me->nangles + me->ndihedrals == 0 &&
+                    !(me->surrogate && me->surrogateLayer == l);
+  c.defer = toCaller || me->pairsQueued;
   // handle_neighbor_lists: neighbor_lists.f90:175-195.  When the recent
   // trigger values make a rebuild unlikely, the pair kernels are queued
   // behind the trigger kernels before the answer is read, so the GPU does
@@ -373,19 +380,31 @@ static bool forces_begin(tEmDee* md, bool defer, ForceSums& fs) {
     std::vector<double> zero(3 * (size_t)me->natoms, 0.0);
     dev_upload_forces(me->dev, l - 1, zero.data());
   }
-  return c.defer != 0;
+  return toCaller;
 }
 
-static void forces_end(tEmDee* md, const ForceSums& fs) {
+static void forces_end(tEmDee* md, const ForceSums& fs0) {
   System* me = S(*md);
   double E[6] = { 0, 0, 0, 0, 0, 0 }, W[6] = { 0, 0, 0, 0, 0, 0 };
   enum { PAIR_ = 0, COUL_ = 1, LONG_ = 2, BOND_ = 3, ANGLE_ = 4, DIHEDRAL_ = 5 };
   bool compute = md->Options.Compute || me->atomEnergies;
   int l = me->layer;
+  bool kspace = me->coul[l - 1].requires_kspace;
+  ForceSums fs = fs0;
+  const bool queued = me->pairsQueued;
+  bool ewaldQueued = false;
+  if (queued) {
+    // the pair kernels are queued: the k-space sum and the body virial go
+    // behind them, and the three results are read after one wait
+    me->pairsQueued = false;
+    if (kspace) ewaldQueued = dev_ewald_launch(me->dev, l - 1, me->L);
+    if (me->nbodies != 0) dev_bodies_virial_launch(me->dev, l - 1);
+    dev_wait_posts(me->dev);
+    dev_pairs_result(me->dev, &fs);
+  }
   E[PAIR_] = fs.Epair; E[COUL_] = fs.Ecoul; W[PAIR_] = fs.Wpair; W[COUL_] = fs.Wcoul;
   // compute_bonds / compute_angles, EmDeeData.f90:443-550, and the dihedrals
   // (EmDeeData.f90:554-640, which the reference does not call)
-  bool kspace = me->coul[l - 1].requires_kspace;
   if (me->nbonds + me->nangles + me->ndihedrals > 0 && (me->bonded[l] || kspace)) {
     double o[7];
     dev_structs_compute(me->dev, l - 1, me->L, me->ntypes, me->bonded[l], kspace, me->ewald.alpha, me->ewald.beta, o);
@@ -395,7 +414,8 @@ static void forces_end(tEmDee* md, const ForceSums& fs) {
   }
   // compute_kspace, EmDeeData.f90:689-700; virial EmDeeCode.f90:1248-1252
   if (kspace) {
-    E[LONG_] = dev_ewald_compute(me->dev, l - 1, me->L) + me->ewald.rigidEnergy[l - 1];
+    const double ek = queued ? (ewaldQueued ? dev_ewald_energy(me->dev) : 0.0) : dev_ewald_compute(me->dev, l - 1, me->L);
+    E[LONG_] = ek + me->ewald.rigidEnergy[l - 1];
     W[LONG_] = E[COUL_] + E[LONG_] - W[COUL_];
   }
   // learned model standing in for a costly term of this layer (no reference counterpart)
@@ -406,7 +426,7 @@ static void forces_end(tEmDee* md, const ForceSums& fs) {
   }
   md->Virial = W[0] + W[1] + W[2] + W[3] + W[4] + W[5];
   if (me->nbodies != 0) {
-    md->BodyVirial = dev_bodies_virial(me->dev, l - 1);
+    md->BodyVirial = queued ? dev_bodies_virial_read(me->dev) : dev_bodies_virial(me->dev, l - 1);
     md->Virial = md->Virial + md->BodyVirial;
   }
   if (compute) {

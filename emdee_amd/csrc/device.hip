@@ -62,6 +62,7 @@ static void post_wait(Device* d, unsigned long long seq) {
 }
 
 void dev_post_wait(Device* d, unsigned long long seq) { post_wait(d, seq); }
+void dev_wait_posts(Device* d) { post_wait(d, d->seq); }
 
 __global__ void k_iota(int* p, int n) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;

@@ -146,6 +146,11 @@ void dev_bodies_update(Device* d, double L);          // update_rigid_bodies (un
 void dev_bodies_boost(Device* d, int layer, double CP, double CF, bool translate, bool rotate, double* twoKE6);
 void dev_bodies_move(Device* d, double CR, double CP, double dt, bool translate, bool rotate, int mode);
 double dev_bodies_virial(Device* d, int layer);
+// The same in two halves: queue it, then read it after dev_wait_posts.
+void dev_bodies_virial_launch(Device* d, int layer);
+double dev_bodies_virial_read(const Device* d);
+// Waits until every result posted so far has arrived in host memory.
+void dev_wait_posts(Device* d);
 void dev_bodies_assign_momenta(Device* d, double* twoKE6);   // from momenta staged by dev_upload_momenta
 void dev_bodies_particle_momenta(Device* d);                 // into the staging buffer
 void dev_bodies_place(Device* d);                            // member positions from rcm, q

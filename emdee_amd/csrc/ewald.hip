@@ -361,16 +361,16 @@ void dev_ewald_setup(Device* d, const EwaldSetup& s) {
 // weights lam and self-energy coefficients selfc (null: none): forces added
 // to F (slot order), atom shares to eatom (may be null); returns the energy
 // and, with the state's virial factors, its virial in *W.
-static double ewald_eval(Device* d, EwaldState* e, double* F, const double* lam, const double* selfc, double* eatom,
-                         double L, double* W) {
+static unsigned long long ewald_launch(Device* d, EwaldState* e, double* F, const double* lam, const double* selfc,
+                                       double* eatom, double L, bool virial) {
   // shapes the kernels assume
   if (e->nslots % EWALD_GROUP || e->ngroups * EWALD_GROUP != e->nslots || e->kchS * e->ksS < e->nvecs ||
       e->kchF * e->ksF < e->nvecs || e->nm < 1)
     fatal("kspace computation", "inconsistent k-space set-up");
   if (eatom && !selfc) fatal("kspace computation", "atom energies need the self-energy coefficients");
   int Np = d->Npad;
-  const double* vf = W ? e->vfac.p : nullptr;
-  if (W && !vf) fatal("kspace computation", "no virial factors for this k set");
+  const double* vf = virial ? e->vfac.p : nullptr;
+  if (virial && !vf) fatal("kspace computation", "no virial factors for this k set");
   hipLaunchKernelGGL(k_ew_tables, dim3(div_up(e->nslots, BLOCK)), dim3(BLOCK), 0, d->stream, e->T.p, e->item.p,
                      d->iperm.p, d->R.p, L, e->nm, e->nslots, Np);
   launch_check("ewald tables");
@@ -398,17 +398,32 @@ static double ewald_eval(Device* d, EwaldState* e, double* F, const double* lam,
                      eatom, e->Fpart.p,
                      e->item.p, d->iperm.p, e->q.p, e->groupType.p, selfc, e->nslots, e->ksF, Np);
   launch_check("ewald force add");
-  dev_post_wait(d, seq);
+  return seq;
+}
+
+// the same, read once its energy (and virial) have been posted
+static double ewald_eval(Device* d, EwaldState* e, double* F, const double* lam, const double* selfc, double* eatom,
+                         double L, double* W) {
+  dev_post_wait(d, ewald_launch(d, e, F, lam, selfc, eatom, L, W != nullptr));
   if (W) *W = d->hpost[POST_EWALD + 1];
   return d->hpost[POST_EWALD];
 }
 
-double dev_ewald_compute(Device* d, int layer, double L) {
+bool dev_ewald_launch(Device* d, int layer, double L) {
   EwaldState* e = d->ewald;
-  if (!e || e->nvecs == 0 || e->nslots == 0) return 0.0;
+  if (!e || e->nvecs == 0 || e->nslots == 0) return false;
   double* F = d->F.p + 3 * (size_t)d->Npad * layer;
-  return ewald_eval(d, e, F, e->lambda.p + (size_t)layer * e->nt * e->nt, e->selfc.p + (size_t)layer * e->nt,
-                    d->wantAtomE ? d->eatom.p : nullptr, L, nullptr);
+  ewald_launch(d, e, F, e->lambda.p + (size_t)layer * e->nt * e->nt, e->selfc.p + (size_t)layer * e->nt,
+               d->wantAtomE ? d->eatom.p : nullptr, L, false);
+  return true;
+}
+
+double dev_ewald_energy(const Device* d) { return d->hpost[POST_EWALD]; }
+
+double dev_ewald_compute(Device* d, int layer, double L) {
+  if (!dev_ewald_launch(d, layer, L)) return 0.0;
+  dev_wait_posts(d);
+  return dev_ewald_energy(d);
 }
 
 // A reciprocal-space sum over a partial k set (set-up by lowk_setup): one

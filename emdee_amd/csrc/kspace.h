@@ -52,6 +52,10 @@ double inverse_of_x_plus_ln_x(double y);
 // energy sum (kspace_ewald.f90:188-314).
 void dev_ewald_setup(Device* d, const EwaldSetup& s);
 double dev_ewald_compute(Device* d, int layer, double L);
+// The same in two halves: queue the sum (false: nothing to sum), then, once
+// dev_wait_posts has returned, its energy.
+bool dev_ewald_launch(Device* d, int layer, double L);
+double dev_ewald_energy(const Device* d);
 
 // A partial k set: the vectors of the same half space with |k| <= kcut, the
 // charges and slots of `full` and its kCoul weights of `layer` (1-based).

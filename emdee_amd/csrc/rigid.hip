@@ -859,16 +859,23 @@ void dev_bodies_move(Device* d, double CR, double CP, double dt, bool translate,
   launch_check("body move");
 }
 
-double dev_bodies_virial(Device* d, int layer) {
+void dev_bodies_virial_launch(Device* d, int layer) {
   BodyState& B = d->bodies;
-  if (B.nb == 0) return 0.0;
+  if (B.nb == 0) { d->hpost[POST_BODY_W] = 0.0; return; }
   int g = div_up(B.nb, BLOCK);
   hipLaunchKernelGGL(k_body_virial, dim3(g), dim3(BLOCK), 0, d->stream, B, d->F.p + 3 * (size_t)d->Npad * layer, d->iperm.p, d->Npad);
   const unsigned long long seq = ++d->seq;
   hipLaunchKernelGGL(k_reduce_post<1>, dim3(1), dim3(BLOCK), 0, d->stream, d->hpostDev + POST_BODY_W, d->hseqDev, seq,
                      B.part.p, g);
-  dev_post_wait(d, seq);
-  return -d->hpost[POST_BODY_W];
+}
+
+double dev_bodies_virial_read(const Device* d) { return -d->hpost[POST_BODY_W]; }
+
+double dev_bodies_virial(Device* d, int layer) {
+  if (d->bodies.nb == 0) return 0.0;
+  dev_bodies_virial_launch(d, layer);
+  dev_wait_posts(d);
+  return dev_bodies_virial_read(d);
 }
 
 void dev_bodies_assign_momenta(Device* d, double* twoKE6) {
