@@ -347,11 +347,11 @@ static bool forces_begin(tEmDee* md, bool defer, ForceSums& fs) {
   bool pairsOnly = me->nbonds + me->nangles + me->ndihedrals == 0 && !me->coul[l - 1].requires_kspace &&
                    !(me->surrogate && me->surrogateLayer == l) && me->nbodies == 0;
   // pairs only: the caller may read the pair sums later (after its kick).
-  // Pairs, Ewald and rigid bodies only: forces_end queues the k-space and
-  // body kernels behind the pair kernels and reads all three after one wait.
+  // Pairs with Ewald, a learned model or rigid bodies (no bonded terms):
+  // forces_end queues their kernels behind the pair kernels and reads every
+  // result after one wait.
   bool toCaller = defer && pairsOnly && me->pairs_exist[l];
-  me->pairsQueued = !pairsOnly && me->pairs_exist[l] && me->nbonds + me->nangles + me->ndihedrals == 0 &&
-                    !(me->surrogate && me->surrogateLayer == l);
+  me->pairsQueued = !pairsOnly && me->pairs_exist[l] && me->nbonds + me->nangles + me->ndihedrals == 0;
   c.defer = toCaller || me->pairsQueued;
   // handle_neighbor_lists: neighbor_lists.f90:175-195.  When the recent
   // trigger values make a rebuild unlikely, the pair kernels are queued
@@ -398,6 +398,8 @@ static void forces_end(tEmDee* md, const ForceSums& fs0) {
     // behind them, and the three results are read after one wait
     me->pairsQueued = false;
     if (kspace) ewaldQueued = dev_ewald_launch(me->dev, l - 1, me->L);
+    if (me->surrogate && me->surrogateLayer == l)
+      dev_surrogate_launch(me->dev, me->surrogate, l - 1, surrogate_context(me));
     if (me->nbodies != 0) dev_bodies_virial_launch(me->dev, l - 1);
     dev_wait_posts(me->dev);
     dev_pairs_result(me->dev, &fs);
@@ -420,7 +422,8 @@ static void forces_end(tEmDee* md, const ForceSums& fs0) {
   }
   // learned model standing in for a costly term of this layer (no reference counterpart)
   if (me->surrogate && me->surrogateLayer == l) {
-    double Ws = 0, Es = dev_surrogate_apply(me->dev, me->surrogate, l - 1, surrogate_context(me), &Ws);
+    double Ws = 0, Es = queued ? dev_surrogate_result(me->dev, me->surrogate, &Ws)
+                               : dev_surrogate_apply(me->dev, me->surrogate, l - 1, surrogate_context(me), &Ws);
     E[COUL_] += Es;
     W[COUL_] += Ws;
   }

@@ -123,9 +123,9 @@ __global__ void k_reduce_post(double* __restrict__ host, unsigned long long* __r
 
 // Slots in the posted-results block (64 doubles): the pair sums, the Ewald
 // energy and virial, the learned model's energy, virial and overflow flag,
-// the rigid bodies' kinetic-energy terms and their virial; the other posts
-// use slot 0.
-constexpr int POST_PAIRS = 8, POST_EWALD = 16, POST_MODEL = 24, POST_BODY_KE = 32, POST_BODY_W = 40;
+// the rigid bodies' kinetic-energy terms and their virial, the model's
+// low-k energy and virial; the other posts use slot 0.
+constexpr int POST_PAIRS = 8, POST_EWALD = 16, POST_MODEL = 24, POST_BODY_KE = 32, POST_BODY_W = 40, POST_LOWK = 48;
 
 template <typename T>
 struct DBuf {

@@ -362,7 +362,7 @@ void dev_ewald_setup(Device* d, const EwaldSetup& s) {
 // to F (slot order), atom shares to eatom (may be null); returns the energy
 // and, with the state's virial factors, its virial in *W.
 static unsigned long long ewald_launch(Device* d, EwaldState* e, double* F, const double* lam, const double* selfc,
-                                       double* eatom, double L, bool virial) {
+                                       double* eatom, double L, bool virial, int slot) {
   // shapes the kernels assume
   if (e->nslots % EWALD_GROUP || e->ngroups * EWALD_GROUP != e->nslots || e->kchS * e->ksS < e->nvecs ||
       e->kchF * e->ksF < e->nvecs || e->nm < 1)
@@ -385,10 +385,10 @@ static unsigned long long ewald_launch(Device* d, EwaldState* e, double* F, cons
   // them as soon as they arrive, while the force kernels below still run
   const unsigned long long seq = ++d->seq;
   if (vf)
-    hipLaunchKernelGGL(k_reduce_post<2>, dim3(1), dim3(BLOCK), 0, d->stream, d->hpostDev + POST_EWALD, d->hseqDev, seq,
+    hipLaunchKernelGGL(k_reduce_post<2>, dim3(1), dim3(BLOCK), 0, d->stream, d->hpostDev + slot, d->hseqDev, seq,
                        e->part.p, gk);
   else
-    hipLaunchKernelGGL(k_reduce_post<1>, dim3(1), dim3(BLOCK), 0, d->stream, d->hpostDev + POST_EWALD, d->hseqDev, seq,
+    hipLaunchKernelGGL(k_reduce_post<1>, dim3(1), dim3(BLOCK), 0, d->stream, d->hpostDev + slot, d->hseqDev, seq,
                        e->part.p, gk);
   hipLaunchKernelGGL(k_ew_forces, dim3(e->nslots / EW_BLOCK, e->ksF), dim3(EW_BLOCK), 0, d->stream, e->Fpart.p,
                      e->T.p, e->q.p, e->kn.p, e->kv.p, e->sigma.p, e->groupType.p, e->nm, e->nslots, e->nvecs,
@@ -403,10 +403,10 @@ static unsigned long long ewald_launch(Device* d, EwaldState* e, double* F, cons
 
 // the same, read once its energy (and virial) have been posted
 static double ewald_eval(Device* d, EwaldState* e, double* F, const double* lam, const double* selfc, double* eatom,
-                         double L, double* W) {
-  dev_post_wait(d, ewald_launch(d, e, F, lam, selfc, eatom, L, W != nullptr));
-  if (W) *W = d->hpost[POST_EWALD + 1];
-  return d->hpost[POST_EWALD];
+                         double L, double* W, int slot) {
+  dev_post_wait(d, ewald_launch(d, e, F, lam, selfc, eatom, L, W != nullptr, slot));
+  if (W) *W = d->hpost[slot + 1];
+  return d->hpost[slot];
 }
 
 bool dev_ewald_launch(Device* d, int layer, double L) {
@@ -414,7 +414,7 @@ bool dev_ewald_launch(Device* d, int layer, double L) {
   if (!e || e->nvecs == 0 || e->nslots == 0) return false;
   double* F = d->F.p + 3 * (size_t)d->Npad * layer;
   ewald_launch(d, e, F, e->lambda.p + (size_t)layer * e->nt * e->nt, e->selfc.p + (size_t)layer * e->nt,
-               d->wantAtomE ? d->eatom.p : nullptr, L, false);
+               d->wantAtomE ? d->eatom.p : nullptr, L, false, POST_EWALD);
   return true;
 }
 
@@ -448,7 +448,18 @@ EwaldState* dev_lowk_create(const EwaldSetup& s) {
 
 double dev_lowk_compute(Device* d, EwaldState* e, double* F, double* eatom, double L, double* W) {
   if (!e || e->nvecs == 0 || e->nslots == 0) { *W = 0.0; return 0.0; }
-  return ewald_eval(d, e, F, e->lambda.p, e->selfc.p, eatom, L, W);
+  return ewald_eval(d, e, F, e->lambda.p, e->selfc.p, eatom, L, W, POST_LOWK);
+}
+
+bool dev_lowk_launch(Device* d, EwaldState* e, double* F, double* eatom, double L) {
+  if (!e || e->nvecs == 0 || e->nslots == 0) return false;
+  ewald_launch(d, e, F, e->lambda.p, e->selfc.p, eatom, L, true, POST_LOWK);
+  return true;
+}
+
+double dev_lowk_result(const Device* d, double* W) {
+  *W = d->hpost[POST_LOWK + 1];
+  return d->hpost[POST_LOWK];
 }
 
 int dev_lowk_vectors(const EwaldState* e) { return e ? e->nvecs : 0; }

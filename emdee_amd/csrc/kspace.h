@@ -65,6 +65,10 @@ EwaldSetup lowk_setup(const EwaldSetup& full, int layer, double alpha, double L,
 struct EwaldState;
 EwaldState* dev_lowk_create(const EwaldSetup& s);
 double dev_lowk_compute(Device* d, EwaldState* e, double* F, double* eatom, double L, double* W);
+// The same in two halves: queue (false: nothing to sum), then read after
+// dev_wait_posts.
+bool dev_lowk_launch(Device* d, EwaldState* e, double* F, double* eatom, double L);
+double dev_lowk_result(const Device* d, double* W);
 int dev_lowk_vectors(const EwaldState* e);
 void dev_lowk_release(EwaldState* e);
 

@@ -21,6 +21,10 @@ struct SurrogateContext {
 // the atom-energy array when that is on) and returns the model energy; W
 // receives its virial.
 double dev_surrogate_apply(Device* d, EmDeeSurrogate* s, int layer, const SurrogateContext& c, double* W);
+// The same in two halves: queue the network and the low-k sum, then, after
+// dev_wait_posts, read the model energy and its virial.
+void dev_surrogate_launch(Device* d, EmDeeSurrogate* s, int layer, const SurrogateContext& c);
+double dev_surrogate_result(Device* d, EmDeeSurrogate* s, double* W);
 // The model's descriptor cutoff.
 double surrogate_cutoff(const EmDeeSurrogate* s);
 // Stores the descriptors of the current configuration with per-atom labels

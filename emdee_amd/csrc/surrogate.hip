@@ -113,6 +113,7 @@ struct EmDeeSurrogate {
   // exact long-range channel: the reciprocal sum over |k| <= kcut (0: none)
   double kcut = 0.0;
   EwaldState* lowk = nullptr;
+  bool queued = false, lowkQueued = false;   // dev_surrogate_launch ran; it queued the low-k sum
   const void* lowkOwner = nullptr;
   double lowkL = 0.0;
   DBuf<double> lowkAtom;         // low-k atom energies of a sample (slot order)
@@ -1074,11 +1075,30 @@ __global__ void k_sg_add_atom(double* __restrict__ dst, const float* __restrict_
 // (slot order) and its atom energies to the atom-energy array when that is
 // on; returns the model energy (low-k sum plus network) and its virial in *W.
 double dev_surrogate_apply(Device* d, EmDeeSurrogate* s, int layer, const SurrogateContext& c, double* W) {
+  dev_surrogate_launch(d, s, layer, c);
+  dev_wait_posts(d);
+  return dev_surrogate_result(d, s, W);
+}
+
+double dev_surrogate_result(Device* d, EmDeeSurrogate* s, double* W) {
+  *W = 0.0;
+  if (!s->queued) return 0.0;
+  s->queued = false;
+  double El = 0.0, Wl = 0.0;
+  if (s->lowkQueued) El = dev_lowk_result(d, &Wl);
+  const double* hr = d->hpost + POST_MODEL;
+  if (hr[2] != 0.0) fatal("surrogate", "more neighbours within the descriptor cutoff than the model holds (256)");
+  *W = hr[1] + Wl;
+  return hr[0] + El;
+}
+
+void dev_surrogate_launch(Device* d, EmDeeSurrogate* s, int layer, const SurrogateContext& c) {
   const char* task = "surrogate";
   if (!s->normalised) fatal(task, "the model has not been trained");
   int N = d->N, Np = d->Npad, din = s->sp.din;
-  *W = 0.0;
-  if (N == 0) return 0.0;
+  s->queued = false;
+  s->lowkQueued = false;
+  if (N == 0) return;
   check_context(s, d, c, task);
   s->stream = d->stream;
   double* F = d->F.p + 3 * (size_t)Np * layer;
@@ -1114,16 +1134,10 @@ double dev_surrogate_apply(Device* d, EmDeeSurrogate* s, int layer, const Surrog
   if (d->wantAtomE)
     hipLaunchKernelGGL(k_sg_add_atom, dim3(div_up(N, BLOCK)), dim3(BLOCK), 0, d->stream, d->eatom.p, s->eat.p, N);
   launch_check("surrogate sums");
-  double El = 0.0, Wl = 0.0;
-  // the low-k evaluation waits for its own post, which arrives after this one
+  (void)seq;
   if (EwaldState* lk = ensure_lowk(s, d, c, task))
-    El = dev_lowk_compute(d, lk, F, d->wantAtomE ? d->eatom.p : nullptr, c.L, &Wl);
-  else
-    dev_post_wait(d, seq);
-  const double* hr = d->hpost + POST_MODEL;
-  if (hr[2] != 0.0) fatal(task, "more neighbours within the descriptor cutoff than the model holds (256)");
-  *W = hr[1] + Wl;
-  return hr[0] + El;
+    s->lowkQueued = dev_lowk_launch(d, lk, F, d->wantAtomE ? d->eatom.p : nullptr, c.L);
+  s->queued = true;
 }
 
 // Stores the raw descriptors of a system's current configuration as training
