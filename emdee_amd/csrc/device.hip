@@ -473,6 +473,33 @@ constexpr int CELL_REACH = 3;
 // S > 1: the S consecutive lanes of an atom take S consecutive candidates at
 // a time; a ballot over the group places the accepted ones, so the list
 // keeps the ascending order of the one-lane scan (for long lists: water).
+// Pair test of the multi-lane scans: the fixed-point prefilter with the FP64
+// test in the band, then (unless SIMPLE) the body, type and exclusion filters
+// for the candidates in range.
+template <bool SIMPLE>
+__device__ __forceinline__ bool list_accept(int j, int i, int4 qi, int ti, int bi, const int4* __restrict__ Q4,
+                                            const double4* __restrict__ Rs4, const int* __restrict__ body,
+                                            const int* __restrict__ perm, const uint8_t* __restrict__ interact,
+                                            const int* __restrict__ exItem, int e0, int e1, int ntypes, double xRc2,
+                                            float lo, float hi) {
+  int4 qj = Q4[j];
+  float dx = (float)(int)((unsigned int)qi.x - (unsigned int)qj.x);
+  float dy = (float)(int)((unsigned int)qi.y - (unsigned int)qj.y);
+  float dz = (float)(int)((unsigned int)qi.z - (unsigned int)qj.z);
+  float r2 = dx * dx + dy * dy + dz * dz;
+  bool odd = ((qi.w | qj.w) & Q4_NONFINITE) != 0;
+  bool in = r2 < lo && !odd;
+  if (!in && (r2 < hi || odd)) in = exact_inside(Rs4, i, j, xRc2);
+  if (SIMPLE) return in && j != i;
+  if (!in || j == i) return false;
+  if (body[j] == bi) return false;
+  if (interact && !interact[(qj.w & Q4_TYPE) * ntypes + ti]) return false;   // null: all types interact
+  if (e1 > e0) {
+    int oj = perm[j];
+    for (int e = e0; e < e1; e++) if (exItem[e] == oj) return false;
+  }
+  return true;
+}
 template <bool SIMPLE, int S>
 __device__ __forceinline__ void scan_range(int a, int b, int i, double xi, double yi, double zi, int ti, int bi, int oi,
                                            const double4* __restrict__ Rs4, const int* __restrict__ body,
@@ -482,35 +509,7 @@ __device__ __forceinline__ void scan_range(int a, int b, int i, double xi, doubl
                                            const int4* __restrict__ Q4, int4 qi, float lo, float hi) {
   if (S > 1) {
     auto accept = [&](int j) -> bool {
-      if (SIMPLE) {
-        int4 qj = Q4[j];
-        float dx = (float)(int)((unsigned int)qi.x - (unsigned int)qj.x);
-        float dy = (float)(int)((unsigned int)qi.y - (unsigned int)qj.y);
-        float dz = (float)(int)((unsigned int)qi.z - (unsigned int)qj.z);
-        float r2 = dx * dx + dy * dy + dz * dz;
-        bool odd = ((qi.w | qj.w) & Q4_NONFINITE) != 0;
-        bool in = r2 < lo && !odd;
-        if (!in && (r2 < hi || odd)) in = exact_inside(Rs4, i, j, xRc2);
-        return in && j != i;
-      }
-      // the same fixed-point prefilter with the FP64 test in the band, then
-      // the body, type and exclusion filters for the candidates in range
-      int4 qj = Q4[j];
-      float dx = (float)(int)((unsigned int)qi.x - (unsigned int)qj.x);
-      float dy = (float)(int)((unsigned int)qi.y - (unsigned int)qj.y);
-      float dz = (float)(int)((unsigned int)qi.z - (unsigned int)qj.z);
-      float r2 = dx * dx + dy * dy + dz * dz;
-      bool odd = ((qi.w | qj.w) & Q4_NONFINITE) != 0;
-      bool in = r2 < lo && !odd;
-      if (!in && (r2 < hi || odd)) in = exact_inside(Rs4, i, j, xRc2);
-      if (!in || j == i) return false;
-      if (body[j] == bi) return false;
-      if (interact && !interact[(qj.w & Q4_TYPE) * ntypes + ti]) return false;   // null: all types interact
-      if (e1 > e0) {
-        int oj = perm[j];
-        for (int e = e0; e < e1; e++) if (exItem[e] == oj) return false;
-      }
-      return true;
+      return list_accept<SIMPLE>(j, i, qi, ti, bi, Q4, Rs4, body, perm, interact, exItem, e0, e1, ntypes, xRc2, lo, hi);
     };
     const int lane = threadIdx.x & 63, sub = lane & (S - 1), gbase = lane & ~(S - 1);
     for (int j0 = a; j0 < b; j0 += S) {
@@ -652,14 +651,69 @@ k_build_list(int* __restrict__ nbr, int* __restrict__ nnb, int* __restrict__ fla
       scan_range<SIMPLE, S>(a, b, i, ri.x, ri.y, ri.z, ti, bi, oi, Rs4, body, perm, interact, exItem, e0, e1, ntypes, xRc2,
                          cap, Npad, nbr, cnt, Q4, qi, lo, hi);
     };
-    // the cell starts of the next row are loaded before the current row is
-    // scanned, so their latency overlaps the scan
-    Ranges cur = row(0);
-    for (int r = 0; r < W * W; r++) {
-      Ranges nxt = r + 1 < W * W ? row(r + 1) : Ranges{ 0, 0, 0, 0 };
-      scan(cur.a0, cur.b0);
-      scan(cur.a1, cur.b1);
-      cur = nxt;
+    if constexpr (S == 16) {
+      // Streamed: the atom's 16 lanes form its rows 16 at a time (lane sub
+      // takes rows sub, sub + 16, ...) and keep the non-empty ranges, in scan
+      // order, in the group's slice of LDS.  Each step then takes the next 16
+      // candidates of the concatenated ranges, so no step ends half-used at a
+      // range's end and no lane repeats another's row arithmetic.  The
+      // candidates, their order and the ballot placement are those of the
+      // row-by-row scan, so the list is the same.
+      constexpr int NROW = W * W, SEGS = 2 * NROW;
+      __shared__ int2 segs[BLOCK / S][SEGS];
+      int2* sg = segs[threadIdx.x / S];
+      const int lane = threadIdx.x & 63, gbase = lane & ~(S - 1);
+      const unsigned lower = (1u << sub) - 1u, gmask = (1u << S) - 1u;
+      int nseg = 0;
+      for (int r0 = 0; r0 < NROW; r0 += S) {
+        int r = r0 + sub;
+        Ranges q = r < NROW ? row(r) : Ranges{ 0, 0, 0, 0 };
+        int l0 = q.b0 - q.a0, l1 = q.b1 - q.a1;
+        unsigned g1 = (unsigned)(__ballot(l0 > 0 || l1 > 0) >> gbase) & gmask;
+        unsigned g2 = (unsigned)(__ballot(l0 > 0 && l1 > 0) >> gbase) & gmask;
+        int k = nseg + __popc(g1 & lower) + __popc(g2 & lower);
+        if (l0 > 0) sg[k++] = make_int2(q.a0, l0);
+        if (l1 > 0) sg[k] = make_int2(q.a1, l1);
+        nseg += __popc(g1) + __popc(g2);
+      }
+      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+      __builtin_amdgcn_wave_barrier();
+      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+      // lane position: offset off inside range s, which starts at slot ca and
+      // holds cl candidates
+      int s = 0, off = sub, ca = 0, cl = 0;
+      if (nseg > 0) { int2 t = sg[0]; ca = t.x; cl = t.y; }
+      auto settle = [&]() {
+        while (s < nseg && off >= cl) {
+          off -= cl;
+          s++;
+          if (s < nseg) { int2 t = sg[s]; ca = t.x; cl = t.y; }
+        }
+      };
+      settle();
+      for (;;) {
+        bool has = s < nseg;
+        if (((unsigned)(__ballot(has) >> gbase) & gmask) == 0u) break;
+        int j = ca + off;
+        bool ok = has && list_accept<SIMPLE>(j, i, qi, ti, bi, Q4, Rs4, body, perm, interact, exItem, e0, e1, ntypes,
+                                             xRc2, lo, hi);
+        unsigned g = (unsigned)(__ballot(ok) >> gbase) & gmask;
+        int pos = cnt + __popc(g & lower);
+        if (ok && pos < cap) nbr[nb_at(pos, i, Npad)] = j;
+        cnt += __popc(g);
+        off += S;
+        settle();
+      }
+    } else {
+      // the cell starts of the next row are loaded before the current row is
+      // scanned, so their latency overlaps the scan
+      Ranges cur = row(0);
+      for (int r = 0; r < W * W; r++) {
+        Ranges nxt = r + 1 < W * W ? row(r + 1) : Ranges{ 0, 0, 0, 0 };
+        scan(cur.a0, cur.b0);
+        scan(cur.a1, cur.b1);
+        cur = nxt;
+      }
     }
     if (sub == 0) {
       nnb[i] = cnt;
