@@ -325,17 +325,18 @@ __global__ void k_cell_scatter(int* __restrict__ order, int* __restrict__ fill, 
   int c = key[s];
   order[start[c] + atomicAdd(&fill[c], 1)] = s;
 }
-// Deterministic order inside each cell: ascending original atom index.
-__global__ void k_cell_sort(int* __restrict__ order, const int* __restrict__ start, const int* __restrict__ perm,
-                            int ncells, int N) {
-  int c = blockIdx.x * blockDim.x + threadIdx.x;
-  if (c >= ncells) return;
-  int a = start[c], b = (c + 1 < ncells) ? start[c + 1] : N;
-  for (int x = a + 1; x < b; x++) {
-    int v = order[x], pv = perm[v], y = x - 1;
-    while (y >= a && perm[order[y]] > pv) { order[y + 1] = order[y]; y--; }
-    order[y + 1] = v;
-  }
+// Deterministic order inside each cell: ascending original atom index.  Each
+// entry of the scattered order counts the members of its cell with a smaller
+// original index, which is its place in the cell (original indices are
+// distinct); one thread per atom, so a few large cells do not serialise.
+__global__ void k_cell_rank(int* __restrict__ order, const int* __restrict__ scat, const int* __restrict__ key,
+                            const int* __restrict__ start, const int* __restrict__ perm, int N) {
+  int x = blockIdx.x * blockDim.x + threadIdx.x;
+  if (x >= N) return;
+  const int v = scat[x], c = key[v], a = start[c], b = start[c + 1], pv = perm[v];
+  int rank = 0;
+  for (int y = a; y < b; y++) rank += perm[scat[y]] < pv;
+  order[a + rank] = v;
 }
 // ncomp consecutive [Npad] components at once
 // Every per-slot array gathered into the new slot order in one pass: slot s
@@ -473,16 +474,14 @@ constexpr int CELL_REACH = 3;
 // S > 1: the S consecutive lanes of an atom take S consecutive candidates at
 // a time; a ballot over the group places the accepted ones, so the list
 // keeps the ascending order of the one-lane scan (for long lists: water).
-// Pair test of the multi-lane scans: the fixed-point prefilter with the FP64
-// test in the band, then (unless SIMPLE) the body, type and exclusion filters
-// for the candidates in range.
+// Pair test of the multi-lane scans, on the candidate's fixed-point position
+// qj and body bj loaded by the caller: the fixed-point prefilter with the FP64
+// test in the band, then (unless SIMPLE) the body, type and exclusion filters.
 template <bool SIMPLE>
-__device__ __forceinline__ bool list_accept(int j, int i, int4 qi, int ti, int bi, const int4* __restrict__ Q4,
-                                            const double4* __restrict__ Rs4, const int* __restrict__ body,
-                                            const int* __restrict__ perm, const uint8_t* __restrict__ interact,
-                                            const int* __restrict__ exItem, int e0, int e1, int ntypes, double xRc2,
-                                            float lo, float hi) {
-  int4 qj = Q4[j];
+__device__ __forceinline__ bool list_accept(int j, int4 qj, int bj, int i, int4 qi, int ti, int bi,
+                                            const double4* __restrict__ Rs4, const int* __restrict__ perm,
+                                            const uint8_t* __restrict__ interact, const int* __restrict__ exItem,
+                                            int e0, int e1, int ntypes, double xRc2, float lo, float hi) {
   float dx = (float)(int)((unsigned int)qi.x - (unsigned int)qj.x);
   float dy = (float)(int)((unsigned int)qi.y - (unsigned int)qj.y);
   float dz = (float)(int)((unsigned int)qi.z - (unsigned int)qj.z);
@@ -490,15 +489,15 @@ __device__ __forceinline__ bool list_accept(int j, int i, int4 qi, int ti, int b
   bool odd = ((qi.w | qj.w) & Q4_NONFINITE) != 0;
   bool in = r2 < lo && !odd;
   if (!in && (r2 < hi || odd)) in = exact_inside(Rs4, i, j, xRc2);
-  if (SIMPLE) return in && j != i;
-  if (!in || j == i) return false;
-  if (body[j] == bi) return false;
-  if (interact && !interact[(qj.w & Q4_TYPE) * ntypes + ti]) return false;   // null: all types interact
-  if (e1 > e0) {
+  bool ok = in & (j != i);
+  if (SIMPLE) return ok;
+  ok = ok & (bj != bi);
+  if (interact) ok = ok && interact[(qj.w & Q4_TYPE) * ntypes + ti];   // null: all types interact
+  if (ok && e1 > e0) {
     int oj = perm[j];
     for (int e = e0; e < e1; e++) if (exItem[e] == oj) return false;
   }
-  return true;
+  return ok;
 }
 template <bool SIMPLE, int S>
 __device__ __forceinline__ void scan_range(int a, int b, int i, double xi, double yi, double zi, int ti, int bi, int oi,
@@ -509,7 +508,8 @@ __device__ __forceinline__ void scan_range(int a, int b, int i, double xi, doubl
                                            const int4* __restrict__ Q4, int4 qi, float lo, float hi) {
   if (S > 1) {
     auto accept = [&](int j) -> bool {
-      return list_accept<SIMPLE>(j, i, qi, ti, bi, Q4, Rs4, body, perm, interact, exItem, e0, e1, ntypes, xRc2, lo, hi);
+      return list_accept<SIMPLE>(j, Q4[j], SIMPLE ? 0 : body[j], i, qi, ti, bi, Rs4, perm, interact, exItem, e0, e1,
+                                 ntypes, xRc2, lo, hi);
     };
     const int lane = threadIdx.x & 63, sub = lane & (S - 1), gbase = lane & ~(S - 1);
     for (int j0 = a; j0 < b; j0 += S) {
@@ -691,18 +691,27 @@ k_build_list(int* __restrict__ nbr, int* __restrict__ nnb, int* __restrict__ fla
         }
       };
       settle();
+      // one step ahead: the next step's position and body are loaded before
+      // this step is tested (a lane without a candidate loads its own atom's)
+      bool has = s < nseg;
+      int j = has ? ca + off : i;
+      int4 qj = Q4[j];
+      int bj = SIMPLE ? 0 : body[j];
       for (;;) {
-        bool has = s < nseg;
         if (((unsigned)(__ballot(has) >> gbase) & gmask) == 0u) break;
-        int j = ca + off;
-        bool ok = has && list_accept<SIMPLE>(j, i, qi, ti, bi, Q4, Rs4, body, perm, interact, exItem, e0, e1, ntypes,
+        off += S;
+        settle();
+        const bool hasN = s < nseg;
+        const int jN = hasN ? ca + off : i;
+        const int4 qN = Q4[jN];
+        const int bN = SIMPLE ? 0 : body[jN];
+        bool ok = has && list_accept<SIMPLE>(j, qj, bj, i, qi, ti, bi, Rs4, perm, interact, exItem, e0, e1, ntypes,
                                              xRc2, lo, hi);
         unsigned g = (unsigned)(__ballot(ok) >> gbase) & gmask;
         int pos = cnt + __popc(g & lower);
         if (ok && pos < cap) nbr[nb_at(pos, i, Npad)] = j;
         cnt += __popc(g);
-        off += S;
-        settle();
+        has = hasN; j = jN; qj = qN; bj = bN;
       }
     } else {
       // the cell starts of the next row are loaded before the current row is
@@ -2075,6 +2084,13 @@ void dev_rebuild(Device* d, double L, double xRc, double xInRc) {
   if (N == 0) return;
   d->posWrites++;                      // R and R0 in a new slot order
   int g = div_up(N, BLOCK);
+  // lanes per atom of the list build: as many as keep at most 2^21 lanes in
+  // all and at least four candidates per lane (the scanned region is ~2.6
+  // times the list volume)
+  double vol = 4.18879020478639 * xRc * xRc * xRc * ((double)N / (L * L * L));   // expected list length
+  const bool simple = d->bodies.nb == 0 && d->nexcl == 0 && d->allInteract;
+  const double cand = 2.6 * vol;
+  const int S = (cand >= 16 * 4 && (long)N * 16 <= (1L << 21)) ? 16 : ((cand >= 4 * 4 && (long)N * 4 <= (1L << 21)) ? 4 : 1);
   int M = (int)std::floor(CELL_REACH * L / xRc);
   if (M < 2 * CELL_REACH + 1) M = 2 * CELL_REACH + 1;   // every cell of a row at most once
   if ((long)M * M * M > 64l * 1024 * 1024) M = 400;   // bound the cell table
@@ -2088,8 +2104,11 @@ void dev_rebuild(Device* d, double L, double xRc, double xInRc) {
   hipLaunchKernelGGL(k_scan_sums, dim3(1), dim3(SCAN_T), 0, d->stream, d->blockSums.p, gc, (int*)nullptr);
   hipLaunchKernelGGL(k_scan_add, dim3(gc), dim3(BLOCK), 0, d->stream, d->cellStart.p, d->blockSums.p, nc1);
   HIP_CHECK(hipMemsetAsync(d->cellCount.p, 0, sizeof(int) * nc1, d->stream));
-  hipLaunchKernelGGL(k_cell_scatter, dim3(g), dim3(BLOCK), 0, d->stream, d->order.p, d->cellCount.p, d->key.p, d->cellStart.p, N);
-  hipLaunchKernelGGL(k_cell_sort, dim3(div_up(nc, BLOCK)), dim3(BLOCK), 0, d->stream, d->order.p, d->cellStart.p, d->perm.p, nc, N);
+  // scattered (in-cell order of the atomics) into tmpI, which is free until
+  // the permutation below, then ranked into order
+  hipLaunchKernelGGL(k_cell_scatter, dim3(g), dim3(BLOCK), 0, d->stream, d->tmpI.p, d->cellCount.p, d->key.p, d->cellStart.p, N);
+  hipLaunchKernelGGL(k_cell_rank, dim3(g), dim3(BLOCK), 0, d->stream, d->order.p, d->tmpI.p, d->key.p, d->cellStart.p,
+                     d->perm.p, N);
   launch_check("cell sort");
   // every per-slot array into the new order (written to scratch, swapped),
   // R0 = R, and the inverse permutation
@@ -2128,7 +2147,6 @@ void dev_rebuild(Device* d, double L, double xRc, double xInRc) {
   double xRc2 = (xRc * xRc) / (L * L);
   const double two32 = 4294967296.0, rc2u = xRc2 * two32 * two32, band = band_rel(xRc2);
   float lo = (float)(rc2u * (1.0 - band)), hi = (float)(rc2u * (1.0 + band));
-  double vol = 4.18879020478639 * xRc * xRc * xRc * ((double)N / (L * L * L));   // expected list length
   if (d->cap == 0) {
     d->cap = std::max(32, (int)(1.35 * vol + 16));
     d->cap = (d->cap + 7) & ~7;
@@ -2136,13 +2154,6 @@ void dev_rebuild(Device* d, double L, double xRc, double xInRc) {
   for (;;) {
     d->nbr.alloc((size_t)d->cap * Np);
     HIP_CHECK(hipMemsetAsync(d->flags.p, 0, sizeof(int) * 4, d->stream));
-    bool simple = d->bodies.nb == 0 && d->nexcl == 0 && d->allInteract;
-    // lanes per atom: as many as keep at most 2^21 lanes in all and at least
-    // four candidates per lane (the scanned region is ~2.6 times the list
-    // volume)
-    double cand = 2.6 * vol;
-    long n = N;
-    int S = (cand >= 16 * 4 && n * 16 <= (1L << 21)) ? 16 : ((cand >= 4 * 4 && n * 4 <= (1L << 21)) ? 4 : 1);
     int gl = div_up((long)N * S, BLOCK);
     hipEvent_t e0 = d->profiling ? d->ev0 : nullptr, e1 = d->profiling ? d->ev1 : nullptr;
     // the segment-streaming form addresses list entries with 32-bit element
