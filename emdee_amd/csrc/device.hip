@@ -664,11 +664,12 @@ k_build_list(int* __restrict__ nbr, int* __restrict__ nnb, int* __restrict__ fla
       int2* sg = segs[threadIdx.x / S];
       const int lane = threadIdx.x & 63, gbase = lane & ~(S - 1);
       const unsigned lower = (1u << sub) - 1u, gmask = (1u << S) - 1u;
-      int nseg = 0;
+      int nseg = 0, len = 0;
       for (int r0 = 0; r0 < NROW; r0 += S) {
         int r = r0 + sub;
         Ranges q = r < NROW ? row(r) : Ranges{ 0, 0, 0, 0 };
         int l0 = q.b0 - q.a0, l1 = q.b1 - q.a1;
+        len += l0 + l1;
         unsigned g1 = (unsigned)(__ballot(l0 > 0 || l1 > 0) >> gbase) & gmask;
         unsigned g2 = (unsigned)(__ballot(l0 > 0 && l1 > 0) >> gbase) & gmask;
         int k = nseg + __popc(g1 & lower) + __popc(g2 & lower);
@@ -679,32 +680,38 @@ k_build_list(int* __restrict__ nbr, int* __restrict__ nnb, int* __restrict__ fla
       __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
       __builtin_amdgcn_wave_barrier();
       __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+      // candidates of the atom, hence its number of steps
+      for (int o = S / 2; o > 0; o >>= 1) len += __shfl_xor(len, o, S);
+      const int steps = (len + S - 1) / S;
       // lane position: offset off inside range s, which starts at slot ca and
-      // holds cl candidates
-      int s = 0, off = sub, ca = 0, cl = 0;
+      // holds cl candidates; past the last range cl is INT_MAX
+      int s = 0, off = sub, ca = 0, cl = INT_MAX;
       if (nseg > 0) { int2 t = sg[0]; ca = t.x; cl = t.y; }
       auto settle = [&]() {
-        while (s < nseg && off >= cl) {
+        while (off >= cl) {
           off -= cl;
           s++;
-          if (s < nseg) { int2 t = sg[s]; ca = t.x; cl = t.y; }
+          const int2 t = s < nseg ? sg[s] : make_int2(0, INT_MAX);
+          ca = t.x; cl = t.y;
         }
       };
       settle();
       // one step ahead: the next step's position and body are loaded before
       // this step is tested (a lane without a candidate loads its own atom's)
+      // (32-bit offsets from the array bases: slots are below 2^21 here)
+      auto q4At = [&](int k) { return *(const int4*)((const char*)Q4 + (unsigned)k * 16u); };
+      auto bodyAt = [&](int k) { return *(const int*)((const char*)body + (unsigned)k * 4u); };
       bool has = s < nseg;
       int j = has ? ca + off : i;
-      int4 qj = Q4[j];
-      int bj = SIMPLE ? 0 : body[j];
-      for (;;) {
-        if (((unsigned)(__ballot(has) >> gbase) & gmask) == 0u) break;
+      int4 qj = q4At(j);
+      int bj = SIMPLE ? 0 : bodyAt(j);
+      for (int step = 0; step < steps; step++) {
         off += S;
         settle();
         const bool hasN = s < nseg;
         const int jN = hasN ? ca + off : i;
-        const int4 qN = Q4[jN];
-        const int bN = SIMPLE ? 0 : body[jN];
+        const int4 qN = q4At(jN);
+        const int bN = SIMPLE ? 0 : bodyAt(jN);
         bool ok = has && list_accept<SIMPLE>(j, qj, bj, i, qi, ti, bi, Rs4, perm, interact, exItem, e0, e1, ntypes,
                                              xRc2, lo, hi);
         unsigned g = (unsigned)(__ballot(ok) >> gbase) & gmask;
