@@ -23,10 +23,12 @@ pytestmark = pytest.mark.gpu
 THREADS = min(16, os.cpu_count() or 1)
 
 
-def make(lib, R, L, model, gpu_mixed, threads=1):
+def make(lib, R, L, model, gpu_mixed, threads=1, exclusions=()):
     N = len(R)
     md = lib.system(threads, 1, 2.5, 0.3, N, None, None, None)
     lib.EmDee_set_pair_model(md, 1, 1, model(lib), 0.0)
+    for i, j in exclusions:
+        lib.EmDee_ignore_pair(md, i, j)
     if gpu_mixed:
         lib.EmDeeGPU_set_precision(md, 1)
     lib.upload(md, "box", np.array([L]))
@@ -105,23 +107,26 @@ def cluster_box(nc, seed):
     return R, L
 
 
-@pytest.mark.parametrize("nc,mixed", [(10, False), (10, True), (64, False), (64, True)])
-def test_list_capacity_overflow(gpu, oracle, nc, mixed):
+@pytest.mark.parametrize("nc,mixed,excl", [(10, False, False), (10, True, False), (10, False, True), (10, True, True),
+                                           (64, False, False), (64, True, False)])
+def test_list_capacity_overflow(gpu, oracle, nc, mixed, excl):
     # 4,000 atoms take the 16-lane build, 1,048,576 the one-lane segment
     # build: both must drop the entries past the capacity, report the
-    # longest list and be rebuilt with room for it
+    # longest list and be rebuilt with room for it.  With exclusions (two
+    # pairs inside the cluster) the 16-lane build takes its filtered form.
     R, L = cluster_box(nc, 29)
     N = len(R)
     model = lambda lib: lib.EmDee_pair_lj_cut(1.0, 1.0)
-    mg = make(gpu, R, L, model, mixed)
-    mo = make(oracle, R, L, model, False, THREADS)
+    ex = [(1, 2), (5, 60)] if excl else []
+    mg = make(gpu, R, L, model, mixed, exclusions=ex)
+    mo = make(oracle, R, L, model, False, THREADS, exclusions=ex)
     gpu.compute_forces(mg)
     oracle.compute_forces(mo)
     kg, ko = pair_keys(gpu.neighbor_pairs(mg), N), pair_keys(oracle_pair_array(oracle, mo), N)
-    print(f"cluster N={N} mixed={mixed}: {len(kg)} pairs on the GPU, {len(ko)} in the oracle", flush=True)
+    print(f"cluster N={N} mixed={mixed} excl={excl}: {len(kg)} pairs on the GPU, {len(ko)} in the oracle", flush=True)
     assert len(kg) == len(ko) and np.array_equal(kg, ko)
     ferr, eerr, _ = errors(gpu, oracle, mg, mo, N)
-    print(f"cluster N={N} mixed={mixed}: force rms {ferr:.2e} energy {eerr:.2e}", flush=True)
+    print(f"cluster N={N} mixed={mixed} excl={excl}: force rms {ferr:.2e} energy {eerr:.2e}", flush=True)
     # The cluster's 0.3 sigma pairs dominate the energy.  The mixed form's
     # positions are 32-bit fixed point (L / 2^32 = 2.5e-8 sigma at 1M atoms),
     # a relative error of 8e-8 at 0.3 sigma, which r^-12 raises about twelve
