@@ -2251,6 +2251,9 @@ void dev_compute_pairs(Device* d, const PairCall& c, ForceSums* out) {
     tpa = want >= 16 ? 16 : (want >= 4 ? 4 : 1);
   }
   int gp = div_up((long)N * tpa, BLOCK);
+  // the pair kernels write four sums per block into partials, which holds
+  // 64 (Npad / BLOCK + 1) values: enough for up to 16 lanes per atom
+  if ((size_t)gp * 4 > d->partials.n) fatal("compute_forces", "too many lanes per atom for the partial sums");
   double* ea = d->wantAtomE ? d->eatom.p : nullptr;
   int variant = (c.energy ? 4 : 0) + (U ? 2 : 0) + (LJ ? 1 : 0);
   if (c.mixed) {
