@@ -1206,12 +1206,12 @@ k_pairs(double* __restrict__ F, double* __restrict__ partials,
       }
     } else {
       int k = sub;
-      for (; k + TPA < n; k += 2 * TPA) {
-        int j0 = nbr[nb_at(k, i, Npad)], j1 = nbr[nb_at(k + TPA, i, Npad)];
-        double4 r0 = Rs4[j0], r1 = Rs4[j1];
-        one(j0, r0); one(j1, r1);
+      for (; k + 2 * TPA < n; k += 3 * TPA) {
+        int j0 = nbr[nb_at(k, i, Npad)], j1 = nbr[nb_at(k + TPA, i, Npad)], j2 = nbr[nb_at(k + 2 * TPA, i, Npad)];
+        double4 r0 = Rs4[j0], r1 = Rs4[j1], r2 = Rs4[j2];
+        one(j0, r0); one(j1, r1); one(j2, r2);
       }
-      if (k < n) { int j = nbr[nb_at(k, i, Npad)]; one(j, Rs4[j]); }
+      for (; k < n; k += TPA) { int j = nbr[nb_at(k, i, Npad)]; one(j, Rs4[j]); }
     }
   }
   if (TPA > 1) {
