@@ -326,16 +326,16 @@ k_hmc(const ReplicaDesc* __restrict__ desc, const int* __restrict__ order, doubl
         }
       };
       if (useList) {
-        // four indices loaded before their pairs are evaluated, so that
+        // eight indices loaded before their pairs are evaluated, so that
         // the loads overlap
         const unsigned short* __restrict__ li = nlist + (size_t)d.off * cap + i;
         const int n = nnb[d.off + i];
-        for (int k = 0; k < n; k += 4) {
-          int jb[4];
+        for (int k = 0; k < n; k += 8) {
+          int jb[8];
 #pragma unroll
-          for (int u = 0; u < 4; u++) jb[u] = k + u < n ? (int)li[(size_t)(k + u) * N] : -1;
+          for (int u = 0; u < 8; u++) jb[u] = k + u < n ? (int)li[(size_t)(k + u) * N] : -1;
 #pragma unroll
-          for (int u = 0; u < 4; u++)
+          for (int u = 0; u < 8; u++)
             if (jb[u] >= 0) pair(jb[u]);
         }
       } else {
