@@ -327,16 +327,17 @@ k_hmc(const ReplicaDesc* __restrict__ desc, const int* __restrict__ order, doubl
       };
       if (useList) {
         // eight indices loaded before their pairs are evaluated, so that
-        // the loads overlap
+        // the loads overlap; past the list end the atom itself stands in,
+        // which the pair body skips (FP64) or masks (mixed), so the eight
+        // are evaluated without a branch
         const unsigned short* __restrict__ li = nlist + (size_t)d.off * cap + i;
         const int n = nnb[d.off + i];
         for (int k = 0; k < n; k += 8) {
           int jb[8];
 #pragma unroll
-          for (int u = 0; u < 8; u++) jb[u] = k + u < n ? (int)li[(size_t)(k + u) * N] : -1;
+          for (int u = 0; u < 8; u++) jb[u] = k + u < n ? (int)li[(size_t)(k + u) * N] : i;
 #pragma unroll
-          for (int u = 0; u < 8; u++)
-            if (jb[u] >= 0) pair(jb[u]);
+          for (int u = 0; u < 8; u++) pair(jb[u]);
         }
       } else {
         // a list that would overflow its capacity (more neighbours within
