@@ -1157,7 +1157,9 @@ k_pairs(double* __restrict__ F, double* __restrict__ partials,
     DevPair pu;
     if (UNIFORM) pu = pairs[0];   // uniform address: scalar loads
     const int n = nnb[i];
-    auto one = [&](int j, const double4& rj) {
+    // qjp: the neighbour's charge when the caller loaded it with the position
+    // (LJ == 2 on the multi-lane path), else read here
+    auto one = [&](int j, const double4& rj, double qjp = 0.0, bool haveQ = false) {
       double dx = ri.x - rj.x, dy = ri.y - rj.y, dz = ri.z - rj.z;
       dx = dx - rint(dx); dy = dy - rint(dy); dz = dz - rint(dz);
       double r2;
@@ -1173,7 +1175,7 @@ k_pairs(double* __restrict__ F, double* __restrict__ partials,
           double4 c;
           double kc;
           if (lds) { c = sljt[t]; kc = sljk[t]; } else { c = ljt[t]; kc = ljk[t]; }
-          double qj = charge[j];
+          double qj = haveQ ? qjp : charge[j];
           bool jc = ich && kc != 0.0 && fabs(qj) > Q_EPS;
           double qiqj = jc ? kc * qi * qj : 0.0;
           pair_body_ljc<ENERGY>(c, cm, jc, qiqj, dx, dy, dz, r2, invL2, a);
@@ -1209,7 +1211,12 @@ k_pairs(double* __restrict__ F, double* __restrict__ partials,
       for (; k + 2 * TPA < n; k += 3 * TPA) {
         int j0 = nbr[nb_at(k, i, Npad)], j1 = nbr[nb_at(k + TPA, i, Npad)], j2 = nbr[nb_at(k + 2 * TPA, i, Npad)];
         double4 r0 = Rs4[j0], r1 = Rs4[j1], r2 = Rs4[j2];
-        one(j0, r0); one(j1, r1); one(j2, r2);
+        if (LJ == 2) {
+          const double q0 = charge[j0], q1 = charge[j1], q2 = charge[j2];
+          one(j0, r0, q0, true); one(j1, r1, q1, true); one(j2, r2, q2, true);
+        } else {
+          one(j0, r0); one(j1, r1); one(j2, r2);
+        }
       }
       for (; k < n; k += TPA) { int j = nbr[nb_at(k, i, Npad)]; one(j, Rs4[j]); }
     }
